@@ -1,0 +1,125 @@
+"""ctypes declarations of the C ABI in include/atu.h and the loader of libaturan_amd.so.
+
+The library is loaded from this package directory only; a missing library raises at import of
+the binding (there is no CPU fallback on the product path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libaturan_amd.so"
+
+# ---- field ids (atu_field) ----
+FIELD_NAMES = [
+    "t", "u", "v", "w",
+    "ch4", "ch4_cloud", "ch4_ice", "h2o", "h2o_cloud", "h2o_ice",
+    "h2s", "h2s_cloud", "h2s_ice", "nh3", "nh3_cloud", "nh3_ice",
+    "nh4sh", "tke", "dis",
+    "nue", "rho_mix", "Q_precip",
+    "S_precip_ch4", "S_precip_ch4_cloud", "S_precip_ch4_ice",
+    "S_precip_h2o", "S_precip_h2o_cloud", "S_precip_h2o_ice",
+    "S_precip_h2s", "S_precip_h2s_cloud", "S_precip_h2s_ice",
+    "S_precip_nh3", "S_precip_nh3_cloud", "S_precip_nh3_ice",
+    "massflux_h2s", "massflux_nh3", "massflux_nh4sh",
+    "fluxlim_nh4sh", "tke_source", "dis_source",
+    "p_stat", "p_dyn", "p_hydro", "aux_u", "aux_v", "aux_w",
+]
+FIELD = {n: i for i, n in enumerate(FIELD_NAMES)}
+NPROG = 19
+NFROZEN = 21
+NFIELDS = 46
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+SOLVE_REFERENCE = 0
+SOLVE_CONVERGE = 1
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "non-finite value",
+          -5: "no convergence", -6: "unsupported"}
+
+
+class Grid(C.Structure):
+    _fields_ = [("im", C.c_int32), ("jm", C.c_int32), ("km", C.c_int32), ("pad_", C.c_int32),
+                ("r0", C.c_double), ("dr", C.c_double), ("the0", C.c_double), ("dthe", C.c_double),
+                ("phi0", C.c_double), ("dphi", C.c_double)]
+
+
+_PARAM_DOUBLES = [
+    "dt", "re", "pr", "g", "omega", "u_0", "t_ref", "L_atm",
+    "Coriolis", "centrifugal", "buoyancy", "chemical_reaction",
+    "sc_ch4", "sc_h2o", "sc_h2s", "sc_nh3", "sc_nh4sh",
+    "alpha_sponge", "rho_cond_nh4sh", "r_p_nh4sh",
+    "r_mix", "R_mix", "cp_mix", "mue_mix",
+    "r_h2", "r_he", "r_ch4", "cp_h2", "cp_he", "cp_ch4",
+    "R_metric_km", "sinthe_min", "t_min_K", "t_max_K", "tke_max_m2s2", "dis_min_nd",
+    "turb_coupling", "Pr_t", "sig_k", "sig_w", "precip_coupling", "p_dyn_cap", "coriolis_scale",
+]
+_PARAM_INTS = ["cp_mode", "cp_gas", "turb_active", "lid_vw", "rigid_lid", "radius_positive",
+               "species_clamp", "pad_"]
+
+
+class Params(C.Structure):
+    _fields_ = [(n, C.c_double) for n in _PARAM_DOUBLES] + [(n, C.c_int32) for n in _PARAM_INTS]
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("sweeps", C.c_int32), ("max_sweeps", C.c_int32), ("pad_", C.c_int32),
+                ("omega", C.c_double), ("rel_tol", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("sweeps_done", C.c_int32), ("outer_iters", C.c_int32), ("rel_resid", C.c_double),
+                ("src_norm", C.c_double)]
+
+
+_DP = C.POINTER(C.c_double)
+_SIGS = {
+    "atu_grid_default": (C.c_int, [C.c_int, C.POINTER(Grid)]),
+    "atu_params_default": (C.c_int, [C.POINTER(Grid), C.POINTER(Params)]),
+    "atu_cp_table": (C.c_int, [C.c_int, _DP, C.POINTER(C.c_int)]),
+    "atu_cp_lapse": (C.c_double, [C.POINTER(Params), C.c_double]),
+    "atu_synthetic_state": (C.c_int, [C.POINTER(Grid), C.POINTER(Params), C.c_uint64, _DP]),
+    "atu_create": (C.c_int, [C.POINTER(Grid), C.POINTER(Params), C.c_int, C.POINTER(C.c_void_p)]),
+    "atu_destroy": (None, [C.c_void_p]),
+    "atu_last_error": (C.c_char_p, [C.c_void_p]),
+    "atu_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "atu_synchronize": (C.c_int, [C.c_void_p]),
+    "atu_set_field": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "atu_get_field": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "atu_hydro_prep": (C.c_int, [C.c_void_p, _DP]),
+    "atu_rk4": (C.c_int, [C.c_void_p]),
+    "atu_apply_bc": (C.c_int, [C.c_void_p]),
+    "atu_pressure_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
+    "atu_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
+    "atu_pressure_residual": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "atu_field_device_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
+    "atu_build_info": (C.c_char_p, []),
+    "atu_profile_begin": (C.c_int, [C.c_void_p]),
+    "atu_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.POINTER(C.c_int)]),
+    "atu_profile_end": (C.c_int, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libaturan_amd.so from this package; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python aturan_amd/build.py` "
+                          "(the HIP library is required; there is no fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)

@@ -1,0 +1,180 @@
+"""Python host mirror of the reference's hot-path interface over the C ABI (include/atu.h).
+
+The reference exposes this path as parameterless member functions of its model object
+(planet/cUranusModel.h:1226-1227: RungeKuttaUran, RHSUran; BC_Uran::bcRadius/bcTheta/bcPhi,
+planet/BC_Uran.h:29-41; PressureSolver<cUranusModel>::run, planet/PressureSolver.h:145). `UranusCore`
+keeps those names and their order of effects; the arrays it hands out are NumPy copies of the
+device fields in the reference's [i][j][k] order (u radial, v meridional +southward, w zonal).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+class AtuError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+
+def _check(rc: int, ctx: C.c_void_p | None, what: str) -> None:
+    if rc != 0:
+        lib = L.load()
+        msg = lib.atu_last_error(ctx if ctx else None)
+        raise AtuError(f"{what}: {L.STATUS.get(rc, rc)} ({msg.decode() if msg else ''})")
+
+
+def default_grid(refine: int = 1) -> L.Grid:
+    g = L.Grid()
+    _check(L.load().atu_grid_default(refine, C.byref(g)), None, "atu_grid_default")
+    return g
+
+
+def default_params(grid: L.Grid) -> L.Params:
+    p = L.Params()
+    _check(L.load().atu_params_default(C.byref(grid), C.byref(p)), None, "atu_params_default")
+    return p
+
+
+def synthetic_state(grid: L.Grid, params: L.Params, seed: int = 20261203) -> np.ndarray:
+    """All 46 fields of the seeded synthetic state, shape (46, im, jm, km), FP64 (BC not applied)."""
+    out = np.zeros((L.NFIELDS, grid.im, grid.jm, grid.km), dtype=np.float64)
+    _check(L.load().atu_synthetic_state(C.byref(grid), C.byref(params), C.c_uint64(seed),
+                                        out.ctypes.data_as(L._DP)), None, "atu_synthetic_state")
+    return out
+
+
+def cp_table(mode: int = 2) -> np.ndarray:
+    lib = L.load()
+    n = C.c_int(0)
+    _check(lib.atu_cp_table(mode, None, C.byref(n)), None, "atu_cp_table")
+    out = np.zeros(n.value)
+    _check(lib.atu_cp_table(mode, out.ctypes.data_as(L._DP), C.byref(n)), None, "atu_cp_table")
+    return out
+
+
+@dataclass
+class SolveResult:
+    sweeps_done: int
+    outer_iters: int
+    rel_resid: float
+    src_norm: float
+
+
+def solver_opts(mode: str = "reference", sweeps: int = 1, omega: float = 1.0, rel_tol: float = 1e-8,
+                max_sweeps: int = 200) -> L.SolverOpts:
+    o = L.SolverOpts()
+    o.mode = L.SOLVE_REFERENCE if mode == "reference" else L.SOLVE_CONVERGE
+    o.sweeps = sweeps
+    o.max_sweeps = max_sweeps
+    o.omega = omega
+    o.rel_tol = rel_tol
+    return o
+
+
+class UranusCore:
+    """One device context: the model-owned fields and the step pieces of the reference."""
+
+    def __init__(self, grid: L.Grid | None = None, params: L.Params | None = None, precision: str = "fp64"):
+        self._lib = L.load()
+        self.grid = grid if grid is not None else default_grid(1)
+        self.params = params if params is not None else default_params(self.grid)
+        prec = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}[precision]
+        ctx = C.c_void_p()
+        _check(self._lib.atu_create(C.byref(self.grid), C.byref(self.params), prec, C.byref(ctx)), None,
+               "atu_create")
+        self._ctx = ctx
+        self.shape = (self.grid.im, self.grid.jm, self.grid.km)
+
+    # ---- lifetime ----
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.atu_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- fields ----
+    def set_field(self, name: str | int, arr: np.ndarray) -> None:
+        f = L.FIELD[name] if isinstance(name, str) else name
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"field shape {a.shape} != grid {self.shape}")
+        _check(self._lib.atu_set_field(self._ctx, f, a.ctypes.data_as(L._DP)), self._ctx, "atu_set_field")
+
+    def get_field(self, name: str | int) -> np.ndarray:
+        f = L.FIELD[name] if isinstance(name, str) else name
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(self._lib.atu_get_field(self._ctx, f, out.ctypes.data_as(L._DP)), self._ctx, "atu_get_field")
+        return out
+
+    def set_state(self, fields: np.ndarray) -> None:
+        for f in range(L.NFIELDS):
+            self.set_field(f, fields[f])
+
+    def get_state(self) -> np.ndarray:
+        return np.stack([self.get_field(f) for f in range(L.NFIELDS)])
+
+    # ---- the step pieces, named as in the reference ----
+    def computeHydrostaticPressure(self) -> np.ndarray:
+        """computeBuoyancyRefLevel + computeHydrostaticPressure; returns buoy_ref_level."""
+        lev = np.zeros(self.grid.im)
+        _check(self._lib.atu_hydro_prep(self._ctx, lev.ctypes.data_as(L._DP)), self._ctx, "atu_hydro_prep")
+        return lev
+
+    def RungeKuttaUran(self) -> None:
+        _check(self._lib.atu_rk4(self._ctx), self._ctx, "atu_rk4")
+
+    def boundary_conditions(self) -> None:
+        """BC_Uran(m).bcRadius(); bcTheta(); bcPhi() on the prognostic fields."""
+        _check(self._lib.atu_apply_bc(self._ctx), self._ctx, "atu_apply_bc")
+
+    def PressureSolver(self, mode: str = "reference", sweeps: int = 1, omega: float = 1.0,
+                       rel_tol: float = 1e-8, max_sweeps: int = 200) -> SolveResult:
+        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps)
+        st = L.SolveStats()
+        _check(self._lib.atu_pressure_solve(self._ctx, C.byref(o), C.byref(st)), self._ctx, "atu_pressure_solve")
+        return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)
+
+    def step(self, nsteps: int = 1, mode: str = "reference", sweeps: int = 1, omega: float = 1.0,
+             rel_tol: float = 1e-8, max_sweeps: int = 200) -> SolveResult:
+        """The benchmark's outer step: hydro prep, RK4, BC, Poisson solve; nsteps times."""
+        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps)
+        st = L.SolveStats()
+        _check(self._lib.atu_step(self._ctx, nsteps, C.byref(o), C.byref(st)), self._ctx, "atu_step")
+        return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)
+
+    def pressure_residual(self) -> tuple[float, float]:
+        r, s = C.c_double(), C.c_double()
+        _check(self._lib.atu_pressure_residual(self._ctx, C.byref(r), C.byref(s)), self._ctx,
+               "atu_pressure_residual")
+        return r.value, s.value
+
+    def synchronize(self) -> None:
+        _check(self._lib.atu_synchronize(self._ctx), self._ctx, "atu_synchronize")
+
+    # ---- per-kernel timing ----
+    def profile_begin(self) -> None:
+        _check(self._lib.atu_profile_begin(self._ctx), self._ctx, "atu_profile_begin")
+
+    def profile_query(self, cls: str) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_int()
+        _check(self._lib.atu_profile_query(self._ctx, cls.encode(), C.byref(ms), C.byref(n)), self._ctx,
+               "atu_profile_query")
+        return ms.value, n.value
+
+    def profile_end(self) -> None:
+        _check(self._lib.atu_profile_end(self._ctx), self._ctx, "atu_profile_end")

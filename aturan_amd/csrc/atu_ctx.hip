@@ -1,0 +1,536 @@
+// The C ABI context: device buffers, field transfer and the step orchestration. It plays the role
+// of the model-owned arrays and the hot-path member functions of cUranusModel
+// (planet/cUranusModel.h:1226-1227, planet/cUranusModel.cpp:952-1395).
+
+#include "atu_kernels.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace atu;
+
+namespace {
+
+std::string g_create_error;
+
+struct ProfEvent {
+    int cls;
+    hipEvent_t a, b;
+};
+
+const char* kClassNames[] = {"rk_stage", "rb_pass", "hydro", "bc", "proj_src", "residual", "line", "baro"};
+enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_N };
+
+}  // namespace
+
+struct atu_ctx {
+    atu_grid grid{};
+    atu_params params{};
+    int precision = ATU_PREC_FP64;
+    Shape sh{};
+    StepConsts consts{};
+    Geometry hgeo;
+    DevGeo geo{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+
+    // device buffers (FP64 path)
+    double* Y = nullptr;      // 19 prognostic, start-of-step state y_n (also y_{n+1} after a step)
+    double* SA = nullptr;     // 19, stage input buffer A
+    double* SB = nullptr;     // 19, stage input buffer B
+    double* ACC = nullptr;    // 19, RK accumulator
+    double* FROZEN = nullptr; // 21
+    double* PSTAT = nullptr;
+    double* PDYN = nullptr;
+    double* PHYDRO = nullptr;
+    double* AUX = nullptr;    // 3
+    double* SRC = nullptr;
+    double* RES = nullptr;
+    double* LEVEL = nullptr;  // im
+    double* RPART = nullptr;  // residual partials
+    double* RSUM = nullptr;   // 2
+    void* geo_blob = nullptr;
+
+    // profiling
+    bool prof = false;
+    std::vector<ProfEvent> events;
+    std::vector<hipEvent_t> pool;
+
+    ~atu_ctx();
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    int hip(hipError_t e, const char* what) {
+        if (e == hipSuccess) return ATU_OK;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return ATU_EHIP;
+    }
+    hipEvent_t take_event() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        hipEventCreate(&e);
+        return e;
+    }
+    // Bracket a launch sequence with events on the stream it is launched on, when profiling is on.
+    template <typename F>
+    hipError_t timed(int cls, F&& launch) {
+        return timed_on(cls, stream, static_cast<F&&>(launch));
+    }
+    template <typename F>
+    hipError_t timed_on(int cls, hipStream_t s, F&& launch) {
+        if (!prof) return launch();
+        ProfEvent pe{cls, take_event(), take_event()};
+        hipEventRecord(pe.a, s);
+        const hipError_t e = launch();
+        hipEventRecord(pe.b, s);
+        events.push_back(pe);
+        return e;
+    }
+    // side stream for the hydrostatic prep of the next step, which overlaps the Poisson solve
+    hipStream_t side = nullptr;
+    hipEvent_t ev_bc = nullptr, ev_hydro = nullptr;
+    double* prog(double* buf, int f) const { return buf + (size_t)f * sh.fs; }
+};
+
+atu_ctx::~atu_ctx() {
+    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM})
+        if (p) hipFree(p);
+    if (ev_bc) hipEventDestroy(ev_bc);
+    if (ev_hydro) hipEventDestroy(ev_hydro);
+    if (side) hipStreamDestroy(side);
+    if (geo_blob) hipFree(geo_blob);
+    for (auto& e : events) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    for (auto e : pool) hipEventDestroy(e);
+    if (own_stream && stream) hipStreamDestroy(stream);
+}
+
+namespace {
+
+int upload_geometry(atu_ctx* c) {
+    const Geometry& h = c->hgeo;
+    std::vector<const std::vector<double>*> parts = {
+        &h.rm, &h.inv_rm, &h.inv_rm2, &h.two_inv_rm, &h.sponge, &h.num2,
+        &h.costhe, &h.cotanthe, &h.sinthe_bf, &h.cor_hydro, &h.sin_weight,
+        &h.inv_rmsinthe, &h.inv_rm2sinthe, &h.inv_rm2sinthe2, &h.costhe_inv_rm2sinthe,
+        &h.v_metric, &h.num3, &h.den, &h.inv_rmsinthe_s};
+    const std::vector<double>& cp = cp_h2_table(c->params.cp_mode == 1 ? 1 : 2);
+    size_t total = cp.size();
+    for (auto* v : parts) total += v->size();
+    std::vector<double> blob;
+    blob.reserve(total);
+    std::vector<size_t> off;
+    for (auto* v : parts) {
+        off.push_back(blob.size());
+        blob.insert(blob.end(), v->begin(), v->end());
+    }
+    const size_t cp_off = blob.size();
+    blob.insert(blob.end(), cp.begin(), cp.end());
+    if (c->hip(hipMalloc(&c->geo_blob, blob.size() * sizeof(double)), "hipMalloc(geometry)")) return ATU_EHIP;
+    if (c->hip(hipMemcpy(c->geo_blob, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice),
+               "hipMemcpy(geometry)"))
+        return ATU_EHIP;
+    const double* d = (const double*)c->geo_blob;
+    DevGeo& g = c->geo;
+    const double** dst[] = {&g.rm, &g.inv_rm, &g.inv_rm2, &g.two_inv_rm, &g.sponge, &g.num2,
+                            &g.costhe, &g.cotanthe, &g.sinthe_bf, &g.cor_hydro, &g.sin_weight,
+                            &g.inv_rmsinthe, &g.inv_rm2sinthe, &g.inv_rm2sinthe2, &g.costhe_inv_rm2sinthe,
+                            &g.v_metric, &g.num3, &g.den, &g.inv_rmsinthe_s};
+    for (size_t q = 0; q < parts.size(); q++) *dst[q] = d + off[q];
+    g.cp_tab = d + cp_off;
+    g.cp_n = (int)cp.size();
+    g.inv_2dr = h.inv_2dr;
+    g.inv_2dthe = h.inv_2dthe;
+    g.inv_2dphi = h.inv_2dphi;
+    g.inv_dr2 = h.inv_dr2;
+    g.inv_dthe2 = h.inv_dthe2;
+    g.inv_dphi2 = h.inv_dphi2;
+    g.num1 = h.num1;
+    return ATU_OK;
+}
+
+int alloc_fields(atu_ctx* c, double** p, int nf) {
+    const size_t bytes = (size_t)nf * c->sh.fs * sizeof(double);
+    if (c->hip(hipMalloc((void**)p, bytes), "hipMalloc(fields)")) return ATU_ENOMEM;
+    if (c->hip(hipMemset(*p, 0, bytes), "hipMemset(fields)")) return ATU_EHIP;
+    return ATU_OK;
+}
+
+// Copy a dense [im][jm][km] host array into a padded device field (and back).
+int put_field(atu_ctx* c, double* dst, const double* host) {
+    const Shape& s = c->sh;
+    return c->hip(hipMemcpy2DAsync(dst, (size_t)s.ldk * sizeof(double), host, (size_t)s.km * sizeof(double),
+                                   (size_t)s.km * sizeof(double), (size_t)s.im * s.jm, hipMemcpyHostToDevice,
+                                   c->stream),
+                  "hipMemcpy2DAsync(H2D)");
+}
+int get_field(atu_ctx* c, const double* src, double* host) {
+    const Shape& s = c->sh;
+    return c->hip(hipMemcpy2DAsync(host, (size_t)s.km * sizeof(double), src, (size_t)s.ldk * sizeof(double),
+                                   (size_t)s.km * sizeof(double), (size_t)s.im * s.jm, hipMemcpyDeviceToHost,
+                                   c->stream),
+                  "hipMemcpy2DAsync(D2H)");
+}
+
+double* field_ptr(atu_ctx* c, int f) {
+    if (f >= 0 && f < ATU_NPROG) return c->prog(c->Y, f);
+    if (f >= ATU_NUE && f < ATU_NUE + ATU_NFROZEN) return c->FROZEN + (size_t)(f - ATU_NUE) * c->sh.fs;
+    switch (f) {
+        case ATU_P_STAT: return c->PSTAT;
+        case ATU_P_DYN: return c->PDYN;
+        case ATU_P_HYDRO: return c->PHYDRO;
+        case ATU_AUX_U: return c->AUX;
+        case ATU_AUX_V: return c->AUX + c->sh.fs;
+        case ATU_AUX_W: return c->AUX + 2 * c->sh.fs;
+        default: return nullptr;
+    }
+}
+
+int check_finite_grid(const atu_grid* g) {
+    if (!g) return 0;
+    if (g->im < 6 || g->jm < 6 || g->km < 6) return 0;
+    if (((g->km - 1) & 1) != 0) return 0;   // red-black colouring across the phi seam needs km-1 even
+    if ((size_t)g->im * g->jm * g->km > (size_t)1 << 31) return 0;
+    return 1;
+}
+
+StageArgs<double> stage_args(atu_ctx* c, int stage) {
+    StageArgs<double> a{};
+    a.sh = c->sh;
+    a.geo = c->geo;
+    a.c = c->consts;
+    a.yn = c->Y;
+    a.acc = c->ACC;
+    a.frozen = c->FROZEN;
+    a.pdyn = c->PDYN;
+    a.phydro = c->PHYDRO;
+    a.aux = c->AUX;
+    a.ymirror = nullptr;
+    switch (stage) {
+        case 0: a.xs = c->Y; a.xnext = c->SA; break;
+        case 1: a.xs = c->SA; a.xnext = c->SB; break;
+        case 2: a.xs = c->SB; a.xnext = c->SA; break;
+        default: a.xs = c->SA; a.xnext = nullptr; a.yout = c->Y; break;
+    }
+    return a;
+}
+
+int do_rk4(atu_ctx* c) {
+    for (int s = 0; s < 4; s++) {
+        const StageArgs<double> a = stage_args(c, s);
+        if (c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")) return ATU_EHIP;
+    }
+    return ATU_OK;
+}
+
+int do_hydro(atu_ctx* c, hipStream_t s) {
+    return c->hip(c->timed_on(PC_HYDRO, s,
+                              [&] {
+                                  return launch_hydro(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T),
+                                                      c->prog(c->Y, ATU_V), c->prog(c->Y, ATU_W), c->PSTAT, c->LEVEL,
+                                                      c->PHYDRO, s);
+                              }),
+                  "hydro");
+}
+
+int do_bc(atu_ctx* c) {
+    BcTargets t{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr};
+    return c->hip(c->timed(PC_BC, [&] { return launch_bc(c->sh, c->params, t, c->stream); }), "bc");
+}
+
+int do_residual(atu_ctx* c, double* rel, double* srcn) {
+    int e = c->hip(c->timed(PC_RES,
+                            [&] {
+                                return launch_residual<double>(c->sh, c->geo, c->PDYN, c->SRC, nullptr, c->RPART,
+                                                               c->RSUM, c->stream);
+                            }),
+                   "residual");
+    if (e) return e;
+    double h[2];
+    if (c->hip(hipMemcpyAsync(h, c->RSUM, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpy(residual)"))
+        return ATU_EHIP;
+    if (c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return ATU_EHIP;
+    const double rn = std::sqrt(h[0]), sn = std::sqrt(h[1]);
+    if (srcn) *srcn = sn;
+    if (rel) *rel = sn > 0.0 ? rn / sn : rn;
+    return ATU_OK;
+}
+
+int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
+    atu_solver_opts def{ATU_SOLVE_REFERENCE, 1, 0, 0, 1.0, 0.0};
+    if (!o) o = &def;
+    if (c->hip(c->timed(PC_PROJ,
+                        [&] { return launch_proj_src<double>(c->sh, c->geo, c->consts, c->AUX, c->SRC, c->stream); }),
+               "proj_src"))
+        return ATU_EHIP;
+    if (o->mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EUNSUPPORTED, "solve mode not available in this build");
+    const int sweeps = o->sweeps > 0 ? o->sweeps : 1;   // PressureSolver.h:174-175
+    for (int s = 0; s < sweeps; s++) {
+        for (int colour = 0; colour < 2; colour++) {
+            if (c->hip(c->timed(PC_RB,
+                                [&] {
+                                    return launch_rb_pass<double>(c->sh, c->geo, c->consts, c->PDYN, c->SRC, colour,
+                                                                  o->omega, c->stream);
+                                }),
+                       "rb_pass"))
+                return ATU_EHIP;
+        }
+    }
+    if (c->hip(c->timed(PC_RB, [&] { return launch_p_corners<double>(c->sh, c->consts, c->PDYN, c->stream); }),
+               "p_corners"))
+        return ATU_EHIP;
+    if (st) {
+        st->sweeps_done = sweeps;
+        st->outer_iters = 1;
+        int e = do_residual(c, &st->rel_resid, &st->src_norm);
+        if (e) return e;
+    }
+    return ATU_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** out) {
+    if (!out) return ATU_EINVAL;
+    *out = nullptr;
+    if (!g || !p || !check_finite_grid(g)) {
+        g_create_error = "atu_create: invalid grid (need im,jm,km >= 6 and km-1 even)";
+        return ATU_EINVAL;
+    }
+    if (precision != ATU_PREC_FP64) {
+        g_create_error = "atu_create: precision not available in this build";
+        return ATU_EUNSUPPORTED;
+    }
+    std::unique_ptr<atu_ctx> c(new (std::nothrow) atu_ctx());
+    if (!c) return ATU_ENOMEM;
+    c->grid = *g;
+    c->params = *p;
+    c->precision = precision;
+    c->sh.im = g->im;
+    c->sh.jm = g->jm;
+    c->sh.km = g->km;
+    c->sh.ldk = padded_ldk(g->km);
+    c->sh.plane = (size_t)g->jm * c->sh.ldk;
+    c->sh.fs = (size_t)g->im * c->sh.plane;
+    build_geometry(*g, *p, c->hgeo);
+    build_step_consts(*g, *p, c->consts);
+    int e;
+    if ((e = c->hip(hipStreamCreate(&c->stream), "hipStreamCreate"))) {
+        g_create_error = c->err;
+        return e;
+    }
+    c->own_stream = true;
+    if ((e = c->hip(hipStreamCreate(&c->side), "hipStreamCreate(side)")) ||
+        (e = c->hip(hipEventCreateWithFlags(&c->ev_bc, hipEventDisableTiming), "hipEventCreate")) ||
+        (e = c->hip(hipEventCreateWithFlags(&c->ev_hydro, hipEventDisableTiming), "hipEventCreate"))) {
+        g_create_error = c->err;
+        return e;
+    }
+    if ((e = upload_geometry(c.get())) || (e = alloc_fields(c.get(), &c->Y, ATU_NPROG)) ||
+        (e = alloc_fields(c.get(), &c->SA, ATU_NPROG)) || (e = alloc_fields(c.get(), &c->SB, ATU_NPROG)) ||
+        (e = alloc_fields(c.get(), &c->ACC, ATU_NPROG)) || (e = alloc_fields(c.get(), &c->FROZEN, ATU_NFROZEN)) ||
+        (e = alloc_fields(c.get(), &c->PSTAT, 1)) || (e = alloc_fields(c.get(), &c->PDYN, 1)) ||
+        (e = alloc_fields(c.get(), &c->PHYDRO, 1)) || (e = alloc_fields(c.get(), &c->AUX, 3)) ||
+        (e = alloc_fields(c.get(), &c->SRC, 1))) {
+        g_create_error = c->err;
+        return e;
+    }
+    const size_t nres = (size_t)residual_partials_count(c->sh) * 2;
+    if ((e = c->hip(hipMalloc((void**)&c->LEVEL, (size_t)g->im * sizeof(double)), "hipMalloc")) ||
+        (e = c->hip(hipMalloc((void**)&c->RPART, nres * sizeof(double)), "hipMalloc")) ||
+        (e = c->hip(hipMalloc((void**)&c->RSUM, 2 * sizeof(double)), "hipMalloc"))) {
+        g_create_error = c->err;
+        return e;
+    }
+    if ((e = c->hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) {
+        g_create_error = c->err;
+        return e;
+    }
+    *out = c.release();
+    return ATU_OK;
+}
+
+void atu_destroy(atu_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->side) hipStreamSynchronize(ctx->side);
+    delete ctx;
+}
+
+const char* atu_last_error(const atu_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+static int sync_all(atu_ctx* ctx) {
+    if (ctx->hip(hipStreamSynchronize(ctx->side), "hipStreamSynchronize(side)")) return ATU_EHIP;
+    return ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int atu_set_stream(atu_ctx* ctx, void* s) {
+    if (!ctx) return ATU_EINVAL;
+    if (sync_all(ctx)) return ATU_EHIP;
+    if (s == nullptr) {
+        if (!ctx->own_stream) {
+            if (ctx->hip(hipStreamCreate(&ctx->stream), "hipStreamCreate")) return ATU_EHIP;
+            ctx->own_stream = true;
+        }
+        return ATU_OK;
+    }
+    if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+    ctx->stream = (hipStream_t)s;
+    ctx->own_stream = false;
+    return ATU_OK;
+}
+
+int atu_synchronize(atu_ctx* ctx) {
+    if (!ctx) return ATU_EINVAL;
+    return sync_all(ctx);
+}
+
+int atu_set_field(atu_ctx* ctx, int f, const double* host) {
+    if (!ctx || !host) return ATU_EINVAL;
+    double* d = field_ptr(ctx, f);
+    if (!d) return ctx->fail(ATU_EINVAL, "atu_set_field: unknown field id");
+    int e = put_field(ctx, d, host);
+    if (!e && f < ATU_NPROG) {
+        // restoreVar's role: every stage buffer starts from the same state, boundary planes included
+        if (!e) e = put_field(ctx, ctx->prog(ctx->SA, f), host);
+        if (!e) e = put_field(ctx, ctx->prog(ctx->SB, f), host);
+    }
+    if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return e;
+}
+
+int atu_get_field(atu_ctx* ctx, int f, double* host) {
+    if (!ctx || !host) return ATU_EINVAL;
+    const double* d = field_ptr(ctx, f);
+    if (!d) return ctx->fail(ATU_EINVAL, "atu_get_field: unknown field id");
+    int e = get_field(ctx, d, host);
+    if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return e;
+}
+
+int atu_hydro_prep(atu_ctx* ctx, double* level_means) {
+    if (!ctx) return ATU_EINVAL;
+    int e = do_hydro(ctx, ctx->stream);
+    if (e) return e;
+    if (level_means) {
+        if (ctx->hip(hipMemcpyAsync(level_means, ctx->LEVEL, ctx->sh.im * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream),
+                     "hipMemcpy(level)"))
+            return ATU_EHIP;
+        return ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    }
+    return ATU_OK;
+}
+
+int atu_rk4(atu_ctx* ctx) { return ctx ? do_rk4(ctx) : ATU_EINVAL; }
+int atu_apply_bc(atu_ctx* ctx) { return ctx ? do_bc(ctx) : ATU_EINVAL; }
+
+int atu_pressure_solve(atu_ctx* ctx, const atu_solver_opts* opts, atu_solve_stats* stats) {
+    if (!ctx) return ATU_EINVAL;
+    if (opts && !(opts->omega > 0.0 && opts->omega < 2.0))
+        return ctx->fail(ATU_EINVAL, "atu_pressure_solve: omega must lie in (0, 2)");
+    return do_pressure(ctx, opts, stats);
+}
+
+int atu_pressure_residual(atu_ctx* ctx, double* rel, double* srcn) {
+    if (!ctx) return ATU_EINVAL;
+    return do_residual(ctx, rel, srcn);
+}
+
+int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_stats* stats) {
+    if (!ctx || nsteps < 0) return ATU_EINVAL;
+    if (opts && !(opts->omega > 0.0 && opts->omega < 2.0))
+        return ctx->fail(ATU_EINVAL, "atu_step: omega must lie in (0, 2)");
+    // The hydrostatic prep of step n+1 reads only t, v, w after the BC of step n, and the Poisson
+    // solve of step n reads only aux and p_dyn: they run concurrently, the prep on the side stream.
+    for (int n = 0; n < nsteps; n++) {
+        int e;
+        if (n == 0) {
+            if ((e = do_hydro(ctx, ctx->stream))) return e;
+        } else if ((e = ctx->hip(hipStreamWaitEvent(ctx->stream, ctx->ev_hydro, 0), "hipStreamWaitEvent"))) {
+            return e;
+        }
+        if ((e = do_rk4(ctx)) || (e = do_bc(ctx))) return e;
+        if (n + 1 < nsteps) {
+            if ((e = ctx->hip(hipEventRecord(ctx->ev_bc, ctx->stream), "hipEventRecord")) ||
+                (e = ctx->hip(hipStreamWaitEvent(ctx->side, ctx->ev_bc, 0), "hipStreamWaitEvent")) ||
+                (e = do_hydro(ctx, ctx->side)) ||
+                (e = ctx->hip(hipEventRecord(ctx->ev_hydro, ctx->side), "hipEventRecord")))
+                return e;
+        }
+        if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;
+    }
+    return ATU_OK;
+}
+
+int atu_field_device_ptr(atu_ctx* ctx, int f, void** dptr, int* ld_k, int* elem_bytes) {
+    if (!ctx || !dptr) return ATU_EINVAL;
+    double* d = field_ptr(ctx, f);
+    if (!d) return ctx->fail(ATU_EINVAL, "atu_field_device_ptr: unknown field id");
+    *dptr = d;
+    if (ld_k) *ld_k = ctx->sh.ldk;
+    if (elem_bytes) *elem_bytes = 8;
+    return ATU_OK;
+}
+
+const char* atu_build_info(void) {
+    return "libaturan_amd abi " "1" " gfx950 hipcc -O3 -ffp-contract=off";
+}
+
+int atu_profile_begin(atu_ctx* ctx) {
+    if (!ctx) return ATU_EINVAL;
+    if (sync_all(ctx)) return ATU_EHIP;
+    for (auto& e : ctx->events) {
+        ctx->pool.push_back(e.a);
+        ctx->pool.push_back(e.b);
+    }
+    ctx->events.clear();
+    ctx->prof = true;
+    return ATU_OK;
+}
+
+int atu_profile_query(atu_ctx* ctx, const char* cls, double* total_ms, int* launches) {
+    if (!ctx || !cls) return ATU_EINVAL;
+    int id = -1;
+    for (int q = 0; q < PC_N; q++)
+        if (std::strcmp(cls, kClassNames[q]) == 0) id = q;
+    if (id < 0) return ctx->fail(ATU_EINVAL, "atu_profile_query: unknown kernel class");
+    if (sync_all(ctx)) return ATU_EHIP;
+    double tot = 0.0;
+    int n = 0;
+    for (auto& e : ctx->events) {
+        if (e.cls != id) continue;
+        float ms = 0.f;
+        if (ctx->hip(hipEventElapsedTime(&ms, e.a, e.b), "hipEventElapsedTime")) return ATU_EHIP;
+        tot += ms;
+        n++;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = n;
+    return ATU_OK;
+}
+
+int atu_profile_end(atu_ctx* ctx) {
+    if (!ctx) return ATU_EINVAL;
+    ctx->prof = false;
+    return ATU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,459 @@
+// Host side of libaturan_amd: grid and parameter defaults, the cp(T) table of H2, the geometry
+// tables uploaded to the device, and the seeded synthetic initial state.
+//
+// Every formula that reaches the step is evaluated here in FP64 with the operand order of the
+// reference expression it restates (cited per function), so the device reads the same bits the
+// reference's CPU code forms inline.
+
+#include "atu_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace atu {
+
+// ---------------------------------------------------------------------------------------------
+// cp of H2 (planet/CpH2.h:55-93): rotational level sum with theta_r = 87.57 K plus a vibrational
+// term, tabulated at 0.5 K from 5 to 1500 K and linearly interpolated.
+// ---------------------------------------------------------------------------------------------
+namespace {
+constexpr double kRgas = 8.314462618, kMH2 = 2.01588e-3, kThR = 87.57, kThV = 6332.0;
+constexpr double kTlo = 5.0, kThi = 1500.0, kDT = 0.5;
+
+void level_sums(double T, int J0, double wgt, double& Z, double& E1, double& E2) {
+    for (int J = J0; J < 60; J += 2) {
+        const double e = kThR * J * (J + 1.0);
+        const double b = wgt * (2.0 * J + 1.0) * std::exp(-e / T);
+        Z += b;
+        E1 += b * e;
+        E2 += b * e * e;
+    }
+}
+
+double rot_heat(double T, double Z, double E1, double E2) {
+    const double m1 = E1 / Z, m2 = E2 / Z;
+    return (m2 - m1 * m1) / (T * T);
+}
+
+// cp/R of H2; mode 1 = normal (ortho:para frozen 3:1), 2 = equilibrium (planet/CpH2.h:55-75).
+double cp_over_R(double T, int mode) {
+    double c;
+    if (mode == 1) {
+        double Zp = 0, Ep = 0, Fp = 0, Zo = 0, Eo = 0, Fo = 0;
+        level_sums(T, 0, 1.0, Zp, Ep, Fp);
+        level_sums(T, 1, 1.0, Zo, Eo, Fo);
+        c = 0.25 * rot_heat(T, Zp, Ep, Fp) + 0.75 * rot_heat(T, Zo, Eo, Fo);
+    } else {
+        double Z = 0, E1 = 0, E2 = 0;
+        level_sums(T, 0, 1.0, Z, E1, E2);
+        level_sums(T, 1, 3.0, Z, E1, E2);
+        c = rot_heat(T, Z, E1, E2);
+    }
+    const double x = kThV / T;
+    const double cv = (x < 700.0) ? x * x * std::exp(x) / ((std::exp(x) - 1.0) * (std::exp(x) - 1.0)) : 0.0;
+    return 2.5 + c + cv;
+}
+
+std::vector<double> make_table(int mode) {
+    const int n = int((kThi - kTlo) / kDT) + 1;
+    std::vector<double> t(n);
+    for (int i = 0; i < n; i++) t[i] = cp_over_R(kTlo + i * kDT, mode) * kRgas / kMH2;
+    return t;
+}
+}  // namespace
+
+const std::vector<double>& cp_h2_table(int mode) {
+    static const std::vector<double> t1 = make_table(1);
+    static const std::vector<double> t2 = make_table(2);
+    return mode == 1 ? t1 : t2;
+}
+
+static double cp_h2_at(double T, int mode, double cp_const) {
+    if (mode <= 0) return cp_const;
+    const std::vector<double>& t = cp_h2_table(mode);
+    const int n = (int)t.size();
+    const double x = (std::min(std::max(T, kTlo), kThi) - kTlo) / kDT;
+    const int i = std::min(n - 2, (int)x);
+    const double f = x - i;
+    return t[i] + f * (t[i + 1] - t[i]);
+}
+
+// CpH2::Lapse::cp with the model's dyn_on (planet/CpH2.h:128-134).
+double cp_lapse(const atu_params& p, double T) {
+    if (p.cp_mode == 0 && !p.cp_gas) return p.cp_mix;
+    const double ch = cp_h2_at(T, p.cp_mode, p.cp_h2);
+    if (p.cp_gas) return (p.r_h2 * ch + p.r_he * p.cp_he + p.r_ch4 * p.cp_ch4) / (p.r_h2 + p.r_he + p.r_ch4);
+    return p.cp_mix + (p.r_h2 / p.r_mix) * (ch - p.cp_h2);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Geometry tables.
+// ---------------------------------------------------------------------------------------------
+void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo) {
+    const int im = g.im, jm = g.jm, km = g.km;
+    geo.im = im; geo.jm = jm; geo.km = km;
+    geo.radz.assign(im, 0.0);
+    geo.thez.assign(jm, 0.0);
+    // Coordinates by cumulative addition, lib/Array_1D.cpp:50-61.
+    geo.radz[0] = g.r0;
+    for (int l = 1; l < im; l++) geo.radz[l] = geo.radz[l - 1] + g.dr;
+    geo.thez[0] = g.the0;
+    for (int l = 1; l < jm; l++) geo.thez[l] = geo.thez[l - 1] + g.dthe;
+
+    geo.inv_2dr = 1.0 / (2.0 * g.dr);
+    geo.inv_2dthe = 1.0 / (2.0 * g.dthe);
+    geo.inv_2dphi = 1.0 / (2.0 * g.dphi);
+    geo.inv_dr2 = 1.0 / (g.dr * g.dr);
+    geo.inv_dthe2 = 1.0 / (g.dthe * g.dthe);
+    geo.inv_dphi2 = 1.0 / (g.dphi * g.dphi);
+    // num1 = exp_2_rm * inv_dr2 with exp_rm = 1 (no coordinate stretching), PressureSolver.h:350
+    const double exp_rm = 1.0, exp_2_rm = exp_rm * exp_rm;
+    geo.num1 = exp_2_rm * geo.inv_dr2;
+
+    // per i: metricRadius (planet/cUranusModel.h:641-647), the sponge (RHS_Uran_Turb.cpp:594-595)
+    geo.rm.assign(im, 0); geo.inv_rm.assign(im, 0); geo.inv_rm2.assign(im, 0);
+    geo.two_inv_rm.assign(im, 0); geo.sponge.assign(im, 0); geo.num2.assign(im, 0);
+    const double shift = (p.R_metric_km > 0.0) ? (p.R_metric_km / p.L_atm - 1.0) : 0.0;
+    for (int i = 0; i < im; i++) {
+        const double rm = (p.R_metric_km > 0.0) ? geo.radz[i] + shift : geo.radz[i];
+        const double rm2 = rm * rm;
+        geo.rm[i] = rm;
+        geo.inv_rm[i] = 1.0 / rm;
+        geo.inv_rm2[i] = 1.0 / rm2;
+        geo.two_inv_rm[i] = 2.0 * geo.inv_rm[i];
+        const double frac = std::max(0.0, (double)(i - (im - 1) * 3 / 4) / (double)((im - 1) / 4));
+        geo.sponge[i] = p.alpha_sponge * frac * frac;
+        geo.num2[i] = geo.inv_rm2[i] * geo.inv_dthe2;
+    }
+
+    // per j: RK tables (RungeKutta_Uran_Turb.cpp:192-197), the body-force true sine
+    // (RHS_Uran_Turb.cpp:224), the hydrostatic Coriolis factor (RungeKutta_Uran_Turb.cpp:132-133)
+    // and the level-mean area weight (RungeKutta_Uran_Turb.cpp:25).
+    geo.sinthe.assign(jm, 0); geo.sinthe2.assign(jm, 0); geo.costhe.assign(jm, 0);
+    geo.cotanthe.assign(jm, 0); geo.sinthe_bf.assign(jm, 0); geo.cor_hydro.assign(jm, 0);
+    geo.sin_weight.assign(jm, 0);
+    std::vector<double> sin_solver(jm);
+    for (int j = 0; j < jm; j++) {
+        const double s = std::max(p.sinthe_min, std::abs(std::sin(geo.thez[j])));
+        const double c = std::cos(geo.thez[j]);
+        geo.sinthe[j] = s;
+        geo.sinthe2[j] = s * s;
+        geo.costhe[j] = c;
+        geo.cotanthe[j] = c / s;
+        geo.sinthe_bf[j] = std::sqrt(std::max(0.0, 1.0 - c * c));
+        geo.cor_hydro[j] = p.Coriolis * p.coriolis_scale * 2.0 * p.omega * std::abs(std::sin(geo.thez[j]));
+        geo.sin_weight[j] = std::sin(geo.thez[j]);
+        // the solver's own table, planet/PressureSolver.h:156-162
+        double ss = std::sin(geo.thez[j]);
+        if (ss < p.sinthe_min) ss = p.sinthe_min;
+        if (ss == 0.0) ss = 1.0e-5;
+        sin_solver[j] = ss;
+    }
+
+    const size_t nij = (size_t)im * jm;
+    geo.inv_rmsinthe.assign(nij, 0); geo.inv_rm2sinthe.assign(nij, 0);
+    geo.inv_rm2sinthe2.assign(nij, 0); geo.costhe_inv_rm2sinthe.assign(nij, 0);
+    geo.v_metric.assign(nij, 0); geo.num3.assign(nij, 0); geo.den.assign(nij, 0);
+    geo.inv_rmsinthe_s.assign(nij, 0);
+    for (int i = 0; i < im; i++) {
+        for (int j = 0; j < jm; j++) {
+            const size_t q = (size_t)i * jm + j;
+            const double rm = geo.rm[i];
+            const double s = geo.sinthe[j];
+            const double inv_rm2 = geo.inv_rm2[i];
+            geo.inv_rmsinthe[q] = 1.0 / (rm * s);
+            geo.inv_rm2sinthe[q] = inv_rm2 / s;
+            geo.inv_rm2sinthe2[q] = inv_rm2 / geo.sinthe2[j];
+            geo.costhe_inv_rm2sinthe[q] = geo.costhe[j] * geo.inv_rm2sinthe[q];
+            geo.v_metric[q] = (1.0 + geo.costhe[j] / geo.sinthe2[j]) * inv_rm2;
+            // solver weights (PressureSolver.h:328-353) from its own sine table
+            const double ss = sin_solver[j];
+            const double ss2 = ss * ss;
+            const double n3 = (inv_rm2 / ss2) * geo.inv_dphi2;
+            geo.num3[q] = n3;
+            geo.den[q] = 2.0 * geo.num1 + 2.0 * geo.num2[i] + 2.0 * n3;
+            geo.inv_rmsinthe_s[q] = 1.0 / (rm * ss);
+        }
+    }
+}
+
+void build_step_consts(const atu_grid& g, const atu_params& p, StepConsts& c) {
+    c.dt = p.dt;
+    c.c_half = 0.5 * p.dt;
+    c.c_full = p.dt;
+    c.dt6 = 6.0;
+    c.coeff_energy_p = p.u_0 * p.u_0 / (p.cp_mix * p.t_ref);
+    c.scale_p = 1.0e5 / (p.r_mix * p.u_0 * p.u_0);
+    c.scale_Cor = p.coriolis_scale;
+    c.re_pr = p.re * p.pr;
+    c.re = p.re;
+    c.inv_re = 1.0 / p.re;
+    c.inv_sig_k = p.sig_k;   // divisor, kept as a division on the device
+    c.inv_sig_w = p.sig_w;
+    c.sc_re_ch4 = p.sc_ch4 * p.re;
+    c.sc_re_h2o = p.sc_h2o * p.re;
+    c.sc_re_h2s = p.sc_h2s * p.re;
+    c.sc_re_nh3 = p.sc_nh3 * p.re;
+    c.sc_re_nh4sh = p.sc_nh4sh * p.re;
+    c.precip_num = p.L_atm * 1.0e3;
+    c.precip_den = p.r_mix * p.cp_mix * p.u_0 * p.t_ref;
+    c.precip_coupling = p.precip_coupling;
+    c.chemical_reaction = p.chemical_reaction;
+    c.stokes_c = (2.0 / 9.0) * p.r_p_nh4sh * p.r_p_nh4sh;
+    c.rho_cond_nh4sh = p.rho_cond_nh4sh;
+    c.g = p.g;
+    c.mue_mix = p.mue_mix;
+    c.u_0 = p.u_0;
+    c.turb_coupling = p.turb_coupling;
+    c.Pr_t = p.Pr_t;
+    c.two_omega = +2.0 * p.omega;
+    c.m_two_omega = -2.0 * p.omega;
+    c.Coriolis = p.Coriolis;
+    c.t_min = p.t_min_K / p.t_ref;
+    c.t_max = p.t_max_K / p.t_ref;
+    c.tke_max = p.tke_max_m2s2 / (p.u_0 * p.u_0);
+    c.dis_min = p.dis_min_nd;
+    c.cp_mix = p.cp_mix;
+    c.t_ref = p.t_ref;
+    c.cp_a = p.r_he * p.cp_he;                          // cp_lapse numerator terms, kept apart
+    c.cp_b = p.r_ch4 * p.cp_ch4;                        // so the sum keeps its operand order
+    c.cp_den = p.r_h2 + p.r_he + p.r_ch4;
+    c.r_h2 = p.r_h2;
+    c.cp_mode = p.cp_mode;
+    c.cp_gas = p.cp_gas;
+    c.turb_active = p.turb_active;
+    c.hyd_bnum = p.r_mix * p.R_mix;
+    c.nd_rel = p.g * (p.L_atm * 1.0e3) / (p.u_0 * p.u_0);
+    c.dr = g.dr;
+    c.p_dyn_cap = p.p_dyn_cap;
+    c.c43 = 4.0 / 3.0;
+    c.c13 = 1.0 / 3.0;
+}
+
+}  // namespace atu
+
+// =============================================================================================
+// C ABI: defaults, cp table, synthetic state
+// =============================================================================================
+using namespace atu;
+
+extern "C" int atu_grid_default(int refine, atu_grid* out) {
+    if (!out || (refine != 1 && refine != 2)) return ATU_EINVAL;
+    std::memset(out, 0, sizeof(*out));
+    // planet/cUranusModel.cpp:339-351: pi180 = 180/pi, dthe = the_degree / pi180
+    const double pi180 = 180.0 / M_PI;
+    if (refine == 1) {
+        out->im = 41; out->jm = 181; out->km = 361;
+        out->dr = 0.025;
+        out->dthe = 1.0 / pi180;
+        out->dphi = 1.0 / pi180;
+    } else {
+        out->im = 81; out->jm = 361; out->km = 721;
+        out->dr = 0.0125;
+        out->dthe = 0.5 / pi180;
+        out->dphi = 0.5 / pi180;
+    }
+    out->r0 = 1.0;
+    out->the0 = 0.0;
+    out->phi0 = 0.0;
+    return ATU_OK;
+}
+
+extern "C" int atu_params_default(const atu_grid* g, atu_params* p) {
+    if (!g || !p) return ATU_EINVAL;
+    std::memset(p, 0, sizeof(*p));
+    p->dt = 0.001;                                      // planet/cUranusModel.cpp:475
+    p->re = 1000.0; p->pr = 0.69; p->g = 8.69; p->omega = 1.01e-4;   // param.py:59-88
+    p->u_0 = 100.0; p->t_ref = 76.4; p->L_atm = 225.0;
+    p->Coriolis = 1.0; p->centrifugal = 1.0; p->buoyancy = 1.0; p->chemical_reaction = 1.0;
+    p->sc_ch4 = 0.99; p->sc_h2o = 0.61; p->sc_h2s = 0.94; p->sc_nh3 = 0.61; p->sc_nh4sh = 0.7;  // cUranusModel.h:1120-1124
+    p->alpha_sponge = 10.0;                             // cUranusModel.h:1127
+    p->rho_cond_nh4sh = 1170.0; p->r_p_nh4sh = 1.0e-5;  // cUranusModel.h:987,1022
+
+    // Reference densities and species properties (planet/cUranusModel.h:1004-1089), then the
+    // mass-weighted mixture of ThermalPropertiesUran (planet/ChemistryUran.h:437-490).
+    double r_h2 = 0.864, r_he = 0.136, r_ch4 = 0.19, r_nh3 = 0.1, r_h2s = 0.3, r_h2o = 0.08, r_nh4sh = 0.007;
+    const double R_h2 = 4124.2, R_he = 2077.1, R_nh3 = 488.21, R_nh4sh = 261.0, R_h2s = 243.96,
+                 R_h2o = 461.52, R_ch4 = 518.28;
+    const double mue_h2 = 0.84e-5, mue_he = 1.87e-5, mue_nh3 = 0.92e-5, mue_nh4sh = 0.99e-5,
+                 mue_h2s = 1.3e-5, mue_h2o = 0.9e-5, mue_ch4 = 1.107e-5;
+    const double cp_h2 = 14.32e3, cp_he = 5.19e3, cp_nh3 = 2.19e3, cp_nh4sh = 2.00e3, cp_h2s = 2.24e3,
+                 cp_h2o = 1.93e3, cp_ch4 = 2.232e3;
+    double r_mix = r_h2 + r_he + r_h2s + r_nh3 + r_nh4sh + r_h2o + r_ch4;
+    const double cp_mix = (r_h2 * cp_h2 + r_h2s * cp_h2s + r_he * cp_he + r_nh3 * cp_nh3
+                           + r_nh4sh * cp_nh4sh + r_h2o * cp_h2o + r_ch4 * cp_ch4) / r_mix;
+    const double mue_mix = (r_h2 * mue_h2 + r_h2s * mue_h2s + r_he * mue_he + r_nh3 * mue_nh3
+                            + r_nh4sh * mue_nh4sh + r_h2o * mue_h2o + r_ch4 * mue_ch4) / r_mix;
+    const double R_mix = (r_h2 * R_h2 + r_he * R_he + r_h2o * R_h2o + r_h2s * R_h2s + r_nh3 * R_nh3
+                          + r_nh4sh * R_nh4sh + r_ch4 * R_ch4)
+                         / (r_h2 + r_he + r_h2o + r_h2s + r_nh3 + r_nh4sh + r_ch4);
+    p->cp_h2 = cp_h2; p->cp_he = cp_he; p->cp_ch4 = cp_ch4;
+    p->cp_mix = cp_mix; p->mue_mix = mue_mix; p->R_mix = R_mix;
+    p->cp_mode = 2; p->cp_gas = 1;                      // CP_H2 2, CP_GAS 1 (cUranusModel.h:596-597)
+    p->r_h2 = r_h2; p->r_he = r_he; p->r_ch4 = r_ch4; p->r_mix = r_mix;
+
+    // T_INIT_ANCHOR (cUranusModel.cpp:504-511): the bottom temperature of the cp(T) adiabat
+    // through 76 K at 1 bar (cUranusModel.h:149-156), then one factor on every density.
+    const double R_ref = 3.61491, p_bottom = 17.3, t_1bar = 76.0;
+    {
+        const double Rk = R_ref * 1.0e3;
+        double T = t_1bar;
+        const int n = 2000;
+        const double h = std::log(p_bottom) / n;
+        for (int s = 0; s < n; s++) {
+            const double k1 = T * Rk / cp_lapse(*p, T), Tm = T + 0.5 * h * k1;
+            T += h * Tm * Rk / cp_lapse(*p, Tm);
+        }
+        const double f = p_bottom * 1.0e5 / (R_mix * T) / r_mix;
+        r_h2 *= f; r_he *= f; r_ch4 *= f; r_nh3 *= f; r_h2s *= f; r_h2o *= f; r_nh4sh *= f;
+        r_mix *= f;
+    }
+    p->r_h2 = r_h2; p->r_he = r_he; p->r_ch4 = r_ch4; p->r_mix = r_mix;
+
+    p->R_metric_km = 25362.0;                           // cUranusModel.h:638
+    p->sinthe_min = 0.4;
+    p->t_min_K = 7.5; p->t_max_K = 2000.0;
+    p->tke_max_m2s2 = 1000.0; p->dis_min_nd = 1.0e-10;
+    p->turb_coupling = 1.0; p->Pr_t = 0.9; p->sig_k = 0.85; p->sig_w = 0.5;
+    p->precip_coupling = 1.0;
+    p->p_dyn_cap = 0.2;
+    p->coriolis_scale = p->L_atm * 1.0e3 / p->u_0;      // CORIOLIS_FULL, cUranusModel.h:1498
+    p->turb_active = 1;                                 // turb_model k_omega_SST (cUranusModel.cpp:482)
+    p->lid_vw = 2;                                      // BC_LID_VW with GEO_BALANCE (BC_Uran.h:155)
+    p->rigid_lid = 1;
+    p->radius_positive = 1;                             // BC_Uran.h:70
+    p->species_clamp = 0;
+    (void)g;
+    return ATU_OK;
+}
+
+extern "C" int atu_cp_table(int mode, double* out, int* n) {
+    if ((mode != 1 && mode != 2) || !n) return ATU_EINVAL;
+    const std::vector<double>& t = cp_h2_table(mode);
+    *n = (int)t.size();
+    if (out) std::memcpy(out, t.data(), t.size() * sizeof(double));
+    return ATU_OK;
+}
+
+extern "C" double atu_cp_lapse(const atu_params* p, double T_K) {
+    if (!p) return NAN;
+    return cp_lapse(*p, T_K);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Synthetic state. SplitMix64 (Steele, Lea & Flood 2014) gives 64-bit words; the top 53 bits make
+// a uniform in (0, 1]; Box-Muller turns two uniforms into one normal. Each field has its own
+// stream (seed mixed with the field id) drawn in (i, j, k) order over the full array.
+// ---------------------------------------------------------------------------------------------
+namespace {
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 1.0) * (1.0 / 9007199254740992.0); }   // (0, 1]
+    double normal() {
+        const double u1 = uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    }
+};
+SplitMix64 stream(uint64_t seed, int field) {
+    SplitMix64 r{seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(field + 1))};
+    for (int w = 0; w < 4; w++) r.next();
+    return r;
+}
+}  // namespace
+
+extern "C" int atu_synthetic_state(const atu_grid* g, const atu_params* p, uint64_t seed, double* fields) {
+    if (!g || !p || !fields || g->im < 4 || g->jm < 4 || g->km < 4) return ATU_EINVAL;
+    const int im = g->im, jm = g->jm, km = g->km;
+    const size_t n3 = (size_t)im * jm * km;
+    std::memset(fields, 0, n3 * ATU_NFIELDS * sizeof(double));
+    auto F = [&](int f) { return fields + (size_t)f * n3; };
+    auto at = [&](int i, int j, int k) { return ((size_t)i * jm + j) * km + k; };
+
+    Geometry geo;
+    build_geometry(*g, *p, geo);
+    const double t_eq = 60.65, t_pole = 49.15, lapse = 0.7, R_ref = 3.61491, p_bottom = 17.3;
+    const double deg_per_row = (g->dthe * 180.0 / M_PI);
+
+    // Height of level i in km: L_atm (rad.z - r0) / (rad.z[im-1] - r0) -- 5.625 i on the reference grid.
+    std::vector<double> z_km(im);
+    for (int i = 0; i < im; i++) z_km[i] = p->L_atm * (geo.radz[i] - g->r0) / (geo.radz[im - 1] - g->r0);
+
+    // Temperature: the top profile T_top(j) = (t_pole - t_eq)(d^2/90^2 - 2d/90) + t_pole with d the
+    // colatitude in degrees (planet/InitVariables_Uran.cpp:35-36), warming downward at 0.7 K/km.
+    std::vector<double> T_top(jm);
+    for (int j = 0; j < jm; j++) {
+        const double d = j * deg_per_row;
+        T_top[j] = (t_pole - t_eq) * (d * d / (90.0 * 90.0) - 2.0 * d / 90.0) + t_pole;
+    }
+    {
+        SplitMix64 r = stream(seed, ATU_T);
+        for (int i = 0; i < im; i++)
+            for (int j = 0; j < jm; j++)
+                for (int k = 0; k < km; k++)
+                    F(ATU_T)[at(i, j, k)] = (T_top[j] + lapse * (p->L_atm - z_km[i])) / p->t_ref
+                                           + (0.05 / p->t_ref) * r.normal();
+    }
+    // Zonal wind w = cos(3 lat) (1 + 0.1 N); v and the radial u ~ 0.001 N (0.1 m/s over u_0).
+    {
+        SplitMix64 r = stream(seed, ATU_W);
+        for (int i = 0; i < im; i++)
+            for (int j = 0; j < jm; j++) {
+                const double lat = 0.5 * M_PI - geo.thez[j];
+                const double c3 = std::cos(3.0 * lat);
+                for (int k = 0; k < km; k++) F(ATU_W)[at(i, j, k)] = c3 * (1.0 + 0.1 * r.normal());
+            }
+    }
+    {
+        SplitMix64 r = stream(seed, ATU_V);
+        for (size_t q = 0; q < n3; q++) F(ATU_V)[q] = 0.001 * r.normal();
+    }
+    {
+        SplitMix64 r = stream(seed, ATU_U);
+        for (size_t q = 0; q < n3; q++) F(ATU_U)[q] = 0.001 * r.normal();
+    }
+    // p_stat: exact hydrostatic pressure of the noise-free linear-T column from 17.3 bar at the
+    // bottom, p *= (T1/T0)^(g dz / (R dT)) (planet/cUranusModel.h:171-180).
+    for (int j = 0; j < jm; j++) {
+        std::vector<double> col(im);
+        double pr = p_bottom;
+        col[0] = pr;
+        for (int i = 1; i < im; i++) {
+            const double T0 = T_top[j] + lapse * (p->L_atm - z_km[i - 1]);
+            const double T1 = T_top[j] + lapse * (p->L_atm - z_km[i]);
+            const double a = p->g * (z_km[i] - z_km[i - 1]) / R_ref;
+            const double dT = T0 - T1;
+            pr *= (std::fabs(dT) > 1.0e-9 * T0) ? std::pow(T1 / T0, a / dT) : std::exp(-a / T0);
+            col[i] = pr;
+        }
+        for (int i = 0; i < im; i++)
+            for (int k = 0; k < km; k++) F(ATU_P_STAT)[at(i, j, k)] = col[i];
+    }
+    for (size_t q = 0; q < n3; q++)
+        F(ATU_RHO_MIX)[q] = F(ATU_P_STAT)[q] * 1.0e5 / (p->R_mix * F(ATU_T)[q] * p->t_ref);
+    // CH4 = 0.02 exp(-z / 40 km); every other species 0.
+    for (int i = 0; i < im; i++) {
+        const double c = 0.02 * std::exp(-z_km[i] / 40.0);
+        for (int j = 0; j < jm; j++)
+            for (int k = 0; k < km; k++) F(ATU_CH4)[at(i, j, k)] = c;
+    }
+    // Turbulence: k = 1 m^2/s^2 background, omega_bg of a 5625 m length (planet/Turbulence.h:441-443),
+    // and the seeded eddy viscosity nue* = 1.65e-4 (planet/RHS_Uran_Turb.cpp:433).
+    const double C_nue = 0.09, karman = 0.41, L_bg = 5625.0, k_bg = 1.0;
+    const double om_bg = std::pow(C_nue, -0.25) * std::sqrt(k_bg) / (karman * L_bg);
+    const double tke0 = k_bg / (p->u_0 * p->u_0);
+    const double dis0 = om_bg * (p->L_atm * 1.0e3) / p->u_0;
+    for (size_t q = 0; q < n3; q++) {
+        F(ATU_TKE)[q] = tke0;
+        F(ATU_DIS)[q] = dis0;
+        F(ATU_NUE)[q] = 1.65e-4;
+    }
+    return ATU_OK;
+}

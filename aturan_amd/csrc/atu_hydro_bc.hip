@@ -1,0 +1,231 @@
+// Hydrostatic prep (planet/RungeKutta_Uran_Turb.cpp:18-177) and the domain boundary conditions
+// (planet/BoundaryConditions.h:152-518, planet/BC_Uran.h:91-171, field lists planet/BC_Uran.cpp).
+
+#include "atu_kernels.h"
+
+namespace atu {
+namespace {
+
+// ---- level means: sum over the full (j, k) plane of sin(theta) * b, skipping T <= 0 and
+// non-finite b (computeBuoyancyRefLevel, :18-37).
+//
+// The reference adds the terms of a level one after another in (j, k) order, and the buoyancy
+// anomaly divides by this mean, which magnifies any rounding difference of the sum ~200-fold in
+// p_hydro. So the sum is formed in exactly that order: one wavefront per level computes 64
+// consecutive terms in parallel (a skipped term contributes an exact +0.0), parks them in LDS, and
+// lane 0 adds them in sequence. The chain is ~65k dependent adds per level; the step runs this
+// kernel on a side stream, concurrently with the previous step's Poisson solve, which it does not
+// depend on.
+__global__ void __launch_bounds__(64) k_level_serial(Shape sh, DevGeo g, StepConsts c, const double* t,
+                                                     const double* pstat, double* level) {
+    const int i = blockIdx.x;
+    const int n = sh.jm * sh.km;
+    __shared__ double tb[64], tw[64];
+    double s = 0.0, ws = 0.0;
+    for (int q0 = 0; q0 < n; q0 += 64) {
+        const int q = q0 + (int)threadIdx.x;
+        double term = 0.0, wt = 0.0;
+        if (q < n) {
+            const int j = q / sh.km, k = q - j * sh.km;
+            const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+            const double T = t[x];
+            if (T > 0.0) {
+                const double b = c.g * pstat[x] / (c.hyd_bnum * T * c.t_ref);
+                if (isfinite(b)) {
+                    const double wgt = g.sin_weight[j];
+                    term = wgt * b;
+                    wt = wgt;
+                }
+            }
+        }
+        tb[threadIdx.x] = term;
+        tw[threadIdx.x] = wt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(64, n - q0);
+            for (int l = 0; l < m; l++) {
+                s += tb[l];
+                ws += tw[l];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) level[i] = (ws > 0.0) ? s / ws : 0.0;
+}
+
+// ---- the column scan: p_hydro[0] = 0, p_hydro[i] = p_hydro[i-1] + 0.5 (f(i-1) + f(i)) dr with
+// f = buoyancy anomaly relative to the level mean (BUOY_RHO_LEVEL) + (v^2 + w^2)/rm + Coriolis
+// (METRIC_SPLIT + CORIOLIS_SPLIT branch, :104-161). One thread per (j, k) column.
+__global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepConsts c, const double* t,
+                                                    const double* v, const double* w, const double* pstat,
+                                                    const double* level, double* phydro) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (k >= sh.km) return;
+    const double cor = g.cor_hydro[j];
+    auto f_at = [&](int i) -> double {
+        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+        const double T = t[x];
+        double buoy = 0.0;
+        if (T > 0.0) {
+            const double ref = level[i];
+            const double b = c.g * pstat[x] / (c.hyd_bnum * T * c.t_ref) - ref;
+            const double f = -c.nd_rel * 1.0 * 1.0 * b / ref;
+            buoy = isfinite(f) ? f : 0.0;
+        }
+        const double vv = v[x], ww = w[x];
+        return buoy + (vv * vv + ww * ww) / g.rm[i] + cor * ww;
+    };
+    double p = 0.0;
+    double f_prev = f_at(0);
+    phydro[(size_t)j * sh.ldk + k] = 0.0;
+    for (int i = 1; i < sh.im; i++) {
+        const double fi = f_at(i);
+        p = p + 0.5 * (f_prev + fi) * c.dr;
+        phydro[((size_t)i * sh.jm + j) * sh.ldk + k] = p;
+        f_prev = fi;
+    }
+}
+
+// ---- boundary conditions ----
+struct BcCtx {
+    Shape sh;
+    BcTargets t;
+    double c43, c13;
+    int lid_vw, rigid_lid, radius_positive;
+};
+
+__device__ __forceinline__ void bc_put(const BcCtx& b, size_t q, double val) {
+    b.t.y[q] = val;
+    if (b.t.m64a) b.t.m64a[q] = val;
+    if (b.t.m64b) b.t.m64b[q] = val;
+    if (b.t.m32a) b.t.m32a[q] = (float)val;
+    if (b.t.m32b) b.t.m32b[q] = (float)val;
+    if (b.t.m32c) b.t.m32c[q] = (float)val;
+}
+
+constexpr int kNumBcFields = 17;   // t, u, v, w and the 13 species: the prognostic part of the lists
+
+// bcRadius with margin 0 (every j, every k), NEUMANN form, then the rigid lid on u, the species
+// floor (floorRadialSpecies) and the lid form of v and w (BC_LID_VW).
+__global__ void __launch_bounds__(256) k_bc_radius(BcCtx b) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (k >= b.sh.km) return;
+    const Shape& sh = b.sh;
+    const size_t col = (size_t)j * sh.ldk + k;
+    const size_t P = sh.plane;
+    const int L = sh.im - 1;
+    for (int f = 0; f < kNumBcFields; f++) {
+        const double* F = b.t.y + (size_t)f * sh.fs + col;
+        const double lo = b.c43 * F[1 * P] - b.c13 * F[2 * P];
+        const double hi = b.c43 * F[(size_t)(L - 1) * P] - b.c13 * F[(size_t)(L - 2) * P];
+        bc_put(b, (size_t)f * sh.fs + col, lo);
+        bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, hi);
+    }
+    if (b.rigid_lid) {
+        bc_put(b, (size_t)ATU_U * sh.fs + col, 0.0);
+        bc_put(b, (size_t)ATU_U * sh.fs + col + (size_t)L * P, 0.0);
+    }
+    if (b.radius_positive) {
+        for (int f = ATU_CH4; f <= ATU_NH4SH; f++) {
+            const double* F = b.t.y + (size_t)f * sh.fs + col;
+            const double f0 = F[0], fl = F[(size_t)L * P];
+            if (f0 < 0.0) {
+                const double f1 = F[P];
+                bc_put(b, (size_t)f * sh.fs + col, b.radius_positive == 2 ? (0.0 < f1 ? f1 : 0.0) : 0.0);
+            }
+            if (fl < 0.0) {
+                const double f1 = F[(size_t)(L - 1) * P];
+                bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P,
+                       b.radius_positive == 2 ? (0.0 < f1 ? f1 : 0.0) : 0.0);
+            }
+        }
+    }
+    if (b.lid_vw == 2 || b.lid_vw == 3) {
+        for (int f = ATU_V; f <= ATU_W; f++) {
+            const double* F = b.t.y + (size_t)f * sh.fs + col;
+            const double fa = F[(size_t)(L - 1) * P], fb = F[(size_t)(L - 2) * P], fc = F[(size_t)(L - 3) * P];
+            const double val = (b.lid_vw == 2) ? 2.0 * fa - fb : fc - 3.0 * fb + 3.0 * fa;
+            bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, val);
+        }
+    }
+}
+
+// bcTheta with margin 0: v, w zeroed at both poles, t, u and the species by the NEUMANN form.
+__global__ void __launch_bounds__(256) k_bc_theta(BcCtx b) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (k >= b.sh.km) return;
+    const Shape& sh = b.sh;
+    const int ldk = sh.ldk;
+    const int Jl = sh.jm - 1;
+    const size_t base = (size_t)i * sh.plane + k;
+    for (int f = 0; f < kNumBcFields; f++) {
+        const size_t q0 = (size_t)f * sh.fs + base;
+        if (f == ATU_V || f == ATU_W) {
+            bc_put(b, q0, 0.0);
+            bc_put(b, q0 + (size_t)Jl * ldk, 0.0);
+            continue;
+        }
+        const double* F = b.t.y + q0;
+        const double lo = b.c43 * F[(size_t)1 * ldk] - b.c13 * F[(size_t)2 * ldk];
+        const double hi = b.c43 * F[(size_t)(Jl - 1) * ldk] - b.c13 * F[(size_t)(Jl - 2) * ldk];
+        bc_put(b, q0, lo);
+        bc_put(b, q0 + (size_t)Jl * ldk, hi);
+    }
+}
+
+// bcPhi with the periodic seam: k = km-1 is the copy of k = 0.
+__global__ void __launch_bounds__(256) k_bc_phi(BcCtx b) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= b.sh.jm) return;
+    const Shape& sh = b.sh;
+    const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
+    for (int f = 0; f < kNumBcFields; f++) {
+        const size_t q = (size_t)f * sh.fs + row;
+        bc_put(b, q + sh.km - 1, b.t.y[q]);
+    }
+}
+
+// clampNegativeSpecies (planet/BC_Uran.h:110-128): every species value that is not >= 0 -> 0.
+__global__ void __launch_bounds__(256) k_species_clamp(BcCtx b) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= b.sh.km) return;
+    const size_t x = ((size_t)i * b.sh.jm + j) * b.sh.ldk + k;
+    for (int f = ATU_CH4; f <= ATU_NH4SH; f++) {
+        const size_t q = (size_t)f * b.sh.fs + x;
+        if (!(b.t.y[q] >= 0.0)) bc_put(b, q, 0.0);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
+                        const double* w, const double* pstat, double* level_out, double* phydro, hipStream_t s) {
+    hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(64), 0, s, sh, g, c, t, pstat, level_out);
+    hipLaunchKernelGGL(k_hydro_scan, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, sh, g, c, t, v, w, pstat,
+                       level_out, phydro);
+    return hipGetLastError();
+}
+
+hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s) {
+    BcCtx b;
+    b.sh = sh;
+    b.t = t;
+    b.c43 = 4.0 / 3.0;
+    b.c13 = 1.0 / 3.0;
+    b.lid_vw = p.lid_vw;
+    b.rigid_lid = p.rigid_lid;
+    b.radius_positive = p.radius_positive;
+    hipLaunchKernelGGL(k_bc_radius, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_bc_theta, dim3((sh.km + 255) / 256, sh.im), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_bc_phi, dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, b);
+    if (p.species_clamp)
+        hipLaunchKernelGGL(k_species_clamp, dim3((sh.km + 255) / 256, sh.jm, sh.im), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+}  // namespace atu

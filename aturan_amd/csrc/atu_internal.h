@@ -1,0 +1,60 @@
+// Internal declarations shared by the host code and the HIP kernels of libaturan_amd.
+#pragma once
+
+#include "atu.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace atu {
+
+// Host-built geometry. Every metric factor the stencils use is formed here once, in FP64, with
+// the same operand order as the reference's CellGeometry (planet/RungeKutta_Uran_Turb.cpp:192-283)
+// and solver weights (planet/PressureSolver.h:156-162,327-353), so the device reads bits the CPU
+// would have computed. Per-(i,j) arrays are [im][jm].
+struct Geometry {
+    int im = 0, jm = 0, km = 0;
+    std::vector<double> radz, thez;          // cumulative coordinates
+    // per i
+    std::vector<double> rm, inv_rm, inv_rm2, two_inv_rm, sponge, num2;
+    // per j
+    std::vector<double> sinthe, sinthe2, costhe, cotanthe, sinthe_bf, cor_hydro, sin_weight;
+    // per (i,j)
+    std::vector<double> inv_rmsinthe, inv_rm2sinthe, inv_rm2sinthe2, costhe_inv_rm2sinthe,
+        v_metric, num3, den, inv_rmsinthe_s;
+    // scalars
+    double inv_2dr = 0, inv_2dthe = 0, inv_2dphi = 0, inv_dr2 = 0, inv_dthe2 = 0, inv_dphi2 = 0;
+    double num1 = 0;
+};
+
+void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo);
+
+// cp table of H2 for a CpH2 mode (1 normal, 2 equilibrium), J/(kg K).
+const std::vector<double>& cp_h2_table(int mode);
+double cp_lapse(const atu_params& p, double T_K);
+
+// Derived per-run scalar constants of the RHS, formed in the reference's operand order.
+struct StepConsts {
+    double dt, c_half, c_full, dt6;
+    double coeff_energy_p, scale_p, scale_Cor;
+    double re_pr, re, inv_sig_k, inv_sig_w, inv_re;
+    double sc_re_ch4, sc_re_h2o, sc_re_h2s, sc_re_nh3, sc_re_nh4sh;
+    double precip_num, precip_den, precip_coupling, chemical_reaction;
+    double stokes_c, rho_cond_nh4sh, g, mue_mix, u_0;
+    double turb_coupling, Pr_t, two_omega, m_two_omega, Coriolis;
+    double t_min, t_max, tke_max, dis_min;
+    double cp_mix, t_ref, cp_a, cp_b, cp_den, r_h2;
+    int cp_mode, cp_gas, turb_active;
+    // hydro prep
+    double hyd_bnum, nd_rel, dr;
+    // solver
+    double p_dyn_cap, c43, c13;
+};
+void build_step_consts(const atu_grid& g, const atu_params& p, StepConsts& c);
+
+// Padded k stride of device fields: km rounded up to a multiple of 32 elements.
+inline int padded_ldk(int km) { return (km + 31) / 32 * 32; }
+
+}  // namespace atu

@@ -1,0 +1,238 @@
+/*
+ * atu.h -- C ABI of the MI355X dynamical-core step (RHS + RK4 + BC + red-black Poisson).
+ *
+ * Plain C: pointers, sizes and PODs only, no C++ or torch types, so a C++ host loop, a ctypes
+ * binding and a cgo/JNI stub can all bind it. Every function returns 0 on success and a negative
+ * atu_status otherwise; atu_last_error() then holds a message. No exception crosses this ABI.
+ *
+ * Host arrays handed to atu_set_field / atu_get_field are dense [im][jm][km] FP64, k (phi)
+ * fastest -- the element order of the reference's jagged Array x[i][j][k] (lib/Array.h:31) once
+ * each (i,j) row is packed. The context owns every device buffer; calls on one context are
+ * serialised on its HIP stream and are not reentrant.
+ *
+ * Field naming follows the reference (planet/cUranusModel.h:1356-1373): u is RADIAL, v is the
+ * theta (colatitude, +southward) component and w is ZONAL.
+ */
+#ifndef ATU_H
+#define ATU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ATU_ABI_VERSION 1
+
+/* ---- status codes ---- */
+typedef enum atu_status {
+    ATU_OK = 0,
+    ATU_EINVAL = -1,      /* bad argument (null pointer, unknown id, size out of range)   */
+    ATU_EHIP = -2,        /* a HIP runtime call failed                                      */
+    ATU_ENOMEM = -3,      /* device or host allocation failed                               */
+    ATU_ENONFINITE = -4,  /* a non-finite value was detected where the step forbids it      */
+    ATU_ENOCONV = -5,     /* the Poisson solve did not reach its tolerance within max_sweeps */
+    ATU_EUNSUPPORTED = -6 /* option combination this build does not implement               */
+} atu_status;
+
+/* ---- fields ----
+ * 0..18   the prognostic set P19 (planet/cUranusModel.h:1356-1373, 352-353)
+ * 19..39  the 21 frozen per-step inputs the RHS reads at default knobs
+ *         (planet/RHS_Uran_Turb.cpp:439,544,793-913,1011-1014)
+ * 40..45  p_stat, p_dyn, p_hydro and the projection fields aux_u/v/w
+ */
+typedef enum atu_field {
+    ATU_T = 0, ATU_U, ATU_V, ATU_W,
+    ATU_CH4, ATU_CH4_CLOUD, ATU_CH4_ICE,
+    ATU_H2O, ATU_H2O_CLOUD, ATU_H2O_ICE,
+    ATU_H2S, ATU_H2S_CLOUD, ATU_H2S_ICE,
+    ATU_NH3, ATU_NH3_CLOUD, ATU_NH3_ICE,
+    ATU_NH4SH, ATU_TKE, ATU_DIS,                       /* 19 prognostic */
+    ATU_NUE = 19, ATU_RHO_MIX, ATU_Q_PRECIP,
+    ATU_SP_CH4, ATU_SP_CH4_CLOUD, ATU_SP_CH4_ICE,
+    ATU_SP_H2O, ATU_SP_H2O_CLOUD, ATU_SP_H2O_ICE,
+    ATU_SP_H2S, ATU_SP_H2S_CLOUD, ATU_SP_H2S_ICE,
+    ATU_SP_NH3, ATU_SP_NH3_CLOUD, ATU_SP_NH3_ICE,
+    ATU_MF_H2S, ATU_MF_NH3, ATU_MF_NH4SH,
+    ATU_FLUXLIM_NH4SH, ATU_TKE_SOURCE, ATU_DIS_SOURCE, /* 21 frozen   */
+    ATU_P_STAT = 40, ATU_P_DYN, ATU_P_HYDRO,
+    ATU_AUX_U, ATU_AUX_V, ATU_AUX_W,
+    ATU_NFIELDS = 46
+} atu_field;
+
+#define ATU_NPROG 19
+#define ATU_NFROZEN 21
+
+/* ---- grid ----
+ * Coordinates are built by cumulative addition z[l] = z[l-1] + d (lib/Array_1D.cpp:50-61).
+ * The reference grid: im,jm,km = 41,181,361 (planet/cUranusModel.h:867), r0 = 1, dr = 0.025,
+ * the0 = phi0 = 0, dthe = dphi = 1/(180/pi) (planet/cUranusModel.cpp:339-351).
+ * phi is periodic over km-1 distinct meridians; k = km-1 is the copy of k = 0.
+ */
+typedef struct atu_grid {
+    int32_t im, jm, km;
+    int32_t pad_;
+    double r0, dr;
+    double the0, dthe;
+    double phi0, dphi;
+} atu_grid;
+
+/* ---- constants and default knobs consumed by the step (planet/cUranusModel.h, param.py) ---- */
+typedef struct atu_params {
+    double dt;                 /* nondimensional step, planet/cUranusModel.cpp:475          */
+    double re, pr, g, omega, u_0, t_ref, L_atm;
+    double Coriolis, centrifugal, buoyancy, chemical_reaction;
+    double sc_ch4, sc_h2o, sc_h2s, sc_nh3, sc_nh4sh;
+    double alpha_sponge;
+    double rho_cond_nh4sh, r_p_nh4sh;
+    double r_mix, R_mix, cp_mix, mue_mix;      /* mixture, after the T_INIT_ANCHOR rescale   */
+    double r_h2, r_he, r_ch4;                  /* rescaled reference densities (cp(T) mix)   */
+    double cp_h2, cp_he, cp_ch4;               /* constant specific heats, J/(kg K)          */
+    double R_metric_km;                        /* metricRadius shift (planet/cUranusModel.h:641-647) */
+    double sinthe_min;                         /* polar metric floor (planet/cUranusModel.h:675-688) */
+    double t_min_K, t_max_K;                   /* RK clamps (planet/cUranusModel.h:116-117)  */
+    double tke_max_m2s2, dis_min_nd;           /* planet/RungeKutta_Uran_Turb.cpp:210-211    */
+    double turb_coupling, Pr_t, sig_k, sig_w;
+    double precip_coupling;
+    double p_dyn_cap;                          /* planet/PressureSolver.h:168                */
+    double coriolis_scale;                     /* L_atm*1e3/u_0 with CORIOLIS_FULL            */
+    int32_t cp_mode;                           /* CpH2 mode: 0 constant, 1 normal, 2 equilibrium */
+    int32_t cp_gas;                            /* CpH2 CP_GAS: 1 = H2-He-CH4 gas mix          */
+    int32_t turb_active;                       /* tke/dis transport on                        */
+    int32_t lid_vw;                            /* ATURAN_BC_LID_VW: 0 Neumann, 2 linear, 3 cubic */
+    int32_t rigid_lid;                         /* u = 0 on both radial planes                 */
+    int32_t radius_positive;                   /* ATURAN_BC_RADIUS_POSITIVE: 0 off, 1 floor at 0 */
+    int32_t species_clamp;                     /* ATURAN_SPECIES_CLAMP after the BC (0 = off)  */
+    int32_t pad_;
+} atu_params;
+
+/* ---- precision of a context ---- */
+typedef enum atu_precision {
+    ATU_PREC_FP64 = 0,   /* every array and every operation in FP64                          */
+    ATU_PREC_MIXED = 1   /* FP32 stencil/stage states/smoother; FP64 y_n, accumulator,
+                            residual, refinement correction and pole-row metric factors      */
+} atu_precision;
+
+/* ---- Poisson solve options ---- */
+typedef enum atu_solve_mode {
+    ATU_SOLVE_REFERENCE = 0, /* `sweeps` red-black sweeps with the wall BCs inside each sweep,
+                                exactly planet/PressureSolver.h:318-757 (omega = 1 is the
+                                reference's Gauss-Seidel; other omega is SOR)                 */
+    ATU_SOLVE_CONVERGE = 1   /* iterate to ||src - Lc p||_2 / ||src||_2 <= rel_tol            */
+} atu_solve_mode;
+
+typedef struct atu_solver_opts {
+    int32_t mode;            /* atu_solve_mode                                                */
+    int32_t sweeps;          /* REFERENCE: sweeps to run                                      */
+    int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing sweeps                     */
+    int32_t pad_;
+    double omega;            /* relaxation factor (1 = Gauss-Seidel)                          */
+    double rel_tol;          /* CONVERGE: relative residual target (FP64)                     */
+} atu_solver_opts;
+
+typedef struct atu_solve_stats {
+    int32_t sweeps_done;
+    int32_t outer_iters;
+    double rel_resid;        /* ||src - Lc p||_2 / ||src||_2 after the solve (FP64)           */
+    double src_norm;         /* ||src||_2 over the interior                                   */
+} atu_solve_stats;
+
+typedef struct atu_ctx atu_ctx;
+
+/* ---- grid and parameter defaults ---- */
+
+/* The reference grid (refine = 1) or the refined shell (refine = 2: 81x361x721, dr/2, 0.5 deg).
+ * Replaces the compile-time im, jm, km of planet/cUranusModel.h:867 and the coordinate set-up of
+ * planet/cUranusModel.cpp:365-370. */
+int atu_grid_default(int refine, atu_grid* out);
+
+/* The constants of the default configuration, including the mixture properties of
+ * ThermalPropertiesUran (planet/ChemistryUran.h:437-490) and the T_INIT_ANCHOR rescale of the
+ * reference densities (planet/cUranusModel.cpp:504-511). */
+int atu_params_default(const atu_grid* g, atu_params* out);
+
+/* The H2 cp table of planet/CpH2.h:77-86 (mode 1 or 2; 2991 entries from 5 K in 0.5 K steps), in
+ * J/(kg K). `n` receives the length; pass out = NULL to query it. */
+int atu_cp_table(int mode, double* out, int* n);
+
+/* cp(T) for lapse rates and the RHS compression term, CpH2::cp_lapse (planet/CpH2.h:128-134). */
+double atu_cp_lapse(const atu_params* p, double T_K);
+
+/* Deterministic synthetic initial state (SplitMix64 -> 53-bit uniforms -> Box-Muller, drawn in
+ * (field, i, j, k) order). Writes all 46 fields into `fields` (ATU_NFIELDS consecutive dense
+ * [im][jm][km] arrays). The domain BC is not applied: callers run atu_apply_bc once after upload.
+ * No counterpart in the reference: it stands in for the reference's initialisation
+ * (planet/InitVariables_Uran.cpp), which the benchmark replaces by this seeded state. */
+int atu_synthetic_state(const atu_grid* g, const atu_params* p, uint64_t seed, double* fields);
+
+/* ---- context ---- */
+
+/* Create a context on the current HIP device. Allocates every device buffer: y_n, two stage
+ * states and the accumulator for the 19 prognostic fields, the frozen inputs, p_stat, p_dyn,
+ * p_hydro, aux, the solver source and the geometry and cp tables. Replaces the model-owned
+ * arrays of cUranusModel::resetArrays (planet/cUranusModel.cpp:1416-1634). */
+int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** out);
+void atu_destroy(atu_ctx* ctx);
+const char* atu_last_error(const atu_ctx* ctx);   /* NULL ctx: the last create error */
+
+/* Attach the context to a HIP stream (hipStream_t passed as void*; NULL = its own stream). */
+int atu_set_stream(atu_ctx* ctx, void* hip_stream);
+int atu_synchronize(atu_ctx* ctx);
+
+/* Copy one field in or out (dense [im][jm][km] FP64 host array). atu_set_field on a prognostic
+ * field writes the start-of-step state y_n and every stage buffer, the role restoreVar plays in
+ * the reference (planet/cUranusModel.cpp:1638-1685). */
+int atu_set_field(atu_ctx* ctx, int field, const double* host);
+int atu_get_field(atu_ctx* ctx, int field, double* host);
+
+/* ---- the step pieces ---- */
+
+/* computeBuoyancyRefLevel + computeHydrostaticPressure (planet/RungeKutta_Uran_Turb.cpp:18-177):
+ * the sin(theta)-weighted buoyancy level means and the trapezoid column scan into p_hydro.
+ * `level_means` (optional, im doubles) receives buoy_ref_level. */
+int atu_hydro_prep(atu_ctx* ctx, double* level_means);
+
+/* cUranusModel::RungeKuttaUran (planet/RungeKutta_Uran_Turb.cpp:179-404) with RHSUran
+ * (planet/RHS_Uran_Turb.cpp:20-1023) fused into its stages: four stages, the accumulator, the
+ * clamps and the final update, in place on the prognostic fields. aux_u/v/w are those of the last
+ * stage. Boundary planes are not written (the stencils read the start-of-step BC values). */
+int atu_rk4(atu_ctx* ctx);
+
+/* BC_Uran::bcRadius/bcTheta/bcPhi (planet/BC_Uran.h:140-173, planet/BoundaryConditions.h:152-518)
+ * on the prognostic fields, then clampNegativeSpecies when params.species_clamp is set. */
+int atu_apply_bc(atu_ctx* ctx);
+
+/* prepareProjectionBoundaries (planet/Pressure_Uran.cpp:30-146) followed by the red-black solve
+ * of PressureSolver<Planet>::run (planet/PressureSolver.h:145-766) for p_dyn. */
+int atu_pressure_solve(atu_ctx* ctx, const atu_solver_opts* opts, atu_solve_stats* stats);
+
+/* One outer step as the benchmark loop orders it: hydro prep, RK4, BC, Poisson, `nsteps` times.
+ * `stats` (optional) receives the last solve's statistics. */
+int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_stats* stats);
+
+/* Relative residual ||src - Lc p||_2 / ||src||_2 of the current p_dyn against the current source
+ * (FP64; the source must have been prepared by a solve). */
+int atu_pressure_residual(atu_ctx* ctx, double* rel_resid, double* src_norm);
+
+/* Device pointer of a field's storage (for zero-copy interop); `ld_k` receives the k stride in
+ * elements, `elem_bytes` 8 or 4. The layout is [im][jm][ld_k]. */
+int atu_field_device_ptr(atu_ctx* ctx, int field, void** dptr, int* ld_k, int* elem_bytes);
+
+/* Library build information string (arch, flags). */
+const char* atu_build_info(void);
+
+/* ---- per-kernel timing (no counterpart in the reference; used by the benchmark) ----
+ * Between atu_profile_begin and atu_profile_query every kernel launch of the context is bracketed
+ * by HIP events recorded on the context's stream. atu_profile_query synchronises and returns the
+ * summed duration and the launch count of one kernel class: "rk_stage", "rb_pass", "hydro", "bc",
+ * "proj_src", "residual", "line", "baro". */
+int atu_profile_begin(atu_ctx* ctx);
+int atu_profile_query(atu_ctx* ctx, const char* kernel_class, double* total_ms, int* launches);
+int atu_profile_end(atu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATU_H */

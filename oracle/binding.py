@@ -1,0 +1,113 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (oracle.c).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker of the
+HIP library. The product (aturan_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_DP = C.POINTER(C.c_double)
+_lib = None
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    V = C.c_void_p
+    sig = {
+        "orc_cp_table": (C.c_int, [C.c_int, _DP]),
+        "orc_cp_lapse": (C.c_double, [V, C.c_double]),
+        "orc_rhs": (C.c_int, [V, V, _DP, _DP]),
+        "orc_rk4": (C.c_int, [V, V, _DP]),
+        "orc_hydro": (C.c_int, [V, V, _DP, _DP]),
+        "orc_bc": (C.c_int, [V, V, _DP]),
+        "orc_pressure": (C.c_int, [V, V, _DP, C.c_int, C.c_double, _DP]),
+        "orc_residual": (C.c_int, [V, V, _DP, _DP, _DP, _DP]),
+        "orc_step": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double]),
+    }
+    for n, (res, args) in sig.items():
+        f = getattr(lib, n)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(x):
+    return C.cast(C.pointer(x), C.c_void_p)
+
+
+def _d(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_DP)
+
+
+def _ok(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {rc}")
+
+
+def cp_table(mode: int) -> np.ndarray:
+    out = np.zeros(2991)
+    n = load().orc_cp_table(mode, _d(out))
+    assert n == 2991
+    return out
+
+
+def cp_lapse(params, T: float) -> float:
+    return load().orc_cp_lapse(_p(params), T)
+
+
+def rhs(grid, params, fields: np.ndarray) -> np.ndarray:
+    """One RHS evaluation: (22, im, jm, km) = 19 tendencies + aux_u, aux_v, aux_w (interior only)."""
+    out = np.zeros((22,) + fields.shape[1:])
+    _ok(load().orc_rhs(_p(grid), _p(params), _d(fields), _d(out)), "orc_rhs")
+    return out
+
+
+def rk4(grid, params, fields: np.ndarray) -> None:
+    _ok(load().orc_rk4(_p(grid), _p(params), _d(fields)), "orc_rk4")
+
+
+def hydro(grid, params, fields: np.ndarray) -> np.ndarray:
+    lev = np.zeros(grid.im)
+    _ok(load().orc_hydro(_p(grid), _p(params), _d(fields), _d(lev)), "orc_hydro")
+    return lev
+
+
+def bc(grid, params, fields: np.ndarray) -> None:
+    _ok(load().orc_bc(_p(grid), _p(params), _d(fields)), "orc_bc")
+
+
+def pressure(grid, params, fields: np.ndarray, sweeps: int, omega: float) -> np.ndarray:
+    src = np.zeros(fields.shape[1:])
+    _ok(load().orc_pressure(_p(grid), _p(params), _d(fields), sweeps, omega, _d(src)), "orc_pressure")
+    return src
+
+
+def residual(grid, params, fields: np.ndarray, src: np.ndarray) -> tuple[float, float]:
+    r, s = C.c_double(), C.c_double()
+    _ok(load().orc_residual(_p(grid), _p(params), _d(fields), _d(src), C.byref(r), C.byref(s)), "orc_residual")
+    return r.value, s.value
+
+
+def step(grid, params, fields: np.ndarray, nsteps: int, sweeps: int, omega: float) -> None:
+    _ok(load().orc_step(_p(grid), _p(params), _d(fields), nsteps, sweeps, omega), "orc_step")

@@ -1,0 +1,745 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE ONLY. A plain-C, FP64, OpenMP restatement of the reference's
+ * dynamical-core step, used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+ * the checker of the HIP library. Nothing in the product links or calls it.
+ *
+ * It follows the reference's own structure rather than the GPU design: RHSUran writes rhs_* and
+ * aux_* arrays per cell (pass A), a second pass folds them into acc_* and the next stage input from
+ * the n-copies (pass B), the final update clamps, and the pressure solver runs its wall passes as
+ * separate loops after each sweep. Citations are to files under the reference tree.
+ *
+ *   orc_geometry      lib/Array_1D.cpp:50-61, planet/RungeKutta_Uran_Turb.cpp:192-283,
+ *                     planet/cUranusModel.h:641-647
+ *   orc_cp_h2 / table planet/CpH2.h:55-93, cp_lapse :128-134
+ *   rhs_cell          planet/RHS_Uran_Turb.cpp:20-1023 at the default knobs
+ *   orc_rk4           planet/RungeKutta_Uran_Turb.cpp:179-404
+ *   orc_hydro         planet/RungeKutta_Uran_Turb.cpp:18-177
+ *   orc_bc            planet/BoundaryConditions.h:152-518, planet/BC_Uran.h:91-171, BC_Uran.cpp
+ *   orc_pressure      planet/Pressure_Uran.cpp:30-146, planet/PressureSolver.h:145-766 (plus omega)
+ *
+ * Dense layout: fields[f * n3 + (i * jm + j) * km + k], f in the atu_field order, n3 = im*jm*km.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/atu.h"
+
+#define IDX(i, j, k) (((size_t)(i) * jm + (j)) * km + (k))
+
+static double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
+static double dclamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* ------------------------------------------------------------------------------------------- */
+/* cp of H2 and cp_lapse                                                                        */
+/* ------------------------------------------------------------------------------------------- */
+static double cpR(double T, int mode) {
+    const double thr = 87.57, thv = 6332.0;
+    double Zp = 0, Ep = 0, Fp = 0, Zo = 0, Eo = 0, Fo = 0, c;
+    if (mode == 1) {
+        for (int J = 0; J < 60; J += 2) {
+            double e = thr * J * (J + 1.0), b = 1.0 * (2.0 * J + 1.0) * exp(-e / T);
+            Zp += b; Ep += b * e; Fp += b * e * e;
+        }
+        for (int J = 1; J < 60; J += 2) {
+            double e = thr * J * (J + 1.0), b = 1.0 * (2.0 * J + 1.0) * exp(-e / T);
+            Zo += b; Eo += b * e; Fo += b * e * e;
+        }
+        double m1p = Ep / Zp, m2p = Fp / Zp, m1o = Eo / Zo, m2o = Fo / Zo;
+        c = 0.25 * ((m2p - m1p * m1p) / (T * T)) + 0.75 * ((m2o - m1o * m1o) / (T * T));
+    } else {
+        double Z = 0, E1 = 0, E2 = 0;
+        for (int J = 0; J < 60; J += 2) {
+            double e = thr * J * (J + 1.0), b = 1.0 * (2.0 * J + 1.0) * exp(-e / T);
+            Z += b; E1 += b * e; E2 += b * e * e;
+        }
+        for (int J = 1; J < 60; J += 2) {
+            double e = thr * J * (J + 1.0), b = 3.0 * (2.0 * J + 1.0) * exp(-e / T);
+            Z += b; E1 += b * e; E2 += b * e * e;
+        }
+        double m1 = E1 / Z, m2 = E2 / Z;
+        c = (m2 - m1 * m1) / (T * T);
+    }
+    double x = thv / T;
+    double cv = (x < 700.0) ? x * x * exp(x) / ((exp(x) - 1.0) * (exp(x) - 1.0)) : 0.0;
+    return 2.5 + c + cv;
+}
+
+#define CP_N 2991
+static double cp_tab[3][CP_N];
+static int cp_ready[3];
+
+static void cp_build(int mode) {
+    if (cp_ready[mode]) return;
+#pragma omp critical(orc_cp)
+    {
+        if (!cp_ready[mode]) {
+            for (int i = 0; i < CP_N; i++) cp_tab[mode][i] = cpR(5.0 + i * 0.5, mode) * 8.314462618 / 2.01588e-3;
+            cp_ready[mode] = 1;
+        }
+    }
+}
+
+int orc_cp_table(int mode, double* out) {
+    if (mode != 1 && mode != 2) return -1;
+    cp_build(mode);
+    memcpy(out, cp_tab[mode], sizeof(double) * CP_N);
+    return CP_N;
+}
+
+double orc_cp_lapse(const atu_params* p, double T) {
+    if (p->cp_mode == 0 && !p->cp_gas) return p->cp_mix;
+    double ch = p->cp_h2;
+    if (p->cp_mode > 0) {
+        cp_build(p->cp_mode);
+        const double* t = cp_tab[p->cp_mode];
+        const double lo = (T < 5.0) ? 5.0 : T;              /* std::max(T, T_LO) */
+        const double x = (((1500.0 < lo) ? 1500.0 : lo) - 5.0) / 0.5;   /* std::min(.., T_HI) */
+        int i = (int)x;
+        if (CP_N - 2 < i) i = CP_N - 2;
+        double f = x - i;
+        ch = t[i] + f * (t[i + 1] - t[i]);
+    }
+    if (p->cp_gas) return (p->r_h2 * ch + p->r_he * p->cp_he + p->r_ch4 * p->cp_ch4) / (p->r_h2 + p->r_he + p->r_ch4);
+    return p->cp_mix + (p->r_h2 / p->r_mix) * (ch - p->cp_h2);
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* geometry                                                                                      */
+/* ------------------------------------------------------------------------------------------- */
+typedef struct {
+    double rm, rm2, sinthe, sinthe2, costhe, cotanthe, inv_rm, inv_rm2, inv_rmsinthe, inv_rm2sinthe,
+        inv_rm2sinthe2, costhe_inv_rm2sinthe;
+} cellgeo;
+
+typedef struct {
+    int im, jm, km;
+    double *radz, *thez, *sin_tbl, *cos_tbl;
+    double inv_2dr, inv_2dthe, inv_2dphi, inv_dr2, inv_dthe2, inv_dphi2;
+} orc_geo;
+
+static double metric_radius(const atu_params* p, double r) {
+    if (!(p->R_metric_km > 0.0)) return r;
+    return r + (p->R_metric_km / p->L_atm - 1.0);
+}
+
+static void geo_init(orc_geo* G, const atu_grid* g, const atu_params* p) {
+    G->im = g->im; G->jm = g->jm; G->km = g->km;
+    G->radz = malloc(sizeof(double) * g->im);
+    G->thez = malloc(sizeof(double) * g->jm);
+    G->sin_tbl = malloc(sizeof(double) * g->jm);
+    G->cos_tbl = malloc(sizeof(double) * g->jm);
+    G->radz[0] = g->r0;
+    for (int l = 1; l < g->im; l++) G->radz[l] = G->radz[l - 1] + g->dr;
+    G->thez[0] = g->the0;
+    for (int l = 1; l < g->jm; l++) G->thez[l] = G->thez[l - 1] + g->dthe;
+    for (int j = 0; j < g->jm; j++) {
+        G->sin_tbl[j] = dmax(p->sinthe_min, fabs(sin(G->thez[j])));
+        G->cos_tbl[j] = cos(G->thez[j]);
+    }
+    G->inv_2dr = 1.0 / (2.0 * g->dr);
+    G->inv_2dthe = 1.0 / (2.0 * g->dthe);
+    G->inv_2dphi = 1.0 / (2.0 * g->dphi);
+    G->inv_dr2 = 1.0 / (g->dr * g->dr);
+    G->inv_dthe2 = 1.0 / (g->dthe * g->dthe);
+    G->inv_dphi2 = 1.0 / (g->dphi * g->dphi);
+}
+
+static void geo_free(orc_geo* G) {
+    free(G->radz); free(G->thez); free(G->sin_tbl); free(G->cos_tbl);
+}
+
+static cellgeo cell_geo(const orc_geo* G, const atu_params* p, int i, int j) {
+    cellgeo c;
+    c.rm = metric_radius(p, G->radz[i]);
+    c.rm2 = c.rm * c.rm;
+    c.sinthe = G->sin_tbl[j];
+    c.sinthe2 = c.sinthe * c.sinthe;
+    c.costhe = G->cos_tbl[j];
+    c.cotanthe = c.costhe / c.sinthe;
+    c.inv_rm = 1.0 / c.rm;
+    c.inv_rm2 = 1.0 / c.rm2;
+    c.inv_rmsinthe = 1.0 / (c.rm * c.sinthe);
+    c.inv_rm2sinthe = c.inv_rm2 / c.sinthe;
+    c.inv_rm2sinthe2 = c.inv_rm2 / c.sinthe2;
+    c.costhe_inv_rm2sinthe = c.costhe * c.inv_rm2sinthe;
+    return c;
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* RHS of one cell. `st` points at the 19 stage fields, `fz` at the 21 frozen fields, `out` at   */
+/* 22 output fields (19 rhs + aux_u, aux_v, aux_w).                                              */
+/* ------------------------------------------------------------------------------------------- */
+typedef struct {
+    const double* st[ATU_NPROG];
+    const double* fz[ATU_NFROZEN];
+    const double *pdyn, *phydro;
+    double* out[ATU_NPROG + 3];
+} rhs_io;
+
+static void rhs_cell(const orc_geo* G, const atu_params* P, const rhs_io* io, int i, int j, int k,
+                     const cellgeo* geo) {
+    const int jm = G->jm, km = G->km, im = G->im;
+    const int n = km - 1;
+    const int kw = (k - 1 < 0) ? k - 1 + n : k - 1;
+    const int ke = (k + 1 >= n) ? k + 1 - n : k + 1;
+    const size_t c = IDX(i, j, k);
+    const size_t cip = IDX(i + 1, j, k), cim = IDX(i - 1, j, k);
+    const size_t cjp = IDX(i, j + 1, k), cjm = IDX(i, j - 1, k);
+    const size_t cke = IDX(i, j, ke), ckw = IDX(i, j, kw);
+
+    double d1r[ATU_NPROG], d2r[ATU_NPROG], d1t[ATU_NPROG], d2t[ATU_NPROG], d1p[ATU_NPROG], d2p[ATU_NPROG];
+    for (int f = 0; f < ATU_NPROG; f++) {
+        const double* F = io->st[f];
+        d1r[f] = (F[cip] - F[cim]) * G->inv_2dr;
+        d2r[f] = (F[cip] - 2.0 * F[c] + F[cim]) * G->inv_dr2;
+        d1t[f] = (F[cjp] - F[cjm]) * G->inv_2dthe;
+        d2t[f] = (F[cjp] - 2.0 * F[c] + F[cjm]) * G->inv_dthe2;
+        d1p[f] = (F[cke] - F[ckw]) * G->inv_2dphi;
+        d2p[f] = (F[cke] - 2.0 * F[c] + F[ckw]) * G->inv_dphi2;
+    }
+    const double* pd = io->pdyn;
+    const double dpdr = (pd[cip] - pd[cim]) * G->inv_2dr;
+    const double dpdthe = (pd[cjp] - pd[cjm]) * G->inv_2dthe;
+    const double dpdphi = (pd[cke] - pd[ckw]) * G->inv_2dphi;
+
+    const double u = io->st[ATU_U][c], v = io->st[ATU_V][c], w = io->st[ATU_W][c];
+    const double T = io->st[ATU_T][c];
+
+    /* body forces: true sine (CORIOLIS_TRUE_SINE), radial Coriolis and centrifugal inert */
+    const double sinthe_bf = sqrt(dmax(0.0, 1.0 - geo->costhe * geo->costhe));
+    const double Coriolis_the = -2.0 * P->omega * geo->costhe * w;
+    const double Coriolis_phi = +2.0 * P->omega * (+1.0 * geo->costhe * v + sinthe_bf * u);
+
+    const double coeff_energy_p = P->u_0 * P->u_0 / (P->cp_mix * P->t_ref);
+    const double scale_p = 1.0e5 / (P->r_mix * P->u_0 * P->u_0);
+    const double scale_Cor = P->coriolis_scale;
+
+    const double v_invrm = v * geo->inv_rm;
+    const double w_invrs = w * geo->inv_rmsinthe;
+    double pressure_t = coeff_energy_p * scale_p * (u * dpdr + v_invrm * dpdthe + w_invrs * dpdphi);
+    if ((P->cp_mode > 0 || P->cp_gas) && T > 0.0) pressure_t *= P->cp_mix / orc_cp_lapse(P, T * P->t_ref);
+
+    double tr[ATU_NPROG], df[ATU_NPROG];
+    const double two_inv_rm = 2.0 * geo->inv_rm;
+    const double v_metric = (1.0 + geo->costhe / geo->sinthe2) * geo->inv_rm2;
+    for (int f = 0; f < ATU_NPROG; f++) {
+        tr[f] = u * d1r[f] + v_invrm * d1t[f] + w_invrs * d1p[f];
+        df[f] = d2r[f] + d1r[f] * two_inv_rm + d2t[f] * geo->inv_rm2 + d1t[f] * geo->costhe_inv_rm2sinthe
+                + d2p[f] * geo->inv_rm2sinthe2;
+    }
+    /* momentum transport with metric terms (METRIC_SPLIT: the radial metric term is in p_hydro) */
+    const double transport_v = tr[ATU_V] + (u * v - w * w * geo->cotanthe) * geo->inv_rm;
+    const double transport_w = tr[ATU_W] + (w * u + v * w * geo->cotanthe) * geo->inv_rm;
+    const double diffusion_u = d2r[ATU_U] + 2.0 * u * geo->inv_rm2 + d2t[ATU_U] * geo->inv_rm2
+                               + 4.0 * d1r[ATU_U] * geo->inv_rm + d1t[ATU_U] * geo->costhe_inv_rm2sinthe
+                               + d2p[ATU_U] * geo->inv_rm2sinthe2;
+    const double diffusion_v = d2r[ATU_V] + d1r[ATU_V] * two_inv_rm + d2t[ATU_V] * geo->inv_rm2
+                               + d1t[ATU_V] * geo->costhe_inv_rm2sinthe - v_metric * v
+                               + d2p[ATU_V] * geo->inv_rm2sinthe2 + 2.0 * d1t[ATU_U] * geo->inv_rm2
+                               - d1p[ATU_W] * 2.0 * geo->costhe * geo->inv_rm2sinthe2;
+    const double diffusion_w = d2r[ATU_W] + d1r[ATU_W] * two_inv_rm + d2t[ATU_W] * geo->inv_rm2
+                               + d1t[ATU_W] * geo->costhe_inv_rm2sinthe - v_metric * w
+                               + d2p[ATU_W] * geo->inv_rm2sinthe2 + 2.0 * d1p[ATU_U] * geo->inv_rm2sinthe
+                               + d1p[ATU_V] * 2.0 * geo->costhe * geo->inv_rm2sinthe2;
+
+    const double dpdr_term = dpdr * scale_p;
+    const double dpdthe_term = dpdthe * scale_p * geo->inv_rm;
+    const double dpdphi_term = dpdphi * scale_p * geo->inv_rmsinthe;
+
+    const double re_eff = P->re / (1.0 + 2.5 * 0.0);
+    const double nue = io->fz[ATU_NUE - ATU_NUE][c];
+    const double nue_t = (P->turb_coupling != 0.0 && isfinite(nue)) ? P->turb_coupling * dmax(0.0, nue) : 0.0;
+    const double nue_t_s = nue_t / P->Pr_t;
+
+    double precip_t = P->precip_coupling * io->fz[ATU_Q_PRECIP - ATU_NUE][c] * (P->L_atm * 1.0e3)
+                      / (P->r_mix * P->cp_mix * P->u_0 * P->t_ref);
+    if (!isfinite(precip_t)) precip_t = 0.0;
+    else if (precip_t > 0.5) precip_t = 0.5;
+    else if (precip_t < -0.5) precip_t = -0.5;
+
+    io->out[ATU_T][c] = +pressure_t - tr[ATU_T] + df[ATU_T] / (P->re * P->pr) + df[ATU_T] * nue_t_s + precip_t;
+
+    const double frac_sp = dmax(0.0, (double)(i - (im - 1) * 3 / 4) / (double)((im - 1) / 4));
+    const double sponge = P->alpha_sponge * frac_sp * frac_sp;
+
+    const double dphdthe_term = (io->phydro[cjp] - io->phydro[cjm]) * G->inv_2dthe * geo->inv_rm;
+    const double dphdphi_term = (io->phydro[cke] - io->phydro[ckw]) * G->inv_2dphi * geo->inv_rmsinthe;
+
+    const double rhs_u = -dpdr_term - tr[ATU_U] + diffusion_u / re_eff + diffusion_u * nue_t - sponge * u;
+    const double rhs_v = -dpdthe_term - dphdthe_term - transport_v + diffusion_v / re_eff + diffusion_v * nue_t
+                         - P->Coriolis * scale_Cor * Coriolis_the;
+    const double rhs_w = -dpdphi_term - dphdphi_term - transport_w + diffusion_w / re_eff + diffusion_w * nue_t
+                         - P->Coriolis * scale_Cor * Coriolis_phi;
+    io->out[ATU_U][c] = rhs_u;
+    io->out[ATU_V][c] = rhs_v;
+    io->out[ATU_W][c] = rhs_w;
+
+    /* species: f, Schmidt number, precipitation source, reaction mass flux (or -1) */
+    static const int sp_f[12] = {ATU_CH4, ATU_CH4_CLOUD, ATU_CH4_ICE, ATU_H2O, ATU_H2O_CLOUD, ATU_H2O_ICE,
+                                 ATU_H2S, ATU_H2S_CLOUD, ATU_H2S_ICE, ATU_NH3, ATU_NH3_CLOUD, ATU_NH3_ICE};
+    static const int sp_s[12] = {ATU_SP_CH4, ATU_SP_CH4_CLOUD, ATU_SP_CH4_ICE, ATU_SP_H2O, ATU_SP_H2O_CLOUD,
+                                 ATU_SP_H2O_ICE, ATU_SP_H2S, ATU_SP_H2S_CLOUD, ATU_SP_H2S_ICE, ATU_SP_NH3,
+                                 ATU_SP_NH3_CLOUD, ATU_SP_NH3_ICE};
+    const double sc[12] = {P->sc_ch4, P->sc_ch4, P->sc_ch4, P->sc_h2o, P->sc_h2o, P->sc_h2o,
+                           P->sc_h2s, P->sc_h2s, P->sc_h2s, P->sc_nh3, P->sc_nh3, P->sc_nh3};
+    for (int s = 0; s < 12; s++) {
+        const int f = sp_f[s];
+        double r = -tr[f] + df[f] / (sc[s] * P->re) + df[f] * nue_t_s;
+        if (f == ATU_H2S) r = r + P->chemical_reaction * io->fz[ATU_MF_H2S - ATU_NUE][c];
+        if (f == ATU_NH3) r = r + P->chemical_reaction * io->fz[ATU_MF_NH3 - ATU_NUE][c];
+        r = r + P->precip_coupling * io->fz[sp_s[s] - ATU_NUE][c];
+        io->out[f][c] = r;
+    }
+    const double v_stokes = (2.0 / 9.0) * P->r_p_nh4sh * P->r_p_nh4sh
+                            * (P->rho_cond_nh4sh - io->fz[ATU_RHO_MIX - ATU_NUE][c]) * P->g / P->mue_mix;
+    io->out[ATU_NH4SH][c] = -tr[ATU_NH4SH] + io->fz[ATU_FLUXLIM_NH4SH - ATU_NUE][c]
+                            + df[ATU_NH4SH] / (P->sc_nh4sh * P->re) + df[ATU_NH4SH] * nue_t_s
+                            + P->chemical_reaction * io->fz[ATU_MF_NH4SH - ATU_NUE][c]
+                            + (v_stokes / P->u_0) * d1r[ATU_NH4SH];
+
+    if (P->turb_active) {
+        const double sig_k = P->sig_k, sig_w = P->sig_w;
+        for (int q = 0; q < 2; q++) {
+            const int f = q == 0 ? ATU_TKE : ATU_DIS;
+            const double transport = u * d1r[f] + v * d1t[f] * geo->inv_rm + w * d1p[f] * geo->inv_rmsinthe;
+            const double diffusion = d2r[f] + d1r[f] * 2.0 * geo->inv_rm + d2t[f] * geo->inv_rm2
+                                     + d1t[f] * geo->costhe_inv_rm2sinthe + d2p[f] * geo->inv_rm2sinthe2;
+            const double src = io->fz[(q == 0 ? ATU_TKE_SOURCE : ATU_DIS_SOURCE) - ATU_NUE][c];
+            io->out[f][c] = -transport + diffusion * (1.0 / P->re + nue / (q == 0 ? sig_k : sig_w)) + src;
+        }
+    } else {
+        io->out[ATU_TKE][c] = 0.0;
+        io->out[ATU_DIS][c] = 0.0;
+    }
+    io->out[ATU_NPROG + 0][c] = rhs_u + dpdr_term;
+    io->out[ATU_NPROG + 1][c] = rhs_v + dpdthe_term;
+    io->out[ATU_NPROG + 2][c] = rhs_w + dpdphi_term;
+}
+
+static void set_io(rhs_io* io, double* fields, double* const* stage, double* out, size_t n3) {
+    for (int f = 0; f < ATU_NPROG; f++) io->st[f] = stage[f];
+    for (int f = 0; f < ATU_NFROZEN; f++) io->fz[f] = fields + (size_t)(ATU_NUE + f) * n3;
+    io->pdyn = fields + (size_t)ATU_P_DYN * n3;
+    io->phydro = fields + (size_t)ATU_P_HYDRO * n3;
+    for (int f = 0; f < ATU_NPROG + 3; f++) io->out[f] = out + (size_t)f * n3;
+}
+
+/* One RHS evaluation on the current state into out (22 dense fields: 19 rhs + aux u, v, w). */
+int orc_rhs(const atu_grid* g, const atu_params* p, double* fields, double* out) {
+    orc_geo G;
+    geo_init(&G, g, p);
+    const int im = g->im, jm = g->jm, km = g->km;
+    const size_t n3 = (size_t)im * jm * km;
+    double* stage[ATU_NPROG];
+    for (int f = 0; f < ATU_NPROG; f++) stage[f] = fields + (size_t)f * n3;
+    rhs_io io;
+    set_io(&io, fields, stage, out, n3);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++) {
+            cellgeo cg = cell_geo(&G, p, i, j);
+            for (int k = 0; k < km - 1; k++) rhs_cell(&G, p, &io, i, j, k, &cg);
+        }
+    geo_free(&G);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* RK4 (planet/RungeKutta_Uran_Turb.cpp:179-404); aux_* of the last stage land in fields.       */
+/* ------------------------------------------------------------------------------------------- */
+int orc_rk4(const atu_grid* g, const atu_params* p, double* fields) {
+    orc_geo G;
+    geo_init(&G, g, p);
+    const int im = g->im, jm = g->jm, km = g->km;
+    const size_t n3 = (size_t)im * jm * km;
+    double* yn = malloc(sizeof(double) * n3 * ATU_NPROG);
+    double* acc = malloc(sizeof(double) * n3 * ATU_NPROG);
+    double* rhs = malloc(sizeof(double) * n3 * (ATU_NPROG + 3));
+    if (!yn || !acc || !rhs) { free(yn); free(acc); free(rhs); geo_free(&G); return -3; }
+    memcpy(yn, fields, sizeof(double) * n3 * ATU_NPROG);          /* restoreVar */
+    memset(rhs, 0, sizeof(double) * n3 * (ATU_NPROG + 3));
+    double* stage[ATU_NPROG];
+    for (int f = 0; f < ATU_NPROG; f++) stage[f] = fields + (size_t)f * n3;
+    rhs_io io;
+    set_io(&io, fields, stage, rhs, n3);
+    /* aux: keep the model's arrays (their planes carry the previous projection prep) */
+    for (int q = 0; q < 3; q++)
+        memcpy(rhs + (size_t)(ATU_NPROG + q) * n3, fields + (size_t)(ATU_AUX_U + q) * n3, sizeof(double) * n3);
+
+    const double dt = p->dt;
+    const double tke_max = p->tke_max_m2s2 / (p->u_0 * p->u_0), dis_min = p->dis_min_nd;
+    const double t_min = p->t_min_K / p->t_ref, t_max = p->t_max_K / p->t_ref;
+    for (int stage_n = 0; stage_n < 4; stage_n++) {
+        const double c_in = (stage_n == 0 || stage_n == 1) ? 0.5 * dt : (stage_n == 2 ? dt : 0.0);
+        const double wgt = (stage_n == 0 || stage_n == 3) ? 1.0 : 2.0;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 1; i < im - 1; i++)
+            for (int j = 1; j < jm - 1; j++) {
+                cellgeo cg = cell_geo(&G, p, i, j);
+                for (int k = 0; k < km - 1; k++) rhs_cell(&G, p, &io, i, j, k, &cg);
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 1; i < im - 1; i++)
+            for (int j = 1; j < jm - 1; j++)
+                for (int k = 0; k < km - 1; k++) {
+                    const size_t c = IDX(i, j, k);
+                    for (int f = 0; f < ATU_NPROG; f++) {
+                        if (!p->turb_active && (f == ATU_TKE || f == ATU_DIS)) continue;
+                        double* A = acc + (size_t)f * n3;
+                        const double r = rhs[(size_t)f * n3 + c];
+                        A[c] = (stage_n == 0) ? wgt * r : A[c] + wgt * r;
+                        if (stage_n < 3) {
+                            double y = yn[(size_t)f * n3 + c] + c_in * r;
+                            if (f == ATU_TKE) y = dclamp(y, 0.0, tke_max);
+                            if (f == ATU_DIS) y = dmax(y, dis_min);
+                            fields[(size_t)f * n3 + c] = y;
+                        }
+                    }
+                }
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < km - 1; k++) {
+                const size_t c = IDX(i, j, k);
+                for (int f = 0; f < ATU_NPROG; f++) {
+                    if (!p->turb_active && (f == ATU_TKE || f == ATU_DIS)) continue;
+                    double y = yn[(size_t)f * n3 + c] + dt * acc[(size_t)f * n3 + c] / 6.0;
+                    if (f == ATU_T) y = dclamp(y, t_min, t_max);
+                    if (f == ATU_TKE) y = dclamp(dmax(y, 0.0), 0.0, tke_max);
+                    if (f == ATU_DIS) y = dmax(y, dis_min);
+                    fields[(size_t)f * n3 + c] = y;
+                }
+            }
+    for (int q = 0; q < 3; q++)
+        memcpy(fields + (size_t)(ATU_AUX_U + q) * n3, rhs + (size_t)(ATU_NPROG + q) * n3, sizeof(double) * n3);
+    free(yn); free(acc); free(rhs);
+    geo_free(&G);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* hydrostatic prep (planet/RungeKutta_Uran_Turb.cpp:18-177)                                    */
+/* ------------------------------------------------------------------------------------------- */
+int orc_hydro(const atu_grid* g, const atu_params* p, double* fields, double* level) {
+    orc_geo G;
+    geo_init(&G, g, p);
+    const int im = g->im, jm = g->jm, km = g->km;
+    const size_t n3 = (size_t)im * jm * km;
+    const double* t = fields + (size_t)ATU_T * n3;
+    const double* v = fields + (size_t)ATU_V * n3;
+    const double* w = fields + (size_t)ATU_W * n3;
+    const double* ps = fields + (size_t)ATU_P_STAT * n3;
+    double* ph = fields + (size_t)ATU_P_HYDRO * n3;
+    double* ref = malloc(sizeof(double) * im);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < im; i++) {
+        double sum = 0.0, wsum = 0.0;
+        for (int j = 0; j < jm; j++) {
+            const double wgt = sin(G.thez[j]);
+            for (int k = 0; k < km; k++) {
+                const double T = t[IDX(i, j, k)];
+                if (!(T > 0.0)) continue;
+                const double b = p->g * ps[IDX(i, j, k)] / (p->r_mix * p->R_mix * T * p->t_ref);
+                if (!isfinite(b)) continue;
+                sum += wgt * b;
+                wsum += wgt;
+            }
+        }
+        ref[i] = (wsum > 0.0) ? sum / wsum : 0.0;
+    }
+    const double nd_rel = p->g * (p->L_atm * 1.0e3) / (p->u_0 * p->u_0);
+    const double dr = g->dr;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < jm; j++)
+        for (int k = 0; k < km; k++) {
+            const double cor = p->Coriolis * p->coriolis_scale * 2.0 * p->omega * fabs(sin(G.thez[j]));
+            double fprev = 0.0;
+            for (int i = 0; i < im; i++) {
+                const size_t x = IDX(i, j, k);
+                double buoy = 0.0;
+                if (t[x] > 0.0) {
+                    const double b = p->g * ps[x] / (p->r_mix * p->R_mix * t[x] * p->t_ref) - ref[i];
+                    const double f = -nd_rel * 1.0 * p->buoyancy * b / ref[i];
+                    buoy = isfinite(f) ? f : 0.0;
+                }
+                const double fi = buoy + (v[x] * v[x] + w[x] * w[x]) / metric_radius(p, G.radz[i]) + cor * w[x];
+                if (i == 0) ph[x] = 0.0;
+                else ph[x] = ph[IDX(i - 1, j, k)] + 0.5 * (fprev + fi) * dr;
+                fprev = fi;
+            }
+        }
+    if (level) memcpy(level, ref, sizeof(double) * im);
+    free(ref);
+    geo_free(&G);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* domain BC on t, u, v, w and the 13 species                                                    */
+/* ------------------------------------------------------------------------------------------- */
+int orc_bc(const atu_grid* g, const atu_params* p, double* fields) {
+    const int im = g->im, jm = g->jm, km = g->km;
+    const size_t n3 = (size_t)im * jm * km;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    /* radius list: t, u, v, w and the species (the prognostic part of bc_fields_radius) */
+#pragma omp parallel for
+    for (int j = 0; j < jm; j++)
+        for (int k = 0; k < km; k++)
+            for (int f = 0; f <= ATU_NH4SH; f++) {
+                double* F = fields + (size_t)f * n3;
+                F[IDX(0, j, k)] = c43 * F[IDX(1, j, k)] - c13 * F[IDX(2, j, k)];
+                F[IDX(im - 1, j, k)] = c43 * F[IDX(im - 2, j, k)] - c13 * F[IDX(im - 3, j, k)];
+            }
+    if (p->rigid_lid) {
+        double* U = fields + (size_t)ATU_U * n3;
+        for (int j = 0; j < jm; j++)
+            for (int k = 0; k < km; k++) { U[IDX(0, j, k)] = 0.0; U[IDX(im - 1, j, k)] = 0.0; }
+    }
+    if (p->radius_positive) {
+        for (int j = 0; j < jm; j++)
+            for (int k = 0; k < km; k++)
+                for (int f = ATU_CH4; f <= ATU_NH4SH; f++) {
+                    double* F = fields + (size_t)f * n3;
+                    if (F[IDX(0, j, k)] < 0.0)
+                        F[IDX(0, j, k)] = (p->radius_positive == 2) ? dmax(0.0, F[IDX(1, j, k)]) : 0.0;
+                    if (F[IDX(im - 1, j, k)] < 0.0)
+                        F[IDX(im - 1, j, k)] = (p->radius_positive == 2) ? dmax(0.0, F[IDX(im - 2, j, k)]) : 0.0;
+                }
+    }
+    if (p->lid_vw == 2 || p->lid_vw == 3) {
+        const int a = im - 2, b = im - 3, c = im - 4, l = im - 1;
+        for (int q = 0; q < 2; q++) {
+            double* F = fields + (size_t)(q == 0 ? ATU_V : ATU_W) * n3;
+            for (int j = 0; j < jm; j++)
+                for (int k = 0; k < km; k++)
+                    F[IDX(l, j, k)] = (p->lid_vw == 2) ? 2.0 * F[IDX(a, j, k)] - F[IDX(b, j, k)]
+                                                       : F[IDX(c, j, k)] - 3.0 * F[IDX(b, j, k)] + 3.0 * F[IDX(a, j, k)];
+        }
+    }
+    /* poles: v, w zeroed; t, u, species 2-point Neumann */
+#pragma omp parallel for
+    for (int k = 0; k < km; k++)
+        for (int i = 0; i < im; i++) {
+            for (int q = 0; q < 2; q++) {
+                double* F = fields + (size_t)(q == 0 ? ATU_V : ATU_W) * n3;
+                F[IDX(i, 0, k)] = 0.0;
+                F[IDX(i, jm - 1, k)] = 0.0;
+            }
+            for (int f = 0; f <= ATU_NH4SH; f++) {
+                if (f == ATU_V || f == ATU_W) continue;
+                double* F = fields + (size_t)f * n3;
+                F[IDX(i, 0, k)] = c43 * F[IDX(i, 1, k)] - c13 * F[IDX(i, 2, k)];
+                F[IDX(i, jm - 1, k)] = c43 * F[IDX(i, jm - 2, k)] - c13 * F[IDX(i, jm - 3, k)];
+            }
+        }
+    /* periodic seam */
+    for (int i = 0; i < im; i++)
+        for (int j = 0; j < jm; j++)
+            for (int f = 0; f <= ATU_NH4SH; f++) {
+                double* F = fields + (size_t)f * n3;
+                F[IDX(i, j, km - 1)] = F[IDX(i, j, 0)];
+            }
+    if (p->species_clamp) {
+        for (int f = ATU_CH4; f <= ATU_NH4SH; f++) {
+            double* F = fields + (size_t)f * n3;
+            for (size_t q = 0; q < n3; q++)
+                if (!(F[q] >= 0.0)) F[q] = 0.0;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* pressure: projection prep, source, red-black sweeps with the wall passes after each sweep     */
+/* ------------------------------------------------------------------------------------------- */
+typedef struct {
+    double *num2, *num3, *den, *inv_rm, *inv_rmsinthe, num1;
+} orc_pgeo;
+
+static void pgeo_init(orc_pgeo* q, const orc_geo* G, const atu_params* p) {
+    const int im = G->im, jm = G->jm;
+    q->num2 = malloc(sizeof(double) * im);
+    q->inv_rm = malloc(sizeof(double) * im);
+    q->num3 = malloc(sizeof(double) * im * jm);
+    q->den = malloc(sizeof(double) * im * jm);
+    q->inv_rmsinthe = malloc(sizeof(double) * im * jm);
+    double* st = malloc(sizeof(double) * jm);
+    for (int j = 0; j < jm; j++) {
+        st[j] = sin(G->thez[j]);
+        if (st[j] < p->sinthe_min) st[j] = p->sinthe_min;
+        if (st[j] == 0.0) st[j] = 1.0e-5;
+    }
+    const double exp_rm = 1.0, exp_2_rm = exp_rm * exp_rm;
+    q->num1 = exp_2_rm * G->inv_dr2;
+    for (int i = 0; i < im; i++) {
+        const double rm = metric_radius(p, G->radz[i]);
+        const double rm2 = rm * rm;
+        const double inv_rm2 = 1.0 / rm2;
+        q->inv_rm[i] = 1.0 / rm;
+        q->num2[i] = inv_rm2 * G->inv_dthe2;
+        for (int j = 0; j < jm; j++) {
+            const double s2 = st[j] * st[j];
+            const double n3 = (inv_rm2 / s2) * G->inv_dphi2;
+            q->num3[i * jm + j] = n3;
+            q->den[i * jm + j] = 2.0 * q->num1 + 2.0 * q->num2[i] + 2.0 * n3;
+            q->inv_rmsinthe[i * jm + j] = 1.0 / (rm * st[j]);
+        }
+    }
+    free(st);
+}
+
+static void pgeo_free(orc_pgeo* q) {
+    free(q->num2); free(q->num3); free(q->den); free(q->inv_rm); free(q->inv_rmsinthe);
+}
+
+/* prepareProjectionBoundaries with the rigid lid, then the clamped div(aux) source into src. */
+static void proj_prep_src(const orc_geo* G, const orc_pgeo* q, const atu_params* p, double* fields, double* src) {
+    const int im = G->im, jm = G->jm, km = G->km;
+    const size_t n3 = (size_t)im * jm * km;
+    double* au = fields + (size_t)ATU_AUX_U * n3;
+    double* av = fields + (size_t)ATU_AUX_V * n3;
+    double* aw = fields + (size_t)ATU_AUX_W * n3;
+    for (int j = 1; j < jm - 1; j++)
+        for (int k = 0; k < km - 1; k++) {
+            au[IDX(0, j, k)] = 0.0;
+            au[IDX(im - 1, j, k)] = 0.0;
+            double* A[2] = {av, aw};
+            for (int s = 0; s < 2; s++) {
+                double* a = A[s];
+                a[IDX(0, j, k)] = a[IDX(3, j, k)] - 3.0 * a[IDX(2, j, k)] + 3.0 * a[IDX(1, j, k)];
+                a[IDX(im - 1, j, k)] = a[IDX(im - 4, j, k)] - 3.0 * a[IDX(im - 3, j, k)] + 3.0 * a[IDX(im - 2, j, k)];
+            }
+        }
+    for (int k = 0; k < km - 1; k++)
+        for (int i = 1; i < im - 1; i++) {
+            au[IDX(i, 0, k)] = au[IDX(i, 3, k)] - 3.0 * au[IDX(i, 2, k)] + 3.0 * au[IDX(i, 1, k)];
+            av[IDX(i, 0, k)] = 0.0;
+            aw[IDX(i, 0, k)] = 0.0;
+            au[IDX(i, jm - 1, k)] = au[IDX(i, jm - 4, k)] - 3.0 * au[IDX(i, jm - 3, k)] + 3.0 * au[IDX(i, jm - 2, k)];
+            av[IDX(i, jm - 1, k)] = 0.0;
+            aw[IDX(i, jm - 1, k)] = 0.0;
+        }
+    for (int i = 0; i < im; i++)
+        for (int j = 0; j < jm; j++) {
+            au[IDX(i, j, km - 1)] = au[IDX(i, j, 0)];
+            av[IDX(i, j, km - 1)] = av[IDX(i, j, 0)];
+            aw[IDX(i, j, km - 1)] = aw[IDX(i, j, 0)];
+        }
+    const int n = km - 1;
+    memset(src, 0, sizeof(double) * n3);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < n; k++) {
+                const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+                const double du_dr = (au[IDX(i + 1, j, k)] - au[IDX(i - 1, j, k)]) * G->inv_2dr;
+                const double dv_dthe = (av[IDX(i, j + 1, k)] - av[IDX(i, j - 1, k)]) * G->inv_2dthe;
+                const double dw_dphi = (aw[IDX(i, j, ke)] - aw[IDX(i, j, kw)]) * G->inv_2dphi;
+                double div_src = du_dr * 1.0 + dv_dthe * q->inv_rm[i] + dw_dphi * q->inv_rmsinthe[i * jm + j];
+                const double src_max = q->den[i * jm + j] * p->p_dyn_cap;
+                if (!isfinite(div_src)) div_src = 0.0;
+                else if (div_src > src_max) div_src = src_max;
+                else if (div_src < -src_max) div_src = -src_max;
+                src[IDX(i, j, k)] = div_src;
+            }
+}
+
+static void wall_passes(int im, int jm, int km, double* pd) {
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    for (int k = 0; k < km; k++)
+        for (int j = 0; j < jm; j++) {
+            pd[IDX(0, j, k)] = c43 * pd[IDX(1, j, k)] - c13 * pd[IDX(2, j, k)];
+            pd[IDX(im - 1, j, k)] = c43 * pd[IDX(im - 2, j, k)] - c13 * pd[IDX(im - 3, j, k)];
+        }
+    for (int k = 0; k < km; k++)
+        for (int i = 0; i < im; i++) {
+            pd[IDX(i, 0, k)] = c43 * pd[IDX(i, 1, k)] - c13 * pd[IDX(i, 2, k)];
+            pd[IDX(i, jm - 1, k)] = c43 * pd[IDX(i, jm - 2, k)] - c13 * pd[IDX(i, jm - 3, k)];
+        }
+    for (int i = 0; i < im; i++)
+        for (int j = 0; j < jm; j++) pd[IDX(i, j, km - 1)] = pd[IDX(i, j, 0)];
+}
+
+/* `sweeps` red-black sweeps at relaxation factor omega; src_out (optional, dense) receives the
+ * clamped source. */
+int orc_pressure(const atu_grid* g, const atu_params* p, double* fields, int sweeps, double omega, double* src_out) {
+    orc_geo G;
+    geo_init(&G, g, p);
+    orc_pgeo q;
+    pgeo_init(&q, &G, p);
+    const int im = g->im, jm = g->jm, km = g->km, n = km - 1;
+    const size_t n3 = (size_t)im * jm * km;
+    double* src = src_out ? src_out : malloc(sizeof(double) * n3);
+    proj_prep_src(&G, &q, p, fields, src);
+    double* pd = fields + (size_t)ATU_P_DYN * n3;
+    if (sweeps < 1) sweeps = 1;
+    for (int s = 0; s < sweeps; s++) {
+        for (int colour = 0; colour < 2; colour++) {
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int i = 1; i < im - 1; i++)
+                for (int j = 1; j < jm - 1; j++) {
+                    const double num1 = q.num1, num2 = q.num2[i], num3 = q.num3[i * jm + j], den = q.den[i * jm + j];
+                    for (int k = 0; k < n; k++) {
+                        if (((i + j + k) & 1) != colour) continue;
+                        const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+                        const double acc = (pd[IDX(i + 1, j, k)] + pd[IDX(i - 1, j, k)]) * num1
+                                           + (pd[IDX(i, j + 1, k)] + pd[IDX(i, j - 1, k)]) * num2
+                                           + (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)]) * num3;
+                        double pn = (acc - src[IDX(i, j, k)]) / den;
+                        if (omega != 1.0) pn = (1.0 - omega) * pd[IDX(i, j, k)] + omega * pn;
+                        pd[IDX(i, j, k)] = pn;
+                    }
+                }
+        }
+        wall_passes(im, jm, km, pd);
+    }
+    if (!src_out) free(src);
+    pgeo_free(&q);
+    geo_free(&G);
+    return 0;
+}
+
+/* ||src - Lc p||_2 and ||src||_2 over the interior, Lc p = acc - den p with the stored walls. */
+int orc_residual(const atu_grid* g, const atu_params* p, const double* fields, const double* src, double* rnorm,
+                 double* snorm) {
+    orc_geo G;
+    geo_init(&G, g, p);
+    orc_pgeo q;
+    pgeo_init(&q, &G, p);
+    const int im = g->im, jm = g->jm, km = g->km, n = km - 1;
+    const size_t n3 = (size_t)im * jm * km;
+    const double* pd = fields + (size_t)ATU_P_DYN * n3;
+    double rr = 0.0, ss = 0.0;
+#pragma omp parallel for collapse(2) reduction(+ : rr, ss) schedule(static)
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < n; k++) {
+                const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+                const double acc = (pd[IDX(i + 1, j, k)] + pd[IDX(i - 1, j, k)]) * q.num1
+                                   + (pd[IDX(i, j + 1, k)] + pd[IDX(i, j - 1, k)]) * q.num2[i]
+                                   + (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)]) * q.num3[i * jm + j];
+                const double s = src[IDX(i, j, k)];
+                const double r = s - (acc - q.den[i * jm + j] * pd[IDX(i, j, k)]);
+                rr += r * r;
+                ss += s * s;
+            }
+    *rnorm = sqrt(rr);
+    *snorm = sqrt(ss);
+    pgeo_free(&q);
+    geo_free(&G);
+    return 0;
+}
+
+/* The benchmark loop: hydro prep, RK4, BC, `sweeps` red-black sweeps at omega, nsteps times. */
+int orc_step(const atu_grid* g, const atu_params* p, double* fields, int nsteps, int sweeps, double omega) {
+    for (int s = 0; s < nsteps; s++) {
+        int e;
+        if ((e = orc_hydro(g, p, fields, NULL))) return e;
+        if ((e = orc_rk4(g, p, fields))) return e;
+        if ((e = orc_bc(g, p, fields))) return e;
+        if ((e = orc_pressure(g, p, fields, sweeps, omega, NULL))) return e;
+    }
+    return 0;
+}
