@@ -248,7 +248,7 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
 }
 
 int do_bc(atu_ctx* c) {
-    BcTargets t{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr};
+    BcTargets t{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
     return c->hip(c->timed(PC_BC, [&] { return launch_bc(c->sh, c->params, t, c->stream); }), "bc");
 }
 

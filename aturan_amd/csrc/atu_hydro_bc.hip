@@ -105,6 +105,15 @@ __device__ __forceinline__ void bc_put(const BcCtx& b, size_t q, double val) {
 }
 
 constexpr int kNumBcFields = 17;   // t, u, v, w and the 13 species: the prognostic part of the lists
+// the frozen inputs the lists name: the reaction mass fluxes and the NH4SH flux limiter
+__device__ __forceinline__ int frozen_bc(int q) {
+    return q == 0 ? ATU_MF_H2S - ATU_NUE
+                  : (q == 1 ? ATU_MF_NH3 - ATU_NUE : (q == 2 ? ATU_MF_NH4SH - ATU_NUE : ATU_FLUXLIM_NH4SH - ATU_NUE));
+}
+__device__ __forceinline__ void fz_put(const BcCtx& b, size_t q, double val) {
+    b.t.frozen[q] = val;
+    if (b.t.frozen32) b.t.frozen32[q] = (float)val;
+}
 
 // bcRadius with margin 0 (every j, every k), NEUMANN form, then the rigid lid on u, the species
 // floor (floorRadialSpecies) and the lid form of v and w (BC_LID_VW).
@@ -122,6 +131,14 @@ __global__ void __launch_bounds__(256) k_bc_radius(BcCtx b) {
         const double hi = b.c43 * F[(size_t)(L - 1) * P] - b.c13 * F[(size_t)(L - 2) * P];
         bc_put(b, (size_t)f * sh.fs + col, lo);
         bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, hi);
+    }
+    for (int q = 0; q < 4; q++) {
+        const size_t o = (size_t)frozen_bc(q) * sh.fs + col;
+        const double* F = b.t.frozen + o;
+        const double lo = b.c43 * F[1 * P] - b.c13 * F[2 * P];
+        const double hi = b.c43 * F[(size_t)(L - 1) * P] - b.c13 * F[(size_t)(L - 2) * P];
+        fz_put(b, o, lo);
+        fz_put(b, o + (size_t)L * P, hi);
     }
     if (b.rigid_lid) {
         bc_put(b, (size_t)ATU_U * sh.fs + col, 0.0);
@@ -174,6 +191,11 @@ __global__ void __launch_bounds__(256) k_bc_theta(BcCtx b) {
         bc_put(b, q0, lo);
         bc_put(b, q0 + (size_t)Jl * ldk, hi);
     }
+    for (int q = 0; q < 4; q++) {
+        const size_t o = (size_t)frozen_bc(q) * sh.fs + base;
+        fz_put(b, o, 0.0);
+        fz_put(b, o + (size_t)Jl * ldk, 0.0);
+    }
 }
 
 // bcPhi with the periodic seam: k = km-1 is the copy of k = 0.
@@ -186,6 +208,10 @@ __global__ void __launch_bounds__(256) k_bc_phi(BcCtx b) {
     for (int f = 0; f < kNumBcFields; f++) {
         const size_t q = (size_t)f * sh.fs + row;
         bc_put(b, q + sh.km - 1, b.t.y[q]);
+    }
+    for (int q = 0; q < 4; q++) {
+        const size_t o = (size_t)frozen_bc(q) * sh.fs + row;
+        fz_put(b, o + sh.km - 1, b.t.frozen[o]);
     }
 }
 

@@ -55,8 +55,10 @@ hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, c
                         const double* v, const double* w, const double* pstat, double* level_out,
                         double* phydro, hipStream_t s);
 
-// Domain BC on the 17 listed prognostic fields of `y` (FP64); every value written is also written
-// into each non-null mirror buffer (FP64 `m64a`, `m64b`; FP32 `m32a..c`).
+// Domain BC on the fields of the path that the reference's lists name: the 17 prognostic fields of
+// `y` (FP64; every value written also goes to each non-null mirror, FP64 `m64a`, `m64b`, FP32
+// `m32a..c`) and the three reaction mass fluxes and the NH4SH flux limiter of `frozen` (with the
+// optional FP32 mirror `frozen32`).
 struct BcTargets {
     double* y;
     double* m64a;
@@ -64,6 +66,8 @@ struct BcTargets {
     float* m32a;
     float* m32b;
     float* m32c;
+    double* frozen;
+    float* frozen32;
 };
 hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s);
 

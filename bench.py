@@ -34,7 +34,22 @@ def parse():
     ap.add_argument("--omega", type=float, default=1.5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of known size (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy kernel moving a known number of bytes, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 64 * 1024 * 1024                      # 512 MiB per FP64 buffer: far beyond any cache
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                                           "launches": 3}}), flush=True)
 
 
 def traffic_from_profiles():
@@ -71,6 +86,9 @@ def cpu_baseline(sweeps, omega, budget_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported",
                           "n_gpus": args.gpus}))

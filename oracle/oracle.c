@@ -479,15 +479,27 @@ int orc_hydro(const atu_grid* g, const atu_params* p, double* fields, double* le
 /* ------------------------------------------------------------------------------------------- */
 /* domain BC on t, u, v, w and the 13 species                                                    */
 /* ------------------------------------------------------------------------------------------- */
+/* The fields of the path that the reference's BC lists name (planet/BC_Uran.cpp): t, u, v, w, the 13
+ * species, the three reaction mass fluxes and the NH4SH flux limiter. */
+static const int bc_list[21] = {ATU_T, ATU_U, ATU_V, ATU_W, ATU_CH4, ATU_CH4_CLOUD, ATU_CH4_ICE, ATU_H2O,
+                                ATU_H2O_CLOUD, ATU_H2O_ICE, ATU_H2S, ATU_H2S_CLOUD, ATU_H2S_ICE, ATU_NH3,
+                                ATU_NH3_CLOUD, ATU_NH3_ICE, ATU_NH4SH, ATU_MF_H2S, ATU_MF_NH3, ATU_MF_NH4SH,
+                                ATU_FLUXLIM_NH4SH};
+static int zero_at_poles(int f) {
+    return f == ATU_V || f == ATU_W || f == ATU_MF_H2S || f == ATU_MF_NH3 || f == ATU_MF_NH4SH ||
+           f == ATU_FLUXLIM_NH4SH;
+}
+
 int orc_bc(const atu_grid* g, const atu_params* p, double* fields) {
     const int im = g->im, jm = g->jm, km = g->km;
     const size_t n3 = (size_t)im * jm * km;
     const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
-    /* radius list: t, u, v, w and the species (the prognostic part of bc_fields_radius) */
+    /* radius list (bc_fields_radius, restricted to the fields of the path) */
 #pragma omp parallel for
     for (int j = 0; j < jm; j++)
         for (int k = 0; k < km; k++)
-            for (int f = 0; f <= ATU_NH4SH; f++) {
+            for (int q = 0; q < 21; q++) {
+                const int f = bc_list[q];
                 double* F = fields + (size_t)f * n3;
                 F[IDX(0, j, k)] = c43 * F[IDX(1, j, k)] - c13 * F[IDX(2, j, k)];
                 F[IDX(im - 1, j, k)] = c43 * F[IDX(im - 2, j, k)] - c13 * F[IDX(im - 3, j, k)];
@@ -518,27 +530,31 @@ int orc_bc(const atu_grid* g, const atu_params* p, double* fields) {
                                                        : F[IDX(c, j, k)] - 3.0 * F[IDX(b, j, k)] + 3.0 * F[IDX(a, j, k)];
         }
     }
-    /* poles: v, w zeroed; t, u, species 2-point Neumann */
+    /* poles: v, w, the mass fluxes and the flux limiter zeroed (bc_fields_theta_zero); t, u and
+     * the species 2-point Neumann (bc_fields_theta_extrap) */
 #pragma omp parallel for
     for (int k = 0; k < km; k++)
         for (int i = 0; i < im; i++) {
-            for (int q = 0; q < 2; q++) {
-                double* F = fields + (size_t)(q == 0 ? ATU_V : ATU_W) * n3;
+            for (int q = 0; q < 21; q++) {
+                const int f = bc_list[q];
+                if (!zero_at_poles(f)) continue;
+                double* F = fields + (size_t)f * n3;
                 F[IDX(i, 0, k)] = 0.0;
                 F[IDX(i, jm - 1, k)] = 0.0;
             }
-            for (int f = 0; f <= ATU_NH4SH; f++) {
-                if (f == ATU_V || f == ATU_W) continue;
+            for (int q = 0; q < 21; q++) {
+                const int f = bc_list[q];
+                if (zero_at_poles(f)) continue;
                 double* F = fields + (size_t)f * n3;
                 F[IDX(i, 0, k)] = c43 * F[IDX(i, 1, k)] - c13 * F[IDX(i, 2, k)];
                 F[IDX(i, jm - 1, k)] = c43 * F[IDX(i, jm - 2, k)] - c13 * F[IDX(i, jm - 3, k)];
             }
         }
-    /* periodic seam */
+    /* periodic seam (bc_fields_phi) */
     for (int i = 0; i < im; i++)
         for (int j = 0; j < jm; j++)
-            for (int f = 0; f <= ATU_NH4SH; f++) {
-                double* F = fields + (size_t)f * n3;
+            for (int q = 0; q < 21; q++) {
+                double* F = fields + (size_t)bc_list[q] * n3;
                 F[IDX(i, j, km - 1)] = F[IDX(i, j, 0)];
             }
     if (p->species_clamp) {

@@ -1,0 +1,157 @@
+// TEST INFRASTRUCTURE ONLY. Drives the reference's own hot-path member functions, compiled from
+// the reference sources where they lie (see build_ref.sh), to write golden vectors. Nothing from the
+// reference is copied: this file only constructs the model, fills its arrays from an input file,
+// calls the reference functions in the benchmark's order and dumps the arrays.
+//
+// usage: ref_harness <in.bin> <out.bin> <mode> [steps] [sweeps]
+//   mode bc       BC_Uran bcRadius, bcTheta, bcPhi                       (planet/BC_Uran.h:140-173)
+//   mode rk4      restoreVar(1) + RungeKuttaUran (hydro prep + 4 stages) (planet/RungeKutta_Uran_Turb.cpp)
+//   mode pressure PressureSolver<cUranusModel>::run with ATURAN_PRESS_SWEEPS=sweeps
+//   mode step     `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolver)
+// in/out: int32 im, jm, km, nfields, then nfields dense [im][jm][km] FP64 arrays in the atu_field
+// order of include/atu.h; out appends im doubles of buoy_ref_level.
+
+#include <fenv.h>
+#include <omp.h>
+#include <sys/stat.h>
+#include <sys/types.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <csignal>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <functional>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#define private public
+#define protected public
+#include "cUranusModel.h"
+#include "BC_Uran.h"
+#include "ChemistryUran.h"
+#include "PressureSolver.h"
+#undef private
+#undef protected
+
+static const int NF = 46;
+
+static std::vector<Array*> field_list(cUranusModel& m) {
+    return {&m.t, &m.u, &m.v, &m.w, &m.ch4, &m.ch4_cloud, &m.ch4_ice, &m.h2o, &m.h2o_cloud, &m.h2o_ice,
+            &m.h2s, &m.h2s_cloud, &m.h2s_ice, &m.nh3, &m.nh3_cloud, &m.nh3_ice, &m.nh4sh, &m.tke, &m.dis,
+            &m.nue, &m.rho_mix, &m.Q_precip,
+            &m.S_precip_ch4, &m.S_precip_ch4_cloud, &m.S_precip_ch4_ice,
+            &m.S_precip_h2o, &m.S_precip_h2o_cloud, &m.S_precip_h2o_ice,
+            &m.S_precip_h2s, &m.S_precip_h2s_cloud, &m.S_precip_h2s_ice,
+            &m.S_precip_nh3, &m.S_precip_nh3_cloud, &m.S_precip_nh3_ice,
+            &m.massflux_h2s, &m.massflux_nh3, &m.massflux_nh4sh,
+            &m.fluxlim_nh4sh, &m.tke_source, &m.dis_source,
+            &m.p_stat, &m.p_dyn, &m.p_hydro, &m.aux_u, &m.aux_v, &m.aux_w};
+}
+
+int main(int argc, char** argv) {
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s in.bin out.bin mode [steps] [sweeps]\n", argv[0]);
+        return 2;
+    }
+    const std::string mode = argv[3];
+    const int steps = argc > 4 ? std::atoi(argv[4]) : 1;
+    const int sweeps = argc > 5 ? std::atoi(argv[5]) : 1;
+    // knobs read once per process by the reference: set before the first call
+    setenv("ATURAN_PDYN_GEO", "0", 1);
+    char sw[32];
+    std::snprintf(sw, sizeof sw, "%d", sweeps);
+    setenv("ATURAN_PRESS_SWEEPS", sw, 1);
+
+    std::ifstream in(argv[1], std::ios::binary);
+    int32_t hdr[4];
+    in.read((char*)hdr, sizeof hdr);
+    if (!in || hdr[3] != NF) {
+        std::fprintf(stderr, "bad input header\n");
+        return 2;
+    }
+
+    static cUranusModel m;   // constructor: default config, coordinates, layer heights
+    if (hdr[0] != m.im || hdr[1] != m.jm || hdr[2] != m.km) {
+        std::fprintf(stderr, "grid mismatch: the reference is compiled for %d x %d x %d\n", m.im, m.jm, m.km);
+        return 2;
+    }
+    m.resetArrays();
+    m.dt = 0.001;
+    m.turb_active = true;
+    // mixture properties, then the T_INIT_ANCHOR rescale of the densities as Run() does them
+    ChemistryUran(m).ThermalPropertiesUran();
+    m.gam = m.g * 1.0e3 / m.cp_mix;
+    {
+        const double T_bottom = m.t_bottom_anchor_K();
+        const double f = m.p_bottom_bar() * 1.0e5 / (m.R_mix * T_bottom) / m.r_mix;
+        m.r_h2 *= f; m.r_he *= f; m.r_ch4 *= f; m.r_nh3 *= f; m.r_h2s *= f; m.r_h2o *= f; m.r_nh4sh *= f;
+        m.c_h2 *= f; m.c_he *= f; m.c_ch4 *= f; m.c_nh3 *= f; m.c_h2s *= f; m.c_h2o *= f; m.c_nh4sh *= f;
+        m.r_mix *= f; m.c_mix *= f;
+    }
+    std::fprintf(stderr, "ref_harness: r_mix %.17g R_mix %.17g cp_mix %.17g mue_mix %.17g r_h2 %.17g\n", m.r_mix,
+                 m.R_mix, m.cp_mix, m.mue_mix, m.r_h2);
+
+    std::vector<Array*> F = field_list(m);
+    const int im = m.im, jm = m.jm, km = m.km;
+    std::vector<double> buf((size_t)im * jm * km);
+    for (int f = 0; f < NF; f++) {
+        in.read((char*)buf.data(), buf.size() * sizeof(double));
+        for (int i = 0; i < im; i++)
+            for (int j = 0; j < jm; j++)
+                for (int k = 0; k < km; k++) F[f]->x[i][j][k] = buf[((size_t)i * jm + j) * km + k];
+    }
+    if (!in) {
+        std::fprintf(stderr, "short input\n");
+        return 2;
+    }
+    // every other array the RHS multiplies by a zero coefficient at the default knobs stays finite (0)
+
+    auto bc = [&]() {
+        BC_Uran(m).bcRadius();
+        BC_Uran(m).bcTheta();
+        BC_Uran(m).bcPhi();
+    };
+    if (mode == "bc") {
+        bc();
+    } else if (mode == "rk4") {
+        m.restoreVar(1.0);
+        m.RungeKuttaUran();
+    } else if (mode == "pressure") {
+        PressureSolver<cUranusModel>(m).run();
+    } else if (mode == "step") {
+        for (int s = 0; s < steps; s++) {
+            m.restoreVar(1.0);
+            m.RungeKuttaUran();
+            bc();
+            PressureSolver<cUranusModel>(m).run();
+        }
+    } else {
+        std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
+        return 2;
+    }
+
+    std::ofstream out(argv[2], std::ios::binary);
+    out.write((const char*)hdr, sizeof hdr);
+    for (int f = 0; f < NF; f++) {
+        for (int i = 0; i < im; i++)
+            for (int j = 0; j < jm; j++)
+                for (int k = 0; k < km; k++) buf[((size_t)i * jm + j) * km + k] = F[f]->x[i][j][k];
+        out.write((const char*)buf.data(), buf.size() * sizeof(double));
+    }
+    std::vector<double> lev(im, 0.0);
+    for (int i = 0; i < im && i < (int)m.buoy_ref_level.size(); i++) lev[i] = m.buoy_ref_level[i];
+    out.write((const char*)lev.data(), lev.size() * sizeof(double));
+    return out ? 0 : 1;
+}

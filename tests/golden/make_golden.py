@@ -1,0 +1,136 @@
+"""Writes the golden fixtures of tests/golden/ from the compiled reference (oracle/_ref/ref_harness).
+
+Run in the build container only (it needs /root/reference to build the harness):
+    python tests/golden/make_golden.py
+
+Input of every case: the seeded synthetic state of the library, enriched so that every term of the
+path is exercised (all 13 species, every precipitation source, the reaction mass fluxes, the NH4SH
+flux limiter, tke/dis sources, a non-zero p_dyn, Q_precip large enough to hit the +-0.5 cap and an
+eddy viscosity that is negative in places). The enrichment uses integer hashing only, so the test
+that re-creates the input on another machine gets the same bits.
+
+A fixture keeps, per case, every field at a fixed sample of cells (the radial, polar and seam
+planes, the neighbours of the sin(theta) floor edge and the interior) plus exact (math.fsum) sums
+and sums of squares of every field, and the buoyancy level means.
+"""
+from __future__ import annotations
+
+import math
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+
+import aturan_amd as A  # noqa: E402
+
+SEED = 20261203
+SAMPLE_I = [0, 1, 2, 20, 30, 38, 39, 40]
+SAMPLE_J = [0, 1, 2, 23, 24, 90, 156, 157, 178, 179, 180]
+SAMPLE_K = [0, 1, 2, 180, 358, 359, 360]
+CASES = {
+    # name: (harness mode, steps, sweeps, apply BC to the input first)
+    "bc": ("bc", 1, 1, False),
+    "rk4": ("rk4", 1, 1, True),
+    "pressure_1": ("pressure", 1, 1, True),
+    "pressure_20": ("pressure", 1, 20, True),
+    "step_2x3": ("step", 2, 3, True),
+}
+
+
+def hash_uniform(field: int, shape, salt: int) -> np.ndarray:
+    """Uniforms in [0, 1) from a SplitMix64-style mix of (salt, field, flat index), integer only."""
+    n = int(np.prod(shape))
+    x = np.arange(n, dtype=np.uint64) + np.uint64((salt * 1000003 + field * 7919) & 0xFFFFFFFF) * np.uint64(1 << 32)
+    with np.errstate(over="ignore"):
+        x = x * np.uint64(0x9E3779B97F4A7C15)
+        x ^= x >> np.uint64(30)
+        x = x * np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(27)
+        x = x * np.uint64(0x94D049BB133111EB)
+        x ^= x >> np.uint64(31)
+    return ((x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)).reshape(shape)
+
+
+def enriched_state(grid, params, seed: int = SEED) -> np.ndarray:
+    F = A.synthetic_state(grid, params, seed)
+    shp = F.shape[1:]
+    Fi = A.FIELD
+    for n in A.FIELD_NAMES[Fi["ch4_cloud"]:Fi["nh4sh"] + 1]:
+        F[Fi[n]] = 1e-3 * (0.5 + hash_uniform(Fi[n], shp, seed))
+    F[Fi["ch4"]] *= 1.0 + 0.05 * (hash_uniform(Fi["ch4"], shp, seed) - 0.5)
+    F[Fi["tke"]] = 1e-4 * (0.5 + hash_uniform(Fi["tke"], shp, seed))
+    F[Fi["dis"]] = 1.78 * (0.5 + hash_uniform(Fi["dis"], shp, seed))
+    F[Fi["nue"]] = 1.65e-4 * (hash_uniform(Fi["nue"], shp, seed) - 0.1)       # ~10 % negative
+    F[Fi["Q_precip"]] = 1600.0 * (hash_uniform(Fi["Q_precip"], shp, seed) - 0.5)   # |precip_t| capped above ~537
+    for n in A.FIELD_NAMES[Fi["S_precip_ch4"]:Fi["dis_source"] + 1]:
+        F[Fi[n]] = 1e-3 * (hash_uniform(Fi[n], shp, seed) - 0.5)
+    F[Fi["p_dyn"]] = 1e-3 * (hash_uniform(Fi["p_dyn"], shp, seed) - 0.5)
+    for n in ("aux_u", "aux_v", "aux_w"):
+        F[Fi[n]] = 0.1 * (hash_uniform(Fi[n], shp, seed) - 0.5)
+    return F
+
+
+def summarize(F: np.ndarray, level: np.ndarray) -> dict:
+    ii, jj, kk = np.ix_(SAMPLE_I, SAMPLE_J, SAMPLE_K)
+    sums = np.array([math.fsum(f.ravel().tolist()) for f in F])
+    sq = np.array([math.fsum((f.ravel() ** 2).tolist()) for f in F])
+    return {"samples": F[:, ii, jj, kk].copy(), "sums": sums, "sumsq": sq, "level": level.copy()}
+
+
+def write_bin(path: Path, F: np.ndarray) -> None:
+    hdr = np.array([F.shape[1], F.shape[2], F.shape[3], F.shape[0]], dtype=np.int32)
+    with open(path, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(np.ascontiguousarray(F).tobytes())
+
+
+def read_bin(path: Path, nf: int, shape) -> tuple[np.ndarray, np.ndarray]:
+    raw = np.fromfile(path, dtype=np.uint8)
+    body = raw[16:].view(np.float64)
+    n3 = int(np.prod(shape))
+    F = body[: nf * n3].reshape((nf,) + tuple(shape)).copy()
+    level = body[nf * n3: nf * n3 + shape[0]].copy()
+    return F, level
+
+
+def case_input(name: str, grid, params) -> np.ndarray:
+    from oracle import binding as O
+    F = enriched_state(grid, params)
+    if CASES[name][3]:
+        O.bc(grid, params, F)
+    return F
+
+
+def main() -> None:
+    r = subprocess.run(["bash", str(ROOT / "oracle" / "build_ref.sh")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise SystemExit(f"reference build failed:\n{r.stdout}\n{r.stderr}")
+    harness = ROOT / "oracle" / "_ref" / "ref_harness"
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    shape = (g.im, g.jm, g.km)
+    with tempfile.TemporaryDirectory() as td:
+        for name, (mode, steps, sweeps, _) in CASES.items():
+            F = case_input(name, g, p)
+            fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
+            write_bin(fin, F)
+            rr = subprocess.run([str(harness), str(fin), str(fout), mode, str(steps), str(sweeps)],
+                                capture_output=True, text=True, env={"OMP_NUM_THREADS": "8", "PATH": "/usr/bin"})
+            if rr.returncode != 0:
+                raise SystemExit(f"harness {name} failed:\n{rr.stderr[-2000:]}")
+            consts = [ln for ln in rr.stderr.splitlines() if ln.startswith("ref_harness:")]
+            G, level = read_bin(fout, A.NFIELDS, shape)
+            s = summarize(G, level)
+            np.savez_compressed(HERE / f"ref_{name}.npz", **s, mode=mode, steps=steps, sweeps=sweeps,
+                                consts=np.array(consts))
+            print(f"wrote ref_{name}.npz ({consts[0] if consts else ''})", flush=True)
+
+
+if __name__ == "__main__":
+    main()

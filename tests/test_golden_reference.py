@@ -1,0 +1,82 @@
+"""Pins the CPU oracle to golden vectors written by the compiled reference (tests/golden/ref_*.npz,
+from tests/golden/make_golden.py): the domain BC, one RungeKuttaUran (hydro prep + 4 stages), the
+red-black PressureSolver at 1 and 20 sweeps, and two full steps, on an input in which every term of
+the path is non-zero. Per field: the sampled cells within 1e-12 of the field's largest sample, and
+the exact sum and sum of squares within 1e-12 relative."""
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import aturan_amd as A
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+sys.path.insert(0, str(GOLDEN))
+import make_golden as MG  # noqa: E402
+
+TOL = 1e-12
+
+
+def run_oracle(oracle, name, g, p):
+    F = MG.case_input(name, g, p)
+    mode, steps, sweeps, _ = MG.CASES[name]
+    level = np.zeros(g.im)
+    if mode == "bc":
+        oracle.bc(g, p, F)
+    elif mode == "rk4":
+        level = oracle.hydro(g, p, F)
+        oracle.rk4(g, p, F)
+    elif mode == "pressure":
+        oracle.pressure(g, p, F, sweeps, 1.0)
+    else:
+        for _ in range(steps):
+            level = oracle.hydro(g, p, F)
+            oracle.rk4(g, p, F)
+            oracle.bc(g, p, F)
+            oracle.pressure(g, p, F, sweeps, 1.0)
+    return F, level
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", list(MG.CASES))
+def test_oracle_matches_compiled_reference(oracle, name):
+    ref = np.load(GOLDEN / f"ref_{name}.npz")
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F, level = run_oracle(oracle, name, g, p)
+    got = MG.summarize(F, level)
+    worst = {}
+    for f, fname in enumerate(A.FIELD_NAMES):
+        a, b = got["samples"][f], ref["samples"][f]
+        scale = float(np.max(np.abs(b))) or 1.0
+        worst[fname] = float(np.max(np.abs(a - b))) / scale
+        for key in ("sums", "sumsq"):
+            x, y = got[key][f], ref[key][f]
+            assert abs(x - y) <= TOL * max(abs(y), 1e-300) or x == y, (name, fname, key, x, y)
+    bad = {k: v for k, v in worst.items() if v > TOL}
+    print(f"[golden] {name}: worst sampled difference {max(worst.values()):.3e}")
+    assert not bad, bad
+    if MG.CASES[name][0] in ("rk4", "step"):
+        assert np.max(np.abs(got["level"] - ref["level"])) <= TOL * np.max(np.abs(ref["level"]))
+
+
+def test_fixture_constants_match_library_params():
+    """The reference's mixture constants after T_INIT_ANCHOR equal atu_params_default's."""
+    ref = np.load(GOLDEN / "ref_rk4.npz")
+    line = str(ref["consts"][0])
+    vals = dict(zip(line.split()[1::2], map(float, line.split()[2::2])))
+    p = A.default_params(A.default_grid(1))
+    for key in ("r_mix", "R_mix", "cp_mix", "mue_mix", "r_h2"):
+        assert getattr(p, key) == vals[key], key
+
+
+def test_enriched_input_is_deterministic():
+    g = A.default_grid(1)
+    g.im, g.jm, g.km = 8, 9, 13
+    p = A.default_params(g)
+    a = MG.enriched_state(g, p)
+    b = MG.enriched_state(g, p)
+    assert np.array_equal(a, b)
+    assert math.isclose(float(np.mean(a[A.FIELD["nue"]] < 0)), 0.1, abs_tol=0.1)

@@ -54,7 +54,7 @@ def test_bc_matches_oracle(oracle):
         core.boundary_conditions()
         G = core.get_state()
     oracle.bc(g, p, F)
-    errs = {n: rel_err(G[A.FIELD[n]], F[A.FIELD[n]]) for n in PROG}
+    errs = {n: rel_err(G[A.FIELD[n]], F[A.FIELD[n]]) for n in A.FIELD_NAMES}
     assert report("bc", errs) == 0.0
 
 
