@@ -52,11 +52,31 @@ struct atu_ctx {
     double* PHYDRO = nullptr;
     double* AUX = nullptr;    // 3
     double* SRC = nullptr;
-    double* RES = nullptr;
+    double* RES = nullptr;    // the consistent source b = src - shift of the converging solve
     double* LEVEL = nullptr;  // im
     double* RPART = nullptr;  // residual partials
     double* RSUM = nullptr;   // 2
     void* geo_blob = nullptr;
+    // converging solve: tables, row sums, scalars, column residual and its modes
+    SolveTables htab;
+    DevSolve dsol{};
+    void* solve_blob = nullptr;
+    double* SROWS = nullptr;  // (im-2)*(jm-2)
+    double* SCAL = nullptr;   // 8 scalars: 0 y.src, 1 mean before, 2 |b|^2, 3 |r|^2, 4 mean after
+    double* BR = nullptr;     // J*N
+    double* BAC = nullptr;    // (NM+1)*J
+    double* BAS = nullptr;
+    // mixed path: FP32 stencil-side mirrors and stage states, the FP32 correction and residual
+    float* Y32 = nullptr;     // 19, FP32 mirror of y_n (the stencil of stage 0)
+    float* SA32 = nullptr;    // 19
+    float* SB32 = nullptr;    // 19
+    float* FROZEN32 = nullptr;// 21
+    float* PDYN32 = nullptr;
+    float* PHYDRO32 = nullptr;
+    float* AUX32 = nullptr;   // 3
+    float* E32 = nullptr;     // refinement correction
+    float* R32 = nullptr;     // refinement residual
+    bool mixed() const { return precision == ATU_PREC_MIXED; }
 
     // profiling
     bool prof = false;
@@ -105,7 +125,11 @@ struct atu_ctx {
 };
 
 atu_ctx::~atu_ctx() {
-    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM})
+    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SCAL,
+                      BR, BAC, BAS})
+        if (p) hipFree(p);
+    if (solve_blob) hipFree(solve_blob);
+    for (float* p : {Y32, SA32, SB32, FROZEN32, PDYN32, PHYDRO32, AUX32, E32, R32})
         if (p) hipFree(p);
     if (ev_bc) hipEventDestroy(ev_bc);
     if (ev_hydro) hipEventDestroy(ev_hydro);
@@ -125,7 +149,7 @@ int upload_geometry(atu_ctx* c) {
     const Geometry& h = c->hgeo;
     std::vector<const std::vector<double>*> parts = {
         &h.rm, &h.inv_rm, &h.inv_rm2, &h.two_inv_rm, &h.sponge, &h.num2,
-        &h.costhe, &h.cotanthe, &h.sinthe_bf, &h.cor_hydro, &h.sin_weight,
+        &h.costhe, &h.cotanthe, &h.sinthe_bf, &h.cor_hydro, &h.sin_weight, &h.pole_row,
         &h.inv_rmsinthe, &h.inv_rm2sinthe, &h.inv_rm2sinthe2, &h.costhe_inv_rm2sinthe,
         &h.v_metric, &h.num3, &h.den, &h.inv_rmsinthe_s};
     const std::vector<double>& cp = cp_h2_table(c->params.cp_mode == 1 ? 1 : 2);
@@ -147,7 +171,7 @@ int upload_geometry(atu_ctx* c) {
     const double* d = (const double*)c->geo_blob;
     DevGeo& g = c->geo;
     const double** dst[] = {&g.rm, &g.inv_rm, &g.inv_rm2, &g.two_inv_rm, &g.sponge, &g.num2,
-                            &g.costhe, &g.cotanthe, &g.sinthe_bf, &g.cor_hydro, &g.sin_weight,
+                            &g.costhe, &g.cotanthe, &g.sinthe_bf, &g.cor_hydro, &g.sin_weight, &g.pole_row,
                             &g.inv_rmsinthe, &g.inv_rm2sinthe, &g.inv_rm2sinthe2, &g.costhe_inv_rm2sinthe,
                             &g.v_metric, &g.num3, &g.den, &g.inv_rmsinthe_s};
     for (size_t q = 0; q < parts.size(); q++) *dst[q] = d + off[q];
@@ -163,11 +187,53 @@ int upload_geometry(atu_ctx* c) {
     return ATU_OK;
 }
 
-int alloc_fields(atu_ctx* c, double** p, int nf) {
-    const size_t bytes = (size_t)nf * c->sh.fs * sizeof(double);
+int upload_solve_tables(atu_ctx* c) {
+    const SolveTables& t = c->htab;
+    const std::vector<const std::vector<double>*> parts = {&t.line_cp, &t.line_dn, &t.mode_cp, &t.mode_dn,
+                                                           &t.ct, &t.st, &t.ctT, &t.stT};
+    std::vector<double> blob;
+    std::vector<size_t> off;
+    for (auto* v : parts) {
+        off.push_back(blob.size());
+        blob.insert(blob.end(), v->begin(), v->end());
+    }
+    if (c->hip(hipMalloc(&c->solve_blob, blob.size() * sizeof(double)), "hipMalloc(solve tables)")) return ATU_EHIP;
+    if (c->hip(hipMemcpy(c->solve_blob, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice),
+               "hipMemcpy(solve tables)"))
+        return ATU_EHIP;
+    const double* d = (const double*)c->solve_blob;
+    DevSolve& s = c->dsol;
+    const double** dst[] = {&s.line_cp, &s.line_dn, &s.mode_cp, &s.mode_dn, &s.ct, &s.st, &s.ctT, &s.stT};
+    for (size_t q = 0; q < parts.size(); q++) *dst[q] = d + off[q];
+    s.ni = t.ni;
+    s.N = t.N;
+    s.NM = t.NM;
+    s.J = t.J;
+    s.S2 = t.S2;
+    s.yw = t.yw;
+    s.ncell = t.ncell;
+    s.e = t.e;
+    const size_t nrow = (size_t)(c->sh.im - 2) * (c->sh.jm - 2);
+    if (c->hip(hipMalloc((void**)&c->SROWS, nrow * sizeof(double)), "hipMalloc") ||
+        c->hip(hipMalloc((void**)&c->SCAL, 8 * sizeof(double)), "hipMalloc") ||
+        c->hip(hipMalloc((void**)&c->BR, (size_t)t.J * t.N * sizeof(double)), "hipMalloc") ||
+        c->hip(hipMalloc((void**)&c->BAC, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc") ||
+        c->hip(hipMalloc((void**)&c->BAS, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc"))
+        return ATU_ENOMEM;
+    return ATU_OK;
+}
+
+template <typename T>
+int alloc_fields(atu_ctx* c, T** p, int nf) {
+    const size_t bytes = (size_t)nf * c->sh.fs * sizeof(T);
     if (c->hip(hipMalloc((void**)p, bytes), "hipMalloc(fields)")) return ATU_ENOMEM;
     if (c->hip(hipMemset(*p, 0, bytes), "hipMemset(fields)")) return ATU_EHIP;
     return ATU_OK;
+}
+
+// FP32 mirror of one FP64 field (mixed path)
+int mirror(atu_ctx* c, const double* src, float* dst) {
+    return c->hip(launch_convert(src, dst, c->sh.fs, c->stream), "convert");
 }
 
 // Copy a dense [im][jm][km] host array into a padded device field (and back).
@@ -193,9 +259,9 @@ double* field_ptr(atu_ctx* c, int f) {
         case ATU_P_STAT: return c->PSTAT;
         case ATU_P_DYN: return c->PDYN;
         case ATU_P_HYDRO: return c->PHYDRO;
-        case ATU_AUX_U: return c->AUX;
-        case ATU_AUX_V: return c->AUX + c->sh.fs;
-        case ATU_AUX_W: return c->AUX + 2 * c->sh.fs;
+        case ATU_AUX_U:
+        case ATU_AUX_V:
+        case ATU_AUX_W: return c->mixed() ? c->RES : c->AUX + (size_t)(f - ATU_AUX_U) * c->sh.fs;
         default: return nullptr;
     }
 }
@@ -229,10 +295,40 @@ StageArgs<double> stage_args(atu_ctx* c, int stage) {
     return a;
 }
 
+// Mixed path: FP32 stencil states Y32 -> SA32 -> SB32 -> SA32, FP64 y_n and accumulator; stage 3
+// writes y_{n+1} into Y and its FP32 mirror Y32.
+StageArgs<float> stage_args32(atu_ctx* c, int stage) {
+    StageArgs<float> a{};
+    a.sh = c->sh;
+    a.geo = c->geo;
+    a.c = c->consts;
+    a.yn = c->Y;
+    a.acc = c->ACC;
+    a.frozen = c->FROZEN32;
+    a.pdyn = c->PDYN32;
+    a.phydro = c->PHYDRO32;
+    a.aux = c->AUX32;
+    a.ymirror = nullptr;
+    switch (stage) {
+        case 0: a.xs = c->Y32; a.xnext = c->SA32; break;
+        case 1: a.xs = c->SA32; a.xnext = c->SB32; break;
+        case 2: a.xs = c->SB32; a.xnext = c->SA32; break;
+        default: a.xs = c->SA32; a.xnext = nullptr; a.yout = c->Y; a.ymirror = c->Y32; break;
+    }
+    return a;
+}
+
 int do_rk4(atu_ctx* c) {
     for (int s = 0; s < 4; s++) {
-        const StageArgs<double> a = stage_args(c, s);
-        if (c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")) return ATU_EHIP;
+        hipError_t e;
+        if (c->mixed()) {
+            const StageArgs<float> a = stage_args32(c, s);
+            e = c->timed(PC_RK, [&] { return launch_rk_stage_mixed(s, a, c->stream); });
+        } else {
+            const StageArgs<double> a = stage_args(c, s);
+            e = c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); });
+        }
+        if (c->hip(e, "rk_stage")) return ATU_EHIP;
     }
     return ATU_OK;
 }
@@ -242,13 +338,14 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
                               [&] {
                                   return launch_hydro(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T),
                                                       c->prog(c->Y, ATU_V), c->prog(c->Y, ATU_W), c->PSTAT, c->LEVEL,
-                                                      c->PHYDRO, s);
+                                                      c->PHYDRO, c->PHYDRO32, s);
                               }),
                   "hydro");
 }
 
 int do_bc(atu_ctx* c) {
-    BcTargets t{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
+    const BcTargets t = c->mixed() ? BcTargets{c->Y, nullptr, nullptr, c->Y32, c->SA32, c->SB32, c->FROZEN, c->FROZEN32}
+                                   : BcTargets{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
     return c->hip(c->timed(PC_BC, [&] { return launch_bc(c->sh, c->params, t, c->stream); }), "bc");
 }
 
@@ -270,14 +367,111 @@ int do_residual(atu_ctx* c, double* rel, double* srcn) {
     return ATU_OK;
 }
 
-int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
-    atu_solver_opts def{ATU_SOLVE_REFERENCE, 1, 0, 0, 1.0, 0.0};
-    if (!o) o = &def;
-    if (c->hip(c->timed(PC_PROJ,
-                        [&] { return launch_proj_src<double>(c->sh, c->geo, c->consts, c->AUX, c->SRC, c->stream); }),
-               "proj_src"))
-        return ATU_EHIP;
-    if (o->mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EUNSUPPORTED, "solve mode not available in this build");
+// Converging solve on the prepared source SRC: the consistent source b = src - (y.src / y.1) in
+// RES, then iterations until ||b - A p|| <= rel_tol ||b|| (or the iteration limit), then the corners,
+// the seam and the held interior mean.
+//
+// FP64 context: each iteration is two line passes, the walls, the barotropic correction, the walls
+// and the residual norm, all on p in FP64.
+// Mixed context (iterative refinement): each outer iteration forms the residual r = b - A p in
+// FP64 (stored in FP32), solves A e = r approximately with `inner` FP32 line passes plus barotropic
+// corrections on an FP32 correction e, and accumulates p += e in FP64.
+int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
+    const Shape& sh = c->sh;
+    hipStream_t s = c->stream;
+    auto H = [&](hipError_t e, const char* w) { return c->hip(e, w); };
+    int e;
+    if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_YSRC, sh, c->geo, c->PDYN, c->SRC, c->SROWS,
+                                                                  c->SCAL + 0, s); }), "dsum")) ||
+        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_P, sh, c->geo, c->PDYN, nullptr, c->SROWS,
+                                                                  c->SCAL + 1, s); }), "dsum")) ||
+        (e = H(launch_make_b(sh, c->SRC, c->SCAL, c->dsol.yw, c->RES, s), "make_b")) ||
+        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_SQ, sh, c->geo, c->PDYN, c->RES, c->SROWS,
+                                                                  c->SCAL + 2, s); }), "dsum")))
+        return e;
+    const int max_it = o->max_sweeps > 0 ? o->max_sweeps : 1;
+    int it = 0, outer = 0;
+    double rel = INFINITY, bnorm = 0.0;
+    auto read_rel = [&]() -> int {
+        double h[2];
+        if (H(hipMemcpyAsync(h, c->SCAL + 2, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpy(norms)") ||
+            H(hipStreamSynchronize(s), "hipStreamSynchronize"))
+            return ATU_EHIP;
+        bnorm = std::sqrt(h[0]);
+        rel = bnorm > 0.0 ? std::sqrt(h[1]) / bnorm : std::sqrt(h[1]);
+        if (!std::isfinite(rel)) return c->fail(ATU_ENONFINITE, "converging solve: non-finite residual");
+        return ATU_OK;
+    };
+    if (!c->mixed()) {
+        while (it < max_it) {
+            for (int colour = 0; colour < 2; colour++)
+                if ((e = H(c->timed(PC_LINE, [&] { return launch_line<double, double, double>(sh, c->geo, c->dsol,
+                                                                                               c->PDYN, c->RES, colour,
+                                                                                               s); }), "line")))
+                    return e;
+            if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, c->PDYN, s); }), "walls")) ||
+                (e = H(c->timed(PC_BARO, [&] { return launch_baro<double, double>(sh, c->geo, c->dsol, c->PDYN, c->RES,
+                                                                                   c->BR, c->BAC, c->BAS, s); }),
+                       "baro")) ||
+                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, c->PDYN, s); }), "walls")) ||
+                (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, c->PDYN, c->RES,
+                                                                          c->SROWS, c->SCAL + 3, s); }), "dsum")))
+                return e;
+            it++;
+            if ((e = read_rel())) return e;
+            if (rel <= o->rel_tol) break;
+        }
+        outer = 1;
+    } else {
+        const int inner = o->sweeps > 0 ? o->sweeps : 4;
+        while (true) {
+            // FP64 residual of the current p, kept in FP32 as the right-hand side of the correction
+            if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, c->PDYN, c->RES,
+                                                                          c->SROWS, c->SCAL + 3, s, c->R32); }),
+                       "dsum")) ||
+                (e = read_rel()))
+                return e;
+            if (rel <= o->rel_tol || it + inner > max_it) break;
+            if ((e = H(launch_fill(c->E32, 0.0f, sh.fs, s), "fill"))) return e;
+            for (int q = 0; q < inner; q++) {
+                for (int colour = 0; colour < 2; colour++)
+                    if ((e = H(c->timed(PC_LINE, [&] { return launch_line<float, float, float>(sh, c->geo, c->dsol,
+                                                                                                c->E32, c->R32, colour,
+                                                                                                s); }), "line")))
+                        return e;
+                if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s); }), "walls")) ||
+                    (e = H(c->timed(PC_BARO, [&] { return launch_baro<float, float>(sh, c->geo, c->dsol, c->E32,
+                                                                                     c->R32, c->BR, c->BAC, c->BAS,
+                                                                                     s); }), "baro")) ||
+                    (e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s); }), "walls")))
+                    return e;
+                it++;
+            }
+            if ((e = H(c->timed(PC_LINE, [&] { return launch_add_correction(sh, c->E32, c->PDYN, s); }), "add")) ||
+                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, c->PDYN, s); }), "walls")))
+                return e;
+            outer++;
+        }
+    }
+    if ((e = H(launch_p_corners<double>(sh, c->consts, c->PDYN, s), "corners")) ||
+        (e = H(launch_seam<double>(sh, c->PDYN, s), "seam")) ||
+        (e = H(launch_dsum<double>(DSUM_P, sh, c->geo, c->PDYN, nullptr, c->SROWS, c->SCAL + 4, s), "dsum")) ||
+        (e = H(launch_shift_all<double>(sh, c->SCAL, 1, 4, c->dsol.ncell, c->PDYN, s), "shift")))
+        return e;
+    if (st) {
+        st->sweeps_done = it;
+        st->outer_iters = outer;
+        st->rel_resid = rel;
+        st->src_norm = bnorm;
+    }
+    if (rel > o->rel_tol && o->rel_tol > 0.0)
+        return c->fail(ATU_ENOCONV, "converging solve: tolerance not reached within max_sweeps");
+    return ATU_OK;
+}
+
+int do_pressure_fp64(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
+    if (o->mode == ATU_SOLVE_CONVERGE) return do_converge(c, o, st);
+    if (o->mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EINVAL, "unknown solve mode");
     const int sweeps = o->sweeps > 0 ? o->sweeps : 1;   // PressureSolver.h:174-175
     for (int s = 0; s < sweeps; s++) {
         for (int colour = 0; colour < 2; colour++) {
@@ -302,6 +496,20 @@ int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
     return ATU_OK;
 }
 
+int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
+    atu_solver_opts def{ATU_SOLVE_REFERENCE, 1, 0, 0, 1.0, 0.0};
+    if (!o) o = &def;
+    const hipError_t pe = c->mixed()
+        ? c->timed(PC_PROJ, [&] { return launch_proj_src<float>(c->sh, c->geo, c->consts, c->AUX32, c->SRC, c->stream); })
+        : c->timed(PC_PROJ, [&] { return launch_proj_src<double>(c->sh, c->geo, c->consts, c->AUX, c->SRC, c->stream); });
+    if (c->hip(pe, "proj_src")) return ATU_EHIP;
+    int e = do_pressure_fp64(c, o, st);
+    if (e) return e;
+    // the FP32 stencil of the next RK step reads p_dyn's mirror
+    if (c->mixed() && (e = mirror(c, c->PDYN, c->PDYN32))) return e;
+    return ATU_OK;
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -314,9 +522,9 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         g_create_error = "atu_create: invalid grid (need im,jm,km >= 6 and km-1 even)";
         return ATU_EINVAL;
     }
-    if (precision != ATU_PREC_FP64) {
-        g_create_error = "atu_create: precision not available in this build";
-        return ATU_EUNSUPPORTED;
+    if (precision != ATU_PREC_FP64 && precision != ATU_PREC_MIXED) {
+        g_create_error = "atu_create: unknown precision";
+        return ATU_EINVAL;
     }
     std::unique_ptr<atu_ctx> c(new (std::nothrow) atu_ctx());
     if (!c) return ATU_ENOMEM;
@@ -331,6 +539,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     c->sh.fs = (size_t)g->im * c->sh.plane;
     build_geometry(*g, *p, c->hgeo);
     build_step_consts(*g, *p, c->consts);
+    build_solve_tables(*g, *p, c->hgeo, c->htab);
     int e;
     if ((e = c->hip(hipStreamCreate(&c->stream), "hipStreamCreate"))) {
         g_create_error = c->err;
@@ -343,12 +552,31 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         g_create_error = c->err;
         return e;
     }
-    if ((e = upload_geometry(c.get())) || (e = alloc_fields(c.get(), &c->Y, ATU_NPROG)) ||
-        (e = alloc_fields(c.get(), &c->SA, ATU_NPROG)) || (e = alloc_fields(c.get(), &c->SB, ATU_NPROG)) ||
+    if ((e = upload_geometry(c.get())) || (e = upload_solve_tables(c.get())) ||
+        (e = alloc_fields(c.get(), &c->Y, ATU_NPROG)) ||
         (e = alloc_fields(c.get(), &c->ACC, ATU_NPROG)) || (e = alloc_fields(c.get(), &c->FROZEN, ATU_NFROZEN)) ||
         (e = alloc_fields(c.get(), &c->PSTAT, 1)) || (e = alloc_fields(c.get(), &c->PDYN, 1)) ||
-        (e = alloc_fields(c.get(), &c->PHYDRO, 1)) || (e = alloc_fields(c.get(), &c->AUX, 3)) ||
-        (e = alloc_fields(c.get(), &c->SRC, 1))) {
+        (e = alloc_fields(c.get(), &c->PHYDRO, 1)) || (e = alloc_fields(c.get(), &c->SRC, 1)) ||
+        (e = alloc_fields(c.get(), &c->RES, 1))) {
+        g_create_error = c->err;
+        return e;
+    }
+    if (precision == ATU_PREC_FP64) {
+        e = alloc_fields(c.get(), &c->SA, ATU_NPROG);
+        if (!e) e = alloc_fields(c.get(), &c->SB, ATU_NPROG);
+        if (!e) e = alloc_fields(c.get(), &c->AUX, 3);
+    } else {
+        e = alloc_fields(c.get(), &c->Y32, ATU_NPROG);
+        if (!e) e = alloc_fields(c.get(), &c->SA32, ATU_NPROG);
+        if (!e) e = alloc_fields(c.get(), &c->SB32, ATU_NPROG);
+        if (!e) e = alloc_fields(c.get(), &c->FROZEN32, ATU_NFROZEN);
+        if (!e) e = alloc_fields(c.get(), &c->PDYN32, 1);
+        if (!e) e = alloc_fields(c.get(), &c->PHYDRO32, 1);
+        if (!e) e = alloc_fields(c.get(), &c->AUX32, 3);
+        if (!e) e = alloc_fields(c.get(), &c->E32, 1);
+        if (!e) e = alloc_fields(c.get(), &c->R32, 1);
+    }
+    if (e) {
         g_create_error = c->err;
         return e;
     }
@@ -404,13 +632,30 @@ int atu_synchronize(atu_ctx* ctx) {
 
 int atu_set_field(atu_ctx* ctx, int f, const double* host) {
     if (!ctx || !host) return ATU_EINVAL;
+    const bool aux = (f >= ATU_AUX_U && f <= ATU_AUX_W);
     double* d = field_ptr(ctx, f);
     if (!d) return ctx->fail(ATU_EINVAL, "atu_set_field: unknown field id");
+    const size_t fs = ctx->sh.fs;
     int e = put_field(ctx, d, host);
-    if (!e && f < ATU_NPROG) {
+    if (!e && !ctx->mixed() && f < ATU_NPROG) {
         // restoreVar's role: every stage buffer starts from the same state, boundary planes included
-        if (!e) e = put_field(ctx, ctx->prog(ctx->SA, f), host);
+        e = put_field(ctx, ctx->prog(ctx->SA, f), host);
         if (!e) e = put_field(ctx, ctx->prog(ctx->SB, f), host);
+    }
+    if (!e && ctx->mixed()) {
+        if (f < ATU_NPROG) {
+            e = mirror(ctx, d, ctx->Y32 + (size_t)f * fs);
+            if (!e) e = mirror(ctx, d, ctx->SA32 + (size_t)f * fs);
+            if (!e) e = mirror(ctx, d, ctx->SB32 + (size_t)f * fs);
+        } else if (f >= ATU_NUE && f < ATU_NUE + ATU_NFROZEN) {
+            e = mirror(ctx, d, ctx->FROZEN32 + (size_t)(f - ATU_NUE) * fs);
+        } else if (f == ATU_P_DYN) {
+            e = mirror(ctx, d, ctx->PDYN32);
+        } else if (f == ATU_P_HYDRO) {
+            e = mirror(ctx, d, ctx->PHYDRO32);
+        } else if (aux) {
+            e = mirror(ctx, d, ctx->AUX32 + (size_t)(f - ATU_AUX_U) * fs);   // staged through RES
+        }
     }
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
@@ -418,9 +663,14 @@ int atu_set_field(atu_ctx* ctx, int f, const double* host) {
 
 int atu_get_field(atu_ctx* ctx, int f, double* host) {
     if (!ctx || !host) return ATU_EINVAL;
+    const bool aux = (f >= ATU_AUX_U && f <= ATU_AUX_W);
     const double* d = field_ptr(ctx, f);
     if (!d) return ctx->fail(ATU_EINVAL, "atu_get_field: unknown field id");
-    int e = get_field(ctx, d, host);
+    int e = ATU_OK;
+    if (ctx->mixed() && aux)   // aux lives in FP32 on the mixed path: widen it through RES
+        e = ctx->hip(launch_convert(ctx->AUX32 + (size_t)(f - ATU_AUX_U) * ctx->sh.fs, ctx->RES, ctx->sh.fs,
+                                    ctx->stream), "convert");
+    if (!e) e = get_field(ctx, d, host);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
 }

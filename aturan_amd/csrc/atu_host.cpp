@@ -134,6 +134,7 @@ void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo) {
     geo.sinthe.assign(jm, 0); geo.sinthe2.assign(jm, 0); geo.costhe.assign(jm, 0);
     geo.cotanthe.assign(jm, 0); geo.sinthe_bf.assign(jm, 0); geo.cor_hydro.assign(jm, 0);
     geo.sin_weight.assign(jm, 0);
+    geo.pole_row.assign(jm, 0);
     std::vector<double> sin_solver(jm);
     for (int j = 0; j < jm; j++) {
         const double s = std::max(p.sinthe_min, std::abs(std::sin(geo.thez[j])));
@@ -145,6 +146,7 @@ void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo) {
         geo.sinthe_bf[j] = std::sqrt(std::max(0.0, 1.0 - c * c));
         geo.cor_hydro[j] = p.Coriolis * p.coriolis_scale * 2.0 * p.omega * std::abs(std::sin(geo.thez[j]));
         geo.sin_weight[j] = std::sin(geo.thez[j]);
+        geo.pole_row[j] = (std::abs(std::sin(geo.thez[j])) < p.sinthe_min) ? 1.0 : 0.0;
         // the solver's own table, planet/PressureSolver.h:156-162
         double ss = std::sin(geo.thez[j]);
         if (ss < p.sinthe_min) ss = p.sinthe_min;
@@ -175,6 +177,89 @@ void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo) {
             geo.num3[q] = n3;
             geo.den[q] = 2.0 * geo.num1 + 2.0 * geo.num2[i] + 2.0 * n3;
             geo.inv_rmsinthe_s[q] = 1.0 / (rm * ss);
+        }
+    }
+}
+
+// The converging solve works on the system of the red-black sweep with every wall folded in
+// (p_wall = c43 p_1 - c13 p_2 in r and theta, planet/PressureSolver.h:718-732). Its radial lines
+// are the reference's PRESS_LINE rows (:497-539) and its barotropic modes the PRESS_BAROTROPIC
+// theta systems (:637-652), with the column sums weighted by the left null vector (3/2 at the end
+// rows) so that the radial part drops out exactly.
+void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, SolveTables& t) {
+    const int im = g.im, jm = g.jm, km = g.km;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    const int n = im - 2;
+    t.ni = n;
+    t.line_cp.assign((size_t)jm * n, 0.0);
+    t.line_dn.assign((size_t)jm * n, 0.0);
+    for (int j = 1; j < jm - 1; j++) {
+        double* cp = &t.line_cp[(size_t)j * n];
+        double* dn = &t.line_dn[(size_t)j * n];
+        for (int s = 0; s < n; s++) {
+            const int i = s + 1;
+            const double num1 = geo.num1, num2 = geo.num2[i];
+            double diag = -geo.den[(size_t)i * jm + j];
+            if (j == 1 || j == jm - 2) diag = diag + num2 * c43;
+            double a = num1, c = num1;
+            if (s == 0) { diag = diag + num1 * c43; c = num1 - num1 * c13; a = 0.0; }
+            if (s == n - 1) { diag = diag + num1 * c43; a = num1 - num1 * c13; c = 0.0; }
+            if (n == 1) { a = 0.0; c = 0.0; }
+            const double d = (s == 0) ? diag : diag - a * cp[s - 1];
+            dn[s] = d;
+            cp[s] = c / d;
+        }
+    }
+    const int N = km - 1, NM = N / 2, J = jm - 2;
+    t.N = N; t.NM = NM; t.J = J;
+    t.ct.assign((size_t)(NM + 1) * N, 0.0);
+    t.st.assign((size_t)(NM + 1) * N, 0.0);
+    t.ctT.assign((size_t)(NM + 1) * N, 0.0);
+    t.stT.assign((size_t)(NM + 1) * N, 0.0);
+    for (int m = 0; m <= NM; m++)
+        for (int k = 0; k < N; k++) {
+            const double a = 2.0 * M_PI * (double)((long)m * k % N) / N;
+            t.ct[(size_t)m * N + k] = std::cos(a);
+            t.st[(size_t)m * N + k] = std::sin(a);
+            t.ctT[(size_t)k * (NM + 1) + m] = t.ct[(size_t)m * N + k];
+            t.stT[(size_t)k * (NM + 1) + m] = t.st[(size_t)m * N + k];
+        }
+    auto endw = [](int x, int lo, int hi) { return (x == lo || x == hi) ? 1.5 : 1.0; };
+    double S2 = 0.0;
+    for (int i = 1; i < im - 1; i++) {
+        const double rm = geo.rm[i];
+        S2 += endw(i, 1, im - 2) * (1.0 / (rm * rm));
+    }
+    t.S2 = S2;
+    double sa = 0.0, sb = 0.0;
+    for (int i = 1; i < im - 1; i++) sa += endw(i, 1, im - 2);
+    for (int j = 1; j < jm - 1; j++) sb += endw(j, 1, jm - 2);
+    t.yw = sa * sb * N;    // exact: every partial sum is a multiple of 1/4 below 2^53
+    t.ncell = (double)n * (jm - 2) * N;
+    std::vector<double> s2(jm);
+    for (int j = 0; j < jm; j++) {
+        double s = std::sin(geo.thez[j]);
+        if (s < p.sinthe_min) s = p.sinthe_min;
+        if (s == 0.0) s = 1.0e-5;
+        s2[j] = s * s;
+    }
+    const double e = geo.inv_dthe2;
+    t.e = e;
+    t.mode_cp.assign((size_t)(NM + 1) * J, 0.0);
+    t.mode_dn.assign((size_t)(NM + 1) * J, 0.0);
+    for (int m = 0; m <= NM; m++) {
+        const double sl = std::sin(M_PI * m / N);
+        const double lam = 4.0 * sl * sl * geo.inv_dphi2;
+        double prev_cp = 0.0;
+        for (int jj = 0; jj < J; jj++) {
+            double a = e, c = e, b = -2.0 * e - lam / s2[jj + 1];
+            if (jj == 0) { a = 0.0; b = b + c43 * e; c = c - c13 * e; }
+            if (jj == J - 1) { c = 0.0; b = b + c43 * e; a = a - c13 * e; }
+            if (m == 0 && jj == 0) { b = 1.0; c = 0.0; }   // mode 0 is all-Neumann: q pinned at j = 1
+            const double d = (jj == 0) ? b : b - a * prev_cp;
+            t.mode_dn[(size_t)m * J + jj] = d;
+            t.mode_cp[(size_t)m * J + jj] = c / d;
+            prev_cp = t.mode_cp[(size_t)m * J + jj];
         }
     }
 }

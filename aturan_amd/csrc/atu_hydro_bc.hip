@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(64) k_level_serial(Shape sh, DevGeo g, StepCon
 // (METRIC_SPLIT + CORIOLIS_SPLIT branch, :104-161). One thread per (j, k) column.
 __global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepConsts c, const double* t,
                                                     const double* v, const double* w, const double* pstat,
-                                                    const double* level, double* phydro) {
+                                                    const double* level, double* phydro, float* phydro32) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (k >= sh.km) return;
@@ -79,12 +79,26 @@ __global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepCons
     double p = 0.0;
     double f_prev = f_at(0);
     phydro[(size_t)j * sh.ldk + k] = 0.0;
+    if (phydro32) phydro32[(size_t)j * sh.ldk + k] = 0.0f;
     for (int i = 1; i < sh.im; i++) {
         const double fi = f_at(i);
         p = p + 0.5 * (f_prev + fi) * c.dr;
-        phydro[((size_t)i * sh.jm + j) * sh.ldk + k] = p;
+        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+        phydro[x] = p;
+        if (phydro32) phydro32[x] = (float)p;
         f_prev = fi;
     }
+}
+
+template <typename A, typename B>
+__global__ void __launch_bounds__(256) k_convert(const A* src, B* dst, size_t n) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = (B)src[q];
+}
+
+__global__ void __launch_bounds__(256) k_fill(float* dst, float v, size_t n) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = v;
 }
 
 // ---- boundary conditions ----
@@ -230,10 +244,26 @@ __global__ void __launch_bounds__(256) k_species_clamp(BcCtx b) {
 }  // namespace
 
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
-                        const double* w, const double* pstat, double* level_out, double* phydro, hipStream_t s) {
+                        const double* w, const double* pstat, double* level_out, double* phydro, float* phydro32,
+                        hipStream_t s) {
     hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(64), 0, s, sh, g, c, t, pstat, level_out);
     hipLaunchKernelGGL(k_hydro_scan, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, sh, g, c, t, v, w, pstat,
-                       level_out, phydro);
+                       level_out, phydro, phydro32);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(const float* src, double* dst, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<float, double>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, v, n);
     return hipGetLastError();
 }
 

@@ -21,6 +21,7 @@ struct Geometry {
     std::vector<double> rm, inv_rm, inv_rm2, two_inv_rm, sponge, num2;
     // per j
     std::vector<double> sinthe, sinthe2, costhe, cotanthe, sinthe_bf, cor_hydro, sin_weight;
+    std::vector<double> pole_row;            // 1 where the sin(theta) floor of the metric is active
     // per (i,j)
     std::vector<double> inv_rmsinthe, inv_rm2sinthe, inv_rm2sinthe2, costhe_inv_rm2sinthe,
         v_metric, num3, den, inv_rmsinthe_s;
@@ -30,6 +31,19 @@ struct Geometry {
 };
 
 void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo);
+
+// Tables of the converging Poisson solve: Thomas factors of the radial lines (per (j, s), s = i-1,
+// the walls folded), of the per-mode theta systems of the barotropic correction (per (m, jj)), the
+// DFT cosines/sines in both layouts and the a-weighted column weight S2 = sum_i a_i / rm_i^2.
+struct SolveTables {
+    int ni = 0, N = 0, NM = 0, J = 0;
+    std::vector<double> line_cp, line_dn;   // [jm][ni]
+    std::vector<double> mode_cp, mode_dn;   // [NM+1][J]
+    std::vector<double> ct, st;             // [NM+1][N]
+    std::vector<double> ctT, stT;           // [N][NM+1]
+    double S2 = 0.0, yw = 0.0, ncell = 0.0, e = 0.0;
+};
+void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, SolveTables& t);
 
 // cp table of H2 for a CpH2 mode (1 normal, 2 equilibrium), J/(kg K).
 const std::vector<double>& cp_h2_table(int mode);

@@ -10,7 +10,7 @@ namespace atu {
 // Geometry tables in device memory (FP64), built by build_geometry().
 struct DevGeo {
     const double *rm, *inv_rm, *inv_rm2, *two_inv_rm, *sponge, *num2;                 // [im]
-    const double *costhe, *cotanthe, *sinthe_bf, *cor_hydro, *sin_weight;             // [jm]
+    const double *costhe, *cotanthe, *sinthe_bf, *cor_hydro, *sin_weight, *pole_row;  // [jm]
     const double *inv_rmsinthe, *inv_rm2sinthe, *inv_rm2sinthe2, *costhe_inv_rm2sinthe,
         *v_metric, *num3, *den, *inv_rmsinthe_s;                                       // [im][jm]
     const double* cp_tab;                                                              // [cp_n]
@@ -53,7 +53,12 @@ hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream
 // then the column scan into `phydro`.
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t,
                         const double* v, const double* w, const double* pstat, double* level_out,
-                        double* phydro, hipStream_t s);
+                        double* phydro, float* phydro32, hipStream_t s);
+
+// Element-wise precision conversion of n values (FP32 mirrors of FP64 fields and back).
+hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s);
+hipError_t launch_convert(const float* src, double* dst, size_t n, hipStream_t s);
+hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s);
 
 // Domain BC on the fields of the path that the reference's lists name: the 17 prognostic fields of
 // `y` (FP64; every value written also goes to each non-null mirror, FP64 `m64a`, `m64b`, FP32
@@ -85,6 +90,36 @@ hipError_t launch_rb_pass(const Shape& sh, const DevGeo& g, const StepConsts& c,
 // The four corner rows (i = 0, im-1 at j = 0, jm-1) and their seam copies, after the last sweep.
 template <typename P>
 hipError_t launch_p_corners(const Shape& sh, const StepConsts& c, P* p, hipStream_t s);
+
+// ---- the converging solve (atu_solve.hip) ----
+struct DevSolve {
+    const double *line_cp, *line_dn, *mode_cp, *mode_dn, *ct, *st, *ctT, *stT;
+    int ni, N, NM, J;
+    double S2, yw, ncell, e;
+};
+enum { DSUM_P = 0, DSUM_YSRC = 1, DSUM_SQ = 2, DSUM_RESID2 = 3 };
+// P: storage of the unknown, B: of the right-hand side, R: arithmetic of the line pass
+template <typename P, typename B, typename R>
+hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, int colour,
+                       hipStream_t s);
+template <typename P>
+hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s);
+template <typename P>
+hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s);
+template <typename P, typename B>
+hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
+                       double* As, hipStream_t s);
+// canonical-order interior sum of p (DSUM_P), y*v (DSUM_YSRC), v^2 (DSUM_SQ) or (v - A p)^2
+// (DSUM_RESID2, which can also store the residual in FP32 into r_out); `rows` holds
+// (im-2)*(jm-2) doubles; the total lands in *out (device)
+template <typename P>
+hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, const double* v, double* rows,
+                       double* out, hipStream_t s, float* r_out = nullptr);
+hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, double yw, double* b, hipStream_t s);
+hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s);
+template <typename P>
+hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
+                            hipStream_t s);
 
 // Residual r = src - Lc p over the interior: writes per-block partial sums of r^2 and src^2 into
 // `partials` (2 * nblocks doubles) and the fixed-order total into out[0] (r^2), out[1] (src^2).

@@ -88,15 +88,10 @@ __device__ __forceinline__ double cp_lapse_dev(const StageArgs<S>& a, double T) 
     return (a.c.r_h2 * ch + a.c.cp_a + a.c.cp_b) / a.c.cp_den;
 }
 
-template <typename S, int STAGE>
-__global__ void __launch_bounds__(256) k_rk_stage(const StageArgs<S> a) {
-    using R = S;   // arithmetic precision of the tendencies equals the stencil storage precision
+// The tendencies of one cell in arithmetic precision R from stencil storage S, folded in FP64.
+template <typename R, typename S, int STAGE>
+__device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int k) {
     const int nk = a.sh.km - 1;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = 1 + blockIdx.z;
-    if (k >= nk || j > a.sh.jm - 2) return;
-
     const int ldk = a.sh.ldk;
     const size_t plane = a.sh.plane, fs = a.sh.fs;
     const size_t row = ((size_t)i * a.sh.jm + j) * ldk;
@@ -254,6 +249,26 @@ __global__ void __launch_bounds__(256) k_rk_stage(const StageArgs<S> a) {
             const R rhs = -transport + diffusion * (inv_re + nue / (R)C.inv_sig_w) + FZ(ATU_DIS_SOURCE);
             fold<STAGE, CL_DIS>(a, ATU_DIS, c, Y0(ATU_DIS, X.c), (double)rhs);
         }
+    }
+}
+
+// FP64 storage: FP64 arithmetic. FP32 storage (the mixed path): FP32 arithmetic, except on the
+// rows next to the poles where the sin(theta) floor of the metric is active, which run the same
+// stencil in FP64 arithmetic (j is uniform across a wavefront, so the branch does not diverge).
+template <typename S, int STAGE>
+__global__ void __launch_bounds__(256) k_rk_stage(const StageArgs<S> a) {
+    const int nk = a.sh.km - 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 1 + blockIdx.z;
+    if (k >= nk || j > a.sh.jm - 2) return;
+    if constexpr (sizeof(S) == sizeof(double)) {
+        rk_cell<double, S, STAGE>(a, i, j, k);
+    } else {
+        if (a.geo.pole_row[j] != 0.0)
+            rk_cell<double, S, STAGE>(a, i, j, k);
+        else
+            rk_cell<float, S, STAGE>(a, i, j, k);
     }
 }
 

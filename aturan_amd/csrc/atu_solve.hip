@@ -1,0 +1,378 @@
+// Converging solve of the dynamic-pressure system (ATU_SOLVE_CONVERGE): zebra red-black radial line
+// relaxation plus an exact barotropic correction, to a relative residual target.
+//
+// The system is the red-black sweep's (planet/PressureSolver.h:350-491) with its wall passes
+// (:718-732) folded into the equations, so its fixed point is what the reference's sweeps approach.
+// The reference's own cures for this radially dominated operator are its options PRESS_LINE (exact
+// solve along r, :284-303, 497-570) and PRESS_BAROTROPIC (the column-mean mode by a DFT in phi and a
+// tridiagonal solve in theta, :49-67, 572-708); this file runs both every iteration:
+//
+//   line pass (colour 0, 1)   one thread per (j, k) column of a colour, Thomas along i with
+//                             host-built factors; the column's own values are overwritten in place
+//                             (the horizontal neighbours it reads belong to the other colour)
+//   walls                     radial and polar walls re-formed from the interior
+//   column residual           R(j, k) = sum_i a_i (b - A p) / S2, a_i = 3/2 at the end rows
+//   DFT / theta solves / iDFT the correction q(j, k) that zeroes R, added to every level
+//   residual norm             in the canonical order shared with the oracle
+//
+// Every sum runs in a fixed sequential order (the same order the CPU oracle uses), so the FP64
+// result is bit-identical to the oracle at equal iteration counts.
+//
+// Templates: P is the storage of the unknown, B of the right-hand side, R the arithmetic of the line
+// pass. The FP64 solve is <double, double, double>; the mixed path's inner solve of the refinement
+// is <float, float, float> on the correction e with the FP32 residual as its right-hand side.
+
+#include "atu_kernels.h"
+
+namespace atu {
+namespace {
+
+__device__ __forceinline__ double endw(int x, int lo, int hi) { return (x == lo || x == hi) ? 1.5 : 1.0; }
+
+template <typename P, typename B, typename R>
+__global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P* p, const B* b, int colour) {
+    const int N = sh.km - 1;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    if (j > sh.jm - 2) return;
+    const int k = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x) + ((colour + j) & 1);
+    if (k >= N) return;
+    const int ldk = sh.ldk, jm = sh.jm, ni = t.ni;
+    const R c13 = R(1.0) / R(3.0);
+    const int kw = (k == 0) ? N - 1 : k - 1, ke = (k == N - 1) ? 0 : k + 1;
+    const double* cp = t.line_cp + (size_t)j * ni;
+    const double* dn = t.line_dn + (size_t)j * ni;
+    R dprev = R(0.0);
+    for (int s = 0; s < ni; s++) {
+        const int i = s + 1;
+        const size_t row = ((size_t)i * jm + j) * ldk;
+        const size_t x = row + k;
+        const R num1 = (R)g.num1, num2 = (R)g.num2[i], num3 = (R)g.num3[(size_t)i * jm + j];
+        R th;
+        if (j == 1)
+            th = (num2 - num2 * c13) * (R)p[x + ldk];
+        else if (j == jm - 2)
+            th = (num2 - num2 * c13) * (R)p[x - ldk];
+        else
+            th = ((R)p[x + ldk] + (R)p[x - ldk]) * num2;
+        const R rhs = (R)b[x] - th - ((R)p[row + ke] + (R)p[row + kw]) * num3;
+        const R a = (s == 0) ? R(0.0) : ((s == ni - 1) ? num1 - num1 * c13 : num1);
+        const R dp = (rhs - a * dprev) / (R)dn[s];
+        p[x] = (P)dp;
+        dprev = dp;
+    }
+    const size_t stride = sh.plane;
+    size_t x = ((size_t)ni * jm + j) * ldk + k;
+    R xv = (R)p[x];
+    for (int s = ni - 2; s >= 0; s--) {
+        x -= stride;
+        xv = (R)p[x] - (R)cp[s] * xv;
+        p[x] = (P)xv;
+    }
+}
+
+// radial walls (j = 1..jm-2) and polar walls (i = 1..im-2), k = 0..N-1
+template <typename P>
+__global__ void __launch_bounds__(256) k_walls(Shape sh, double c43, double c13, P* p) {
+    const int N = sh.km - 1;
+    const int n1 = (sh.jm - 2) * N, n2 = (sh.im - 2) * N;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ldk = sh.ldk;
+    const size_t PL = sh.plane;
+    if (q < n1) {
+        const int j = 1 + q / N, k = q % N;
+        const size_t c = (size_t)j * ldk + k;
+        const int L = sh.im - 1;
+        p[c] = (P)(c43 * (double)p[c + PL] - c13 * (double)p[c + 2 * PL]);
+        p[c + L * PL] = (P)(c43 * (double)p[c + (L - 1) * PL] - c13 * (double)p[c + (L - 2) * PL]);
+    } else if (q < n1 + n2) {
+        const int r = q - n1;
+        const int i = 1 + r / N, k = r % N;
+        const size_t c = (size_t)i * PL + k;
+        const int J = sh.jm - 1;
+        p[c] = (P)(c43 * (double)p[c + ldk] - c13 * (double)p[c + 2 * ldk]);
+        p[c + (size_t)J * ldk] = (P)(c43 * (double)p[c + (size_t)(J - 1) * ldk] - c13 * (double)p[c + (size_t)(J - 2) * ldk]);
+    }
+}
+
+// k = km-1 is the copy of k = 0 for every (i, j)
+template <typename P>
+__global__ void __launch_bounds__(256) k_seam(Shape sh, P* p) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j >= sh.jm) return;
+    const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
+    p[row + sh.km - 1] = p[row];
+}
+
+template <typename P, typename B>
+__device__ __forceinline__ double resid_at(const Shape& sh, const DevGeo& g, const P* p, const B* b, int i, int j,
+                                           int k) {
+    const int N = sh.km - 1, ldk = sh.ldk;
+    const size_t row = ((size_t)i * sh.jm + j) * ldk;
+    const size_t x = row + k;
+    const size_t xw = row + (k == 0 ? N - 1 : k - 1), xe = row + (k == N - 1 ? 0 : k + 1);
+    const size_t ij = (size_t)i * sh.jm + j;
+    const double acc = ((double)p[x + sh.plane] + (double)p[x - sh.plane]) * g.num1
+                       + ((double)p[x + ldk] + (double)p[x - ldk]) * g.num2[i]
+                       + ((double)p[xe] + (double)p[xw]) * g.num3[ij];
+    return (double)b[x] - (acc - g.den[ij] * (double)p[x]);
+}
+
+// R(jj, k) = sum_i a_i r_i / S2 over each interior column
+template <typename P, typename B>
+__global__ void __launch_bounds__(256) k_col_resid(Shape sh, DevGeo g, DevSolve t, const P* p, const B* b,
+                                                   double* R) {
+    const int N = sh.km - 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    if (k >= N || j > sh.jm - 2) return;
+    double s = 0.0;
+    for (int i = 1; i < sh.im - 1; i++) s += endw(i, 1, sh.im - 2) * resid_at(sh, g, p, b, i, j, k);
+    R[(size_t)(j - 1) * N + k] = s / t.S2;
+}
+
+// forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin
+__global__ void __launch_bounds__(256) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int jj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (m > t.NM || jj >= t.J) return;
+    const int N = t.N, M1 = t.NM + 1;
+    const double* r = R + (size_t)jj * N;
+    double sc = 0.0, ss = 0.0;
+    for (int k = 0; k < N; k++) {
+        const double rv = r[k];
+        sc += rv * t.ctT[(size_t)k * M1 + m];
+        ss += rv * t.stT[(size_t)k * M1 + m];
+    }
+    Ac[(size_t)m * t.J + jj] = sc;
+    As[(size_t)m * t.J + jj] = ss;
+}
+
+// per mode and part: the folded theta system, mode 0 pinned at j = 1
+__global__ void k_baro_tri(DevSolve t, double c13, double* Ac, double* As) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * (t.NM + 1)) return;
+    const int m = q >> 1, part = q & 1;
+    const int J = t.J;
+    double* d = (part == 0 ? Ac : As) + (size_t)m * J;
+    if (part == 1 && (m == 0 || 2 * m == t.N)) {
+        for (int jj = 0; jj < J; jj++) d[jj] = 0.0;
+        return;
+    }
+    const double* cp = t.mode_cp + (size_t)m * J;
+    const double* dn = t.mode_dn + (size_t)m * J;
+    const double e = t.e;
+    double prev = 0.0;
+    for (int jj = 0; jj < J; jj++) {
+        double a = (jj == 0) ? 0.0 : e;
+        if (jj == J - 1) a = e - c13 * e;
+        double rhs = d[jj];
+        if (m == 0 && jj == 0) rhs = 0.0;
+        const double dp = (rhs - a * prev) / dn[jj];
+        d[jj] = dp;
+        prev = dp;
+    }
+    double x = d[J - 1];
+    for (int jj = J - 2; jj >= 0; jj--) {
+        x = d[jj] - cp[jj] * x;
+        d[jj] = x;
+    }
+}
+
+// inverse DFT of the correction and its addition to every interior level of the column
+template <typename P>
+__global__ void __launch_bounds__(256) k_dft_inv_add(Shape sh, DevSolve t, const double* Ac, const double* As, P* p) {
+    const int N = t.N, NM = t.NM, J = t.J;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int jj = blockIdx.y * blockDim.y + threadIdx.y;
+    if (k >= N || jj >= J) return;
+    double s = Ac[jj] * t.ct[k];
+    for (int m = 1; m < NM; m++)
+        s += 2.0 * (Ac[(size_t)m * J + jj] * t.ct[(size_t)m * N + k] + As[(size_t)m * J + jj] * t.st[(size_t)m * N + k]);
+    s += Ac[(size_t)NM * J + jj] * t.ct[(size_t)NM * N + k];
+    const double qv = s / N;
+    for (int i = 1; i < sh.im - 1; i++) {
+        const size_t x = ((size_t)i * sh.jm + jj + 1) * sh.ldk + k;
+        p[x] = (P)((double)p[x] + qv);
+    }
+}
+
+// ---- canonical interior sums: per row, 64-wide trees, chunks, rows and levels in sequence ----
+template <int MODE, typename P>
+__global__ void __launch_bounds__(1024) k_dsum_rows(Shape sh, DevGeo g, const P* p, const double* v, double* rows,
+                                                    float* r_out) {
+    const int N = sh.km - 1;
+    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
+    const int k = threadIdx.x;
+    double term = 0.0;
+    if (k < N) {
+        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+        if (MODE == DSUM_P) term = (double)p[x];
+        if (MODE == DSUM_YSRC) term = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * v[x];
+        if (MODE == DSUM_SQ) term = v[x] * v[x];
+        if (MODE == DSUM_RESID2) {
+            const double r = resid_at(sh, g, p, v, i, j, k);
+            if (r_out) r_out[x] = (float)r;
+            term = r * r;
+        }
+    }
+    for (int h = 32; h > 0; h >>= 1) term += __shfl_down(term, h, 64);
+    __shared__ double chunk[16];
+    if ((threadIdx.x & 63) == 0) chunk[threadIdx.x >> 6] = term;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double row = 0.0;
+        for (int c = 0; c < (int)(blockDim.x >> 6); c++) row += chunk[c];
+        rows[(size_t)(i - 1) * (sh.jm - 2) + (j - 1)] = row;
+    }
+}
+
+__global__ void __launch_bounds__(128) k_dsum_final(int ni, int nj, const double* rows, double* out) {
+    __shared__ double planes[256];
+    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+        double s = 0.0;
+        for (int j = 0; j < nj; j++) s += rows[(size_t)i * nj + j];
+        planes[i] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = 0.0;
+        for (int i = 0; i < ni; i++) total += planes[i];
+        *out = total;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_make_b(Shape sh, const double* src, const double* sc, double yw, double* b) {
+    const int N = sh.km - 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 1 + blockIdx.z;
+    if (k >= N || j > sh.jm - 2) return;
+    const double shift = sc[0] / yw;
+    const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+    b[x] = src[x] - shift;
+}
+
+// hold the interior mean: p -= (mean_now - mean_before) on every cell
+template <typename P>
+__global__ void __launch_bounds__(256) k_shift_all(Shape sh, const double* sc, int slot_before, int slot_now,
+                                                   double ncell, P* p) {
+    const size_t n = sh.fs;
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double mean0 = sc[slot_before] / ncell;
+    const double dm = sc[slot_now] / ncell - mean0;
+    p[q] = (P)((double)p[q] - dm);
+}
+
+// refinement: p += e on the interior (FP64 accumulation of the FP32 correction)
+__global__ void __launch_bounds__(256) k_add_correction(Shape sh, const float* e, double* p) {
+    const int N = sh.km - 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 1 + blockIdx.z;
+    if (k >= N || j > sh.jm - 2) return;
+    const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+    p[x] = p[x] + (double)e[x];
+}
+
+}  // namespace
+
+// ---- launchers ----
+template <typename P, typename B, typename R>
+hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, int colour,
+                       hipStream_t s) {
+    const int half = (sh.km - 1 + 1) / 2;
+    hipLaunchKernelGGL((k_line<P, B, R>), dim3((half + 63) / 64, (sh.jm - 2 + 3) / 4), dim3(64, 4), 0, s, sh, g, t, p,
+                       b, colour);
+    return hipGetLastError();
+}
+
+template <typename P>
+hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s) {
+    const int n = (sh.jm - 2 + sh.im - 2) * (sh.km - 1);
+    hipLaunchKernelGGL((k_walls<P>), dim3((n + 255) / 256), dim3(256), 0, s, sh, 4.0 / 3.0, 1.0 / 3.0, p);
+    return hipGetLastError();
+}
+
+template <typename P>
+hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s) {
+    hipLaunchKernelGGL((k_seam<P>), dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, p);
+    return hipGetLastError();
+}
+
+template <typename P, typename B>
+hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
+                       double* As, hipStream_t s) {
+    const int N = sh.km - 1;
+    hipLaunchKernelGGL((k_col_resid<P, B>), dim3((N + 63) / 64, (sh.jm - 2 + 3) / 4), dim3(64, 4), 0, s, sh, g, t,
+                       (const P*)p, b, R);
+    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, (t.J + 3) / 4), dim3(64, 4), 0, s, t, (const double*)R,
+                       Ac, As);
+    hipLaunchKernelGGL(k_baro_tri, dim3((2 * (t.NM + 1) + 63) / 64), dim3(64), 0, s, t, 1.0 / 3.0, Ac, As);
+    hipLaunchKernelGGL((k_dft_inv_add<P>), dim3((N + 63) / 64, (t.J + 3) / 4), dim3(64, 4), 0, s, sh, t,
+                       (const double*)Ac, (const double*)As, p);
+    return hipGetLastError();
+}
+
+template <typename P>
+hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, const double* v, double* rows,
+                       double* out, hipStream_t s, float* r_out) {
+    const int N = sh.km - 1;
+    const int threads = ((N + 63) / 64) * 64;
+    const dim3 grid(sh.jm - 2, sh.im - 2);
+    switch (mode) {
+        case DSUM_P:
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_P, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr);
+            break;
+        case DSUM_YSRC:
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_YSRC, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr);
+            break;
+        case DSUM_SQ:
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_SQ, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr);
+            break;
+        default:
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_RESID2, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, r_out);
+            break;
+    }
+    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(128), 0, s, sh.im - 2, sh.jm - 2, (const double*)rows, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, double yw, double* b, hipStream_t s) {
+    hipLaunchKernelGGL(k_make_b, dim3((sh.km - 1 + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0, s, sh,
+                       src, sc, yw, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s) {
+    hipLaunchKernelGGL(k_add_correction, dim3((sh.km - 1 + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0,
+                       s, sh, e, p);
+    return hipGetLastError();
+}
+
+template <typename P>
+hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
+                            hipStream_t s) {
+    hipLaunchKernelGGL((k_shift_all<P>), dim3((unsigned)((sh.fs + 255) / 256)), dim3(256), 0, s, sh, sc, slot_before,
+                       slot_now, ncell, p);
+    return hipGetLastError();
+}
+
+template hipError_t launch_line<double, double, double>(const Shape&, const DevGeo&, const DevSolve&, double*,
+                                                        const double*, int, hipStream_t);
+template hipError_t launch_line<float, float, float>(const Shape&, const DevGeo&, const DevSolve&, float*,
+                                                     const float*, int, hipStream_t);
+template hipError_t launch_baro<double, double>(const Shape&, const DevGeo&, const DevSolve&, double*, const double*,
+                                                double*, double*, double*, hipStream_t);
+template hipError_t launch_baro<float, float>(const Shape&, const DevGeo&, const DevSolve&, float*, const float*,
+                                              double*, double*, double*, hipStream_t);
+template hipError_t launch_walls<double>(const Shape&, double*, hipStream_t);
+template hipError_t launch_walls<float>(const Shape&, float*, hipStream_t);
+template hipError_t launch_seam<double>(const Shape&, double*, hipStream_t);
+template hipError_t launch_seam<float>(const Shape&, float*, hipStream_t);
+template hipError_t launch_dsum<double>(int, const Shape&, const DevGeo&, const double*, const double*, double*,
+                                        double*, hipStream_t, float*);
+template hipError_t launch_shift_all<double>(const Shape&, const double*, int, int, double, double*, hipStream_t);
+
+}  // namespace atu

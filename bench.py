@@ -1,10 +1,12 @@
 """Benchmark of the dynamical-core step on one MI355X.
 
-A step is one outer time step of the benchmark loop on the 41x181x361 shell: hydrostatic prep,
-the four fused RK4 stages, the domain BC and the red-black Poisson solve, with every input already
-resident in HBM. Prints ONE JSON line (rank 0) with the throughput in steps/s, the roofline of the
-dominant kernel (the fused RK stage) measured with HIP events on the library's own stream, and the
-CPU oracle timed on a bounded sample of the same workload.
+A step is one outer time step of the 41x181x361 shell with every input resident in HBM: the
+hydrostatic prep, the four fused RK4 stages, the domain BC and the red-black Poisson solve of p_dyn
+to a relative residual of 1e-8. `value` is the plain FP64 GPU step (the benchmark's headline
+configuration); the same line carries the mixed-precision step (FP32 stencils, stage states and
+smoothing; FP64 accumulation, residual and refinement) with its per-field error against the FP64
+run after 1, 10 and 100 steps, the roofline of the dominant kernel measured with HIP events on the
+library's own stream, and the CPU oracle timed on a bounded sample of the same step.
 """
 from __future__ import annotations
 
@@ -20,20 +22,30 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0          # MI355X HBM3E, vendor figure
 N_INT = 39 * 179 * 360         # integrated interior cells
-# Algorithmic bytes of the four fused stages: array passes per interior cell (stage 0: 80,
-# stages 1-2: 118 each, stage 3: 102) at 8 B per FP64 value.
-RK_PASSES = (80, 118, 118, 102)
+# Algorithmic HBM bytes of the four fused RK stages per interior cell: array passes (stage 0: 80,
+# stages 1-2: 118 each, stage 3: 102) x bytes per value of each array.
+RK_PASSES_FP64 = (80, 118, 118, 102)
+# mixed: stencil states, frozen inputs, p_dyn, p_hydro and aux in FP32; y_n and acc in FP64
+RK_BYTES_MIXED = (
+    42 * 4 + 19 * 8 + 19 * 4 + 19 * 8,                         # stage 0: reads 19+2+21 (4 B) + y_n; writes acc, next
+    42 * 4 + 19 * 8 + 19 * 8 + 19 * 8 + 19 * 4,                # stages 1-2: + acc read
+    42 * 4 + 19 * 8 + 19 * 8 + 19 * 8 + 19 * 4,
+    42 * 4 + 19 * 8 + 19 * 8 + 19 * 8 + 19 * 4 + 3 * 4,        # stage 3: writes y (FP64) + mirror (FP32) + aux
+)
+ERR_FIELDS = {"u_zonal(w)": "w", "v": "v", "w_radial(u)": "u", "T": "t", "CH4": "ch4", "p": "p_dyn"}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--sweeps", type=int, default=20)
-    ap.add_argument("--omega", type=float, default=1.5)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--tol", type=float, default=1e-8, help="relative residual of the Poisson solve")
+    ap.add_argument("--inner", type=int, default=3, help="FP32 smoothing iterations per refinement (mixed)")
+    ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-errors", action="store_true", help="skip the 1/10/100-step mixed-vs-FP64 comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of known size (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -62,26 +74,84 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(sweeps, omega, budget_s):
-    import numpy as np  # noqa: F401
+def initial_state():
     import aturan_amd as A
-    from oracle import binding as O
+    from oracle import binding as O   # the oracle's BC of the seeded state: identical to the GPU's (tested)
     g = A.default_grid(1)
     p = A.default_params(g)
     F = A.synthetic_state(g, p, 20261203)
     O.bc(g, p, F)
-    O.step(g, p, F, 1, sweeps, omega)           # warm-up
+    return g, p, F
+
+
+def timed_run(core, steps, warmup, opts, torch):
+    for _ in range(warmup):
+        core.step(1, **opts)
+    core.synchronize()
+    torch.cuda.synchronize()
+    core.profile_begin()
+    t0 = time.perf_counter()
+    st = core.step(steps, **opts)
+    core.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    prof = {c: core.profile_query(c) for c in ("rk_stage", "line", "baro", "residual", "hydro", "bc", "proj_src")}
+    core.profile_end()
+    return el, st, prof
+
+
+def rel_errors(a, b):
+    import numpy as np
+    out = {}
+    for name, f in ERR_FIELDS.items():
+        x, y = a[f][1:-1, 1:-1, :-1], b[f][1:-1, 1:-1, :-1]
+        d = x - y
+        out[name] = {"max": float(np.max(np.abs(d)) / max(np.max(np.abs(y)), 1e-300)),
+                     "rms": float(np.sqrt(np.mean(d * d)) / max(np.sqrt(np.mean(y * y)), 1e-300))}
+    return out
+
+
+def mixed_errors(g, p, F, opts64, optsmx):
+    """Mixed vs FP64 GPU after 1, 10 and 100 steps from the same state."""
+    import aturan_amd as A
+    res = {}
+    with A.UranusCore(g, p, "fp64") as c64, A.UranusCore(g, p, "mixed") as cmx:
+        c64.set_state(F)
+        cmx.set_state(F)
+        done = 0
+        for n in (1, 10, 100):
+            c64.step(n - done, **opts64)
+            cmx.step(n - done, **optsmx)
+            done = n
+            a = {f: cmx.get_field(f) for f in set(ERR_FIELDS.values())}
+            b = {f: c64.get_field(f) for f in set(ERR_FIELDS.values())}
+            res[str(n)] = rel_errors(a, b)
+    return res
+
+
+def cpu_baseline(tol, budget_s):
+    import aturan_amd as A
+    from oracle import binding as O
+    g, p, F = initial_state()
+
+    def one():
+        O.hydro(g, p, F)
+        O.rk4(g, p, F)
+        O.bc(g, p, F)
+        O.pressure_converge(g, p, F, 200, tol)
+
+    one()                                  # warm-up
     n, t0 = 0, time.perf_counter()
     while True:
-        O.step(g, p, F, 1, sweeps, omega)
+        one()
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 20:
+        if el >= budget_s or n >= 50:
             break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": n / el, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} steps (hydro + RK4 + BC + {sweeps} red-black sweeps at omega {omega}) of the "
-                      f"41x181x361 synthetic state, FP64 OpenMP oracle"}
+    return {"value": round(n / el, 4), "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} steps (hydro + RK4 + BC + Poisson to {tol:g}) of the 41x181x361 synthetic state, "
+                      f"FP64 OpenMP oracle (C, -O2), after 1 warm-up step"}
 
 
 def main():
@@ -93,60 +163,56 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    import numpy as np  # noqa: F401
     import torch
     import aturan_amd as A
-    from oracle import binding as O
 
-    g = A.default_grid(1)
-    p = A.default_params(g)
-    F = A.synthetic_state(g, p, 20261203)
-    O.bc(g, p, F)
-    core = A.UranusCore(g, p)
-    core.set_state(F)
-    for _ in range(args.warmup):
-        core.step(1, "reference", sweeps=args.sweeps, omega=args.omega)
-    core.synchronize()
-    torch.cuda.synchronize()
-    core.profile_begin()
-    t0 = time.perf_counter()
-    st = core.step(args.steps, "reference", sweeps=args.sweeps, omega=args.omega)
-    core.synchronize()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    rk_ms, rk_n = core.profile_query("rk_stage")
-    rb_ms, rb_n = core.profile_query("rb_pass")
-    core.profile_end()
-    steps_per_s = args.steps / el
-    # algorithmic bytes of all stage launches / their summed duration = bytes per launch / avg duration
-    bytes_all = sum(RK_PASSES) * 8 * N_INT * (rk_n // 4)
-    achieved = bytes_all / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
-    line = {
-        "metric": "RK4 steps/s at 41x181x361 (plain FP64 GPU step)",
-        "value": round(steps_per_s, 3),
-        "unit": "steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * el / args.steps, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"41x181x361 shell: hydro prep + RK4 + BC + {args.sweeps} red-black sweeps "
-                               f"omega {args.omega}, FP64",
-                   "cell_updates_per_s": round(steps_per_s * N_INT, 1),
-                   "last_rel_resid": st.rel_resid},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "k_rk_stage (fused RHS + RK fold)",
-                     "avg_launch_ms": round(rk_ms / max(rk_n, 1), 5)},
-        "kernels": {"rb_pass_avg_ms": round(rb_ms / max(rb_n, 1), 5), "rb_pass_launches": rb_n},
-    }
-    core.close()
+    g, p, F = initial_state()
+    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60)
+    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner)
+    line = {"metric": "RK4 steps/s at 41x181x361 (plain FP64 GPU step, Poisson to 1e-8)", "value": None,
+            "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic"}
+    cfg = {"workload": f"41x181x361 shell, seeded synthetic state: hydro prep + RK4 + BC + red-black line/barotropic "
+                       f"Poisson solve to relative residual {args.tol:g}, {args.steps} steps"}
+    if args.mode in ("both", "fp64"):
+        with A.UranusCore(g, p, "fp64") as core:
+            core.set_state(F)
+            el, st, prof = timed_run(core, args.steps, args.warmup, opts64, torch)
+        sps = args.steps / el
+        rk_ms, rk_n = prof["rk_stage"]
+        achieved = sum(RK_PASSES_FP64) * 8 * N_INT * (rk_n // 4) / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
+        line.update(value=round(sps, 3), ms_per_step=round(1e3 * el / args.steps, 4))
+        cfg.update(cell_updates_per_s=round(sps * N_INT, 1), solve_iterations_last_step=st.sweeps_done,
+                   solve_rel_resid_last_step=st.rel_resid)
+        line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profiles(),
+                            "kernel": "k_rk_stage<double> (fused RHS + RK fold)",
+                            "avg_launch_ms": round(rk_ms / max(rk_n, 1), 5),
+                            "bytes_per_launch": sum(RK_PASSES_FP64) * 8 * N_INT / 4}
+        line["kernels_ms_per_step"] = {k: round(v[0] / args.steps, 4) for k, v in prof.items()}
+    if args.mode in ("both", "mixed"):
+        with A.UranusCore(g, p, "mixed") as core:
+            core.set_state(F)
+            el, st, prof = timed_run(core, args.steps, args.warmup, optsmx, torch)
+        sps = args.steps / el
+        rk_ms, rk_n = prof["rk_stage"]
+        achieved = sum(RK_BYTES_MIXED) * N_INT * (rk_n // 4) / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
+        mixed = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 * el / args.steps, 4),
+                 "speedup_vs_fp64": round(sps / line["value"], 4) if line["value"] else None,
+                 "rk_stage_GBps": round(achieved, 1), "rk_stage_frac": round(achieved / PEAK_HBM_GBS, 4),
+                 "solve_iterations_last_step": st.sweeps_done, "solve_refinements_last_step": st.outer_iters,
+                 "solve_rel_resid_last_step": st.rel_resid,
+                 "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()}}
+        if args.mode == "mixed":
+            line.update(value=round(sps, 3), ms_per_step=mixed["ms_per_step"], dtype="f32/f64",
+                        metric="RK4 steps/s at 41x181x361 (mixed FP32/FP64 GPU step, Poisson to 1e-8)")
+        if not args.no_errors and args.mode == "both":
+            mixed["rel_error_vs_fp64"] = mixed_errors(g, p, F, opts64, optsmx)
+        line["mixed"] = mixed
+    line["config"] = cfg
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.sweeps, args.omega, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

@@ -119,13 +119,21 @@ typedef enum atu_solve_mode {
     ATU_SOLVE_REFERENCE = 0, /* `sweeps` red-black sweeps with the wall BCs inside each sweep,
                                 exactly planet/PressureSolver.h:318-757 (omega = 1 is the
                                 reference's Gauss-Seidel; other omega is SOR)                 */
-    ATU_SOLVE_CONVERGE = 1   /* iterate to ||src - Lc p||_2 / ||src||_2 <= rel_tol            */
+    ATU_SOLVE_CONVERGE = 1   /* the same system with every wall folded in, iterated to
+                                ||b - A p||_2 / ||b||_2 <= rel_tol, where b is the source less
+                                its component along the all-Neumann left null vector; zebra
+                                red-black radial line relaxation (the reference's PRESS_LINE)
+                                plus an exact barotropic correction (its PRESS_BAROTROPIC) per
+                                iteration; the interior mean of p is held. On a mixed context:
+                                FP64 residual and accumulation, FP32 smoothing (iterative
+                                refinement)                                                    */
 } atu_solve_mode;
 
 typedef struct atu_solver_opts {
     int32_t mode;            /* atu_solve_mode                                                */
-    int32_t sweeps;          /* REFERENCE: sweeps to run                                      */
-    int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing sweeps                     */
+    int32_t sweeps;          /* REFERENCE: sweeps to run. CONVERGE on a mixed context: FP32
+                                smoothing iterations per FP64 refinement (<= 0: 4)            */
+    int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing iterations                 */
     int32_t pad_;
     double omega;            /* relaxation factor (1 = Gauss-Seidel)                          */
     double rel_tol;          /* CONVERGE: relative residual target (FP64)                     */

@@ -41,6 +41,8 @@ def load() -> C.CDLL:
         "orc_bc": (C.c_int, [V, V, _DP]),
         "orc_pressure": (C.c_int, [V, V, _DP, C.c_int, C.c_double, _DP]),
         "orc_residual": (C.c_int, [V, V, _DP, _DP, _DP, _DP]),
+        "orc_pressure_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_double, C.POINTER(C.c_int),
+                                            C.POINTER(C.c_double), _DP]),
         "orc_step": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double]),
     }
     for n, (res, args) in sig.items():
@@ -101,6 +103,15 @@ def pressure(grid, params, fields: np.ndarray, sweeps: int, omega: float) -> np.
     src = np.zeros(fields.shape[1:])
     _ok(load().orc_pressure(_p(grid), _p(params), _d(fields), sweeps, omega, _d(src)), "orc_pressure")
     return src
+
+
+def pressure_converge(grid, params, fields: np.ndarray, max_iters: int, rel_tol: float):
+    """Line + barotropic solve to rel_tol; returns (iterations, relative residual, clamped source)."""
+    src = np.zeros(fields.shape[1:])
+    it, rel = C.c_int(), C.c_double()
+    _ok(load().orc_pressure_converge(_p(grid), _p(params), _d(fields), max_iters, rel_tol, C.byref(it), C.byref(rel),
+                                     _d(src)), "orc_pressure_converge")
+    return it.value, rel.value, src
 
 
 def residual(grid, params, fields: np.ndarray, src: np.ndarray) -> tuple[float, float]:

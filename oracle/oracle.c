@@ -26,6 +26,9 @@
 #include "../include/atu.h"
 
 #define IDX(i, j, k) (((size_t)(i) * jm + (j)) * km + (k))
+#ifndef M_PI
+#define M_PI 3.14159265358979323846   /* glibc's value, as the reference's <cmath> has it */
+#endif
 
 static double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
 static double dclamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -712,6 +715,308 @@ int orc_pressure(const atu_grid* g, const atu_params* p, double* fields, int swe
         wall_passes(im, jm, km, pd);
     }
     if (!src_out) free(src);
+    pgeo_free(&q);
+    geo_free(&G);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* Converging solve of the same folded-wall system (the library's ATU_SOLVE_CONVERGE): zebra     */
+/* red-black radial line relaxation (the reference's PRESS_LINE, planet/PressureSolver.h:284-303, */
+/* 497-570, with the polar walls folded too) and an exact barotropic correction of the            */
+/* a-weighted column residual by a DFT in phi and a tridiagonal solve in theta per mode (the      */
+/* reference's PRESS_BAROTROPIC, :49-67, 572-708). The source is made consistent with the         */
+/* all-Neumann left null vector y = a_i b_j (3/2 at the first and last interior row in i and j,   */
+/* 1 elsewhere) by a uniform shift; the interior mean of p is held (as PRESS_LINE does, :748-755).*/
+/* Every sum runs in a fixed sequential order shared with the GPU kernels.                        */
+/* ------------------------------------------------------------------------------------------- */
+static double endw(int x, int lo, int hi) { return (x == lo || x == hi) ? 1.5 : 1.0; }
+
+/* The canonical order of every interior sum of the converging solve, shared with the GPU: per
+ * (i, j) row, chunks of 64 consecutive k reduced by a halving tree (v[l] += v[l + h], h = 32..1,
+ * missing k as 0), the chunks of a row added in sequence, the rows of a level in sequence (j), the
+ * levels in sequence (i). `v` is a dense field; only interior cells are read. */
+static double dsum_interior(int im, int jm, int km, const double* v) {
+    const int n = km - 1;
+    const int nch = (n + 63) / 64;
+    double total = 0.0;
+    for (int i = 1; i < im - 1; i++) {
+        double plane = 0.0;
+        for (int j = 1; j < jm - 1; j++) {
+            double row = 0.0;
+            for (int c = 0; c < nch; c++) {
+                double t[64];
+                for (int l = 0; l < 64; l++) {
+                    const int k = 64 * c + l;
+                    t[l] = (k < n) ? v[IDX(i, j, k)] : 0.0;
+                }
+                for (int h = 32; h > 0; h >>= 1)
+                    for (int l = 0; l < h; l++) t[l] += t[l + h];
+                row += t[0];
+            }
+            plane += row;
+        }
+        total += plane;
+    }
+    return total;
+}
+
+/* Thomas factors of one column (depend on i and j only): cp[i], dn[i] for i = 1..im-2. */
+static void line_factors(const orc_pgeo* q, int im, int jm, int j, double* cp, double* dn) {
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    const int n = im - 2;
+    for (int s = 0; s < n; s++) {
+        const int i = s + 1;
+        const double num1 = q->num1, num2 = q->num2[i];
+        double diag = -q->den[i * jm + j];
+        if (j == 1 || j == jm - 2) diag = diag + num2 * c43;
+        double a = num1, c = num1;
+        if (s == 0) { diag = diag + num1 * c43; c = num1 - num1 * c13; a = 0.0; }
+        if (s == n - 1) { diag = diag + num1 * c43; a = num1 - num1 * c13; c = 0.0; }
+        if (n == 1) { a = 0.0; c = 0.0; }
+        const double d = (s == 0) ? diag : diag - a * cp[s - 1];
+        dn[s] = d;
+        cp[s] = c / d;
+    }
+}
+
+static void line_pass(const orc_pgeo* q, int im, int jm, int km, double* pd, const double* b, int colour,
+                      const double* cpt, const double* dnt) {
+    const double c13 = 1.0 / 3.0;
+    const int n = km - 1, ni = im - 2;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 1; j < jm - 1; j++)
+        for (int k = 0; k < n; k++) {
+            if (((j + k) & 1) != colour) continue;
+            const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+            const double* cp = cpt + (size_t)j * ni;
+            const double* dn = dnt + (size_t)j * ni;
+            double dprev = 0.0;
+            for (int s = 0; s < ni; s++) {
+                const int i = s + 1;
+                const double num1 = q->num1, num2 = q->num2[i], num3 = q->num3[i * jm + j];
+                double th;
+                if (j == 1) th = (num2 - num2 * c13) * pd[IDX(i, 2, k)];
+                else if (j == jm - 2) th = (num2 - num2 * c13) * pd[IDX(i, jm - 3, k)];
+                else th = (pd[IDX(i, j + 1, k)] + pd[IDX(i, j - 1, k)]) * num2;
+                const double rhs = b[IDX(i, j, k)] - th - (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)]) * num3;
+                const double a = (s == 0) ? 0.0 : ((s == ni - 1) ? num1 - num1 * c13 : num1);
+                const double dp = (rhs - a * dprev) / dn[s];
+                pd[IDX(i, j, k)] = dp;
+                dprev = dp;
+            }
+            double x = pd[IDX(ni, j, k)];
+            for (int s = ni - 2; s >= 0; s--) {
+                x = pd[IDX(s + 1, j, k)] - cp[s] * x;
+                pd[IDX(s + 1, j, k)] = x;
+            }
+        }
+}
+
+/* residual r = b - A p with the stored (folded) walls, written into r (interior) */
+static void resid_field(const orc_pgeo* q, int im, int jm, int km, const double* pd, const double* b, double* r) {
+    const int n = km - 1;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < n; k++) {
+                const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+                const double acc = (pd[IDX(i + 1, j, k)] + pd[IDX(i - 1, j, k)]) * q->num1
+                                   + (pd[IDX(i, j + 1, k)] + pd[IDX(i, j - 1, k)]) * q->num2[i]
+                                   + (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)]) * q->num3[i * jm + j];
+                r[IDX(i, j, k)] = b[IDX(i, j, k)] - (acc - q->den[i * jm + j] * pd[IDX(i, j, k)]);
+            }
+}
+
+typedef struct {
+    int N, NM, J;
+    double *ct, *st;        /* [NM+1][N] */
+    double *cpm, *dnm;      /* per mode Thomas factors [NM+1][J] */
+    double S2;
+} orc_baro;
+
+static void baro_init(orc_baro* B, const orc_geo* G, const orc_pgeo* q, const atu_params* p) {
+    const int im = G->im, jm = G->jm, km = G->km;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    B->N = km - 1;
+    B->NM = B->N / 2;
+    B->J = jm - 2;
+    const int N = B->N, NM = B->NM, J = B->J;
+    B->ct = malloc(sizeof(double) * (NM + 1) * N);
+    B->st = malloc(sizeof(double) * (NM + 1) * N);
+    for (int m = 0; m <= NM; m++)
+        for (int k = 0; k < N; k++) {
+            const double a = 2.0 * M_PI * (double)((long)m * k % N) / N;
+            B->ct[(size_t)m * N + k] = cos(a);
+            B->st[(size_t)m * N + k] = sin(a);
+        }
+    double S2 = 0.0;
+    for (int i = 1; i < im - 1; i++) {
+        const double rm = metric_radius(p, G->radz[i]);
+        S2 += endw(i, 1, im - 2) * (1.0 / (rm * rm));
+    }
+    B->S2 = S2;
+    /* the solver's own sine table */
+    double* s2 = malloc(sizeof(double) * jm);
+    for (int j = 0; j < jm; j++) {
+        double s = sin(G->thez[j]);
+        if (s < p->sinthe_min) s = p->sinthe_min;
+        if (s == 0.0) s = 1.0e-5;
+        s2[j] = s * s;
+    }
+    B->cpm = malloc(sizeof(double) * (NM + 1) * J);
+    B->dnm = malloc(sizeof(double) * (NM + 1) * J);
+    const double e = G->inv_dthe2;
+    for (int m = 0; m <= NM; m++) {
+        const double sl = sin(M_PI * m / N);
+        const double lam = 4.0 * sl * sl * G->inv_dphi2;
+        double prev_cp = 0.0;
+        for (int jj = 0; jj < J; jj++) {
+            double a = e, c = e, b = -2.0 * e - lam / s2[jj + 1];
+            if (jj == 0) { a = 0.0; b = b + c43 * e; c = c - c13 * e; }
+            if (jj == J - 1) { c = 0.0; b = b + c43 * e; a = a - c13 * e; }
+            if (m == 0 && jj == 0) { b = 1.0; c = 0.0; }   /* mode 0: q pinned at j = 1 */
+            const double d = (jj == 0) ? b : b - a * prev_cp;
+            B->dnm[(size_t)m * J + jj] = d;
+            B->cpm[(size_t)m * J + jj] = c / d;
+            prev_cp = B->cpm[(size_t)m * J + jj];
+        }
+    }
+    (void)q;
+    free(s2);
+}
+
+static void baro_free(orc_baro* B) { free(B->ct); free(B->st); free(B->cpm); free(B->dnm); }
+
+/* one barotropic correction: r is a scratch field */
+static void baro_correct(const orc_baro* B, const orc_geo* G, const orc_pgeo* q, double* pd, const double* b,
+                         double* r) {
+    const int im = G->im, jm = G->jm, km = G->km, N = B->N, NM = B->NM, J = B->J;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0, e = G->inv_dthe2;
+    resid_field(q, im, jm, km, pd, b, r);
+    double* R = malloc(sizeof(double) * J * N);
+    double* Ac = malloc(sizeof(double) * (NM + 1) * J);
+    double* As = malloc(sizeof(double) * (NM + 1) * J);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int jj = 0; jj < J; jj++)
+        for (int k = 0; k < N; k++) {
+            double s = 0.0;
+            for (int i = 1; i < im - 1; i++) s += endw(i, 1, im - 2) * r[IDX(i, jj + 1, k)];
+            R[(size_t)jj * N + k] = s / B->S2;
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int m = 0; m <= NM; m++)
+        for (int jj = 0; jj < J; jj++) {
+            double sc = 0.0, ss = 0.0;
+            for (int k = 0; k < N; k++) {
+                sc += R[(size_t)jj * N + k] * B->ct[(size_t)m * N + k];
+                ss += R[(size_t)jj * N + k] * B->st[(size_t)m * N + k];
+            }
+            Ac[(size_t)m * J + jj] = sc;
+            As[(size_t)m * J + jj] = ss;
+        }
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < 2 * (NM + 1); t++) {
+        const int m = t >> 1, part = t & 1;
+        double* d = (part == 0 ? Ac : As) + (size_t)m * J;
+        if (part == 1 && (m == 0 || 2 * m == N)) { for (int jj = 0; jj < J; jj++) d[jj] = 0.0; continue; }
+        const double* cp = B->cpm + (size_t)m * J;
+        const double* dn = B->dnm + (size_t)m * J;
+        double prev = 0.0;
+        for (int jj = 0; jj < J; jj++) {
+            double a = (jj == 0) ? 0.0 : e;
+            if (jj == J - 1) a = e - c13 * e;
+            double rhs = d[jj];
+            if (m == 0 && jj == 0) rhs = 0.0;
+            const double dp = (rhs - a * prev) / dn[jj];
+            d[jj] = dp;
+            prev = dp;
+        }
+        double x = d[J - 1];
+        for (int jj = J - 2; jj >= 0; jj--) {
+            x = d[jj] - cp[jj] * x;
+            d[jj] = x;
+        }
+    }
+    (void)c43;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int jj = 0; jj < J; jj++)
+        for (int k = 0; k < N; k++) {
+            double s = Ac[jj] * B->ct[k];
+            for (int m = 1; m < NM; m++)
+                s += 2.0 * (Ac[(size_t)m * J + jj] * B->ct[(size_t)m * N + k]
+                            + As[(size_t)m * J + jj] * B->st[(size_t)m * N + k]);
+            s += Ac[(size_t)NM * J + jj] * B->ct[(size_t)NM * N + k];
+            const double qv = s / N;
+            for (int i = 0; i < im; i++) pd[IDX(i, jj + 1, k)] += qv;
+        }
+    free(R); free(Ac); free(As);
+}
+
+int orc_pressure_converge(const atu_grid* g, const atu_params* p, double* fields, int max_iters, double rel_tol,
+                          int* iters_out, double* rel_out, double* src_out) {
+    orc_geo G;
+    geo_init(&G, g, p);
+    orc_pgeo q;
+    pgeo_init(&q, &G, p);
+    orc_baro B;
+    baro_init(&B, &G, &q, p);
+    const int im = g->im, jm = g->jm, km = g->km, n = km - 1, ni = im - 2;
+    const size_t n3 = (size_t)im * jm * km;
+    double* src = src_out ? src_out : malloc(sizeof(double) * n3);
+    double* b = malloc(sizeof(double) * n3);
+    double* r = malloc(sizeof(double) * n3);
+    proj_prep_src(&G, &q, p, fields, src);
+    double* pd = fields + (size_t)ATU_P_DYN * n3;
+    /* consistency shift and the mean to hold */
+    memset(r, 0, sizeof(double) * n3);
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < n; k++) r[IDX(i, j, k)] = (endw(i, 1, im - 2) * endw(j, 1, jm - 2)) * src[IDX(i, j, k)];
+    const double ys = dsum_interior(im, jm, km, r);
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < n; k++) r[IDX(i, j, k)] = endw(i, 1, im - 2) * endw(j, 1, jm - 2);
+    const double yw = dsum_interior(im, jm, km, r);
+    const double shift = ys / yw;
+    const double ncell = (double)ni * (jm - 2) * n;
+    const double mean0 = dsum_interior(im, jm, km, pd) / ncell;
+    memset(b, 0, sizeof(double) * n3);
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < n; k++) {
+                b[IDX(i, j, k)] = src[IDX(i, j, k)] - shift;
+                r[IDX(i, j, k)] = b[IDX(i, j, k)] * b[IDX(i, j, k)];
+            }
+    const double bnorm = sqrt(dsum_interior(im, jm, km, r));
+    double* cpt = malloc(sizeof(double) * jm * ni);
+    double* dnt = malloc(sizeof(double) * jm * ni);
+    for (int j = 1; j < jm - 1; j++) line_factors(&q, im, jm, j, cpt + (size_t)j * ni, dnt + (size_t)j * ni);
+    int it = 0;
+    double rel = INFINITY;
+    while (it < max_iters) {
+        line_pass(&q, im, jm, km, pd, b, 0, cpt, dnt);
+        line_pass(&q, im, jm, km, pd, b, 1, cpt, dnt);
+        wall_passes(im, jm, km, pd);
+        baro_correct(&B, &G, &q, pd, b, r);
+        wall_passes(im, jm, km, pd);
+        it++;
+        resid_field(&q, im, jm, km, pd, b, r);
+        for (int i = 1; i < im - 1; i++)
+            for (int j = 1; j < jm - 1; j++)
+                for (int k = 0; k < n; k++) r[IDX(i, j, k)] = r[IDX(i, j, k)] * r[IDX(i, j, k)];
+        const double rr = dsum_interior(im, jm, km, r);
+        rel = bnorm > 0.0 ? sqrt(rr) / bnorm : sqrt(rr);
+        if (rel <= rel_tol) break;
+    }
+    /* hold the interior mean */
+    const double dm = dsum_interior(im, jm, km, pd) / ncell - mean0;
+    for (size_t x = 0; x < n3; x++) pd[x] -= dm;
+    if (iters_out) *iters_out = it;
+    if (rel_out) *rel_out = rel;
+    free(cpt); free(dnt); free(b); free(r);
+    if (!src_out) free(src);
+    baro_free(&B);
     pgeo_free(&q);
     geo_free(&G);
     return 0;

@@ -112,6 +112,41 @@ def test_pressure_reference_matches_oracle(setup, oracle, sweeps, omega):
     assert abs(st.rel_resid - rn / sn) <= 1e-10 * (rn / sn)
 
 
+@pytest.mark.parametrize("iters", [1, 3])
+def test_pressure_converge_fixed_iterations_match_oracle(setup, oracle, iters):
+    print(f"[case] converging solve, {iters} iterations", flush=True)
+    g, p, F0 = setup
+    F = F0.copy()
+    oracle.hydro(g, p, F)
+    oracle.rk4(g, p, F)
+    with A.UranusCore(g, p) as core:
+        core.set_state(F)
+        st = core.PressureSolver("converge", rel_tol=0.0, max_sweeps=iters)
+        pd = core.get_field("p_dyn")
+    it, rel, _ = oracle.pressure_converge(g, p, F, iters, 0.0)
+    e = rel_err(pd, F[A.FIELD["p_dyn"]])
+    print(f"[parity] converge {iters} it: p_dyn {e:.3e}; rel resid gpu {st.rel_resid:.6e} oracle {rel:.6e}",
+          flush=True)
+    assert st.sweeps_done == it == iters
+    assert e <= TOL and st.rel_resid == rel
+
+
+def test_pressure_converge_to_1e8(setup, oracle):
+    print("[case] converging solve to 1e-8", flush=True)
+    g, p, F0 = setup
+    F = F0.copy()
+    oracle.hydro(g, p, F)
+    oracle.rk4(g, p, F)
+    with A.UranusCore(g, p) as core:
+        core.set_state(F)
+        st = core.PressureSolver("converge", rel_tol=1e-8, max_sweeps=50)
+        pd = core.get_field("p_dyn")
+    it, rel, _ = oracle.pressure_converge(g, p, F, 50, 1e-8)
+    e = rel_err(pd, F[A.FIELD["p_dyn"]])
+    print(f"[parity] converge to 1e-8: {st.sweeps_done} iterations, rel {st.rel_resid:.3e}, p_dyn {e:.3e}", flush=True)
+    assert st.rel_resid <= 1e-8 and st.sweeps_done == it and e <= TOL
+
+
 def test_full_step_matches_oracle(setup, oracle):
     print("[case] two full steps (hydro, rk4, bc, 20 sweeps omega 1.5)", flush=True)
     g, p, F0 = setup
