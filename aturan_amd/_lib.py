@@ -73,6 +73,11 @@ class SolveStats(C.Structure):
                 ("src_norm", C.c_double)]
 
 
+class WindowOpts(C.Structure):
+    _fields_ = [("nsteps", C.c_int32), ("checkpoint_every", C.c_int32), ("solve_iters", C.c_int32),
+                ("pad_", C.c_int32), ("rel_tol", C.c_double), ("target_amp", C.c_double)]
+
+
 _DP = C.POINTER(C.c_double)
 _SIGS = {
     "atu_grid_default": (C.c_int, [C.c_int, C.POINTER(Grid)]),
@@ -95,6 +100,8 @@ _SIGS = {
     "atu_pressure_residual": (C.c_int, [C.c_void_p, _DP, _DP]),
     "atu_field_device_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
+    "atu_window_forward": (C.c_int, [C.c_void_p, C.POINTER(WindowOpts), _DP]),
+    "atu_window_backward": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
     "atu_build_info": (C.c_char_p, []),
     "atu_profile_begin": (C.c_int, [C.c_void_p]),
     "atu_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.POINTER(C.c_int)]),

@@ -166,6 +166,26 @@ class UranusCore:
     def synchronize(self) -> None:
         _check(self._lib.atu_synchronize(self._ctx), self._ctx, "atu_synchronize")
 
+    # ---- gradients over a window of steps ----
+    def window_forward(self, nsteps: int, checkpoint_every: int = 10, solve_iters: int = 0,
+                       rel_tol: float = 1e-10, target_amp: float = 1.2) -> float:
+        """Run `nsteps` steps from the current state, checkpointing every `checkpoint_every` steps, and
+        return the zonal-wind misfit L of the final state (atu_window_forward)."""
+        o = L.WindowOpts()
+        o.nsteps, o.checkpoint_every, o.solve_iters = nsteps, checkpoint_every, solve_iters
+        o.rel_tol, o.target_amp = rel_tol, target_amp
+        loss = C.c_double()
+        _check(self._lib.atu_window_forward(self._ctx, C.byref(o), C.byref(loss)), self._ctx, "atu_window_forward")
+        return loss.value
+
+    def window_backward(self, dloss: float = 1.0) -> tuple[np.ndarray, float]:
+        """dloss * dL/dT0 (dense [im][jm][km]) and dloss * dL/dnue of the last window_forward."""
+        g = np.empty((self.grid.im, self.grid.jm, self.grid.km), dtype=np.float64)
+        gn = C.c_double()
+        _check(self._lib.atu_window_backward(self._ctx, float(dloss), g.ctypes.data_as(L._DP), C.byref(gn)),
+               self._ctx, "atu_window_backward")
+        return g, gn.value
+
     # ---- per-kernel timing ----
     def profile_begin(self) -> None:
         _check(self._lib.atu_profile_begin(self._ctx), self._ctx, "atu_profile_begin")

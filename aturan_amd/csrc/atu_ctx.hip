@@ -4,8 +4,10 @@
 
 #include "atu_kernels.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -24,8 +26,8 @@ struct ProfEvent {
     hipEvent_t a, b;
 };
 
-const char* kClassNames[] = {"rk_stage", "rb_pass", "hydro", "bc", "proj_src", "residual", "line", "baro"};
-enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_N };
+const char* kClassNames[] = {"rk_stage", "rb_pass", "hydro", "bc", "proj_src", "residual", "line", "baro", "adjoint"};
+enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_ADJ, PC_N };
 
 }  // namespace
 
@@ -71,12 +73,28 @@ struct atu_ctx {
     float* SA32 = nullptr;    // 19
     float* SB32 = nullptr;    // 19
     float* FROZEN32 = nullptr;// 21
-    float* PDYN32 = nullptr;
-    float* PHYDRO32 = nullptr;
     float* AUX32 = nullptr;   // 3
     float* E32 = nullptr;     // refinement correction
     float* R32 = nullptr;     // refinement residual
     bool mixed() const { return precision == ATU_PREC_MIXED; }
+    int rk_order = 0;         // tile order of the stage kernel (ATU_RK_ORDER: 0 plain 3-D grid, 1 XCD-aware)
+
+    // gradients over a window of steps (allocated by the first atu_window_forward)
+    struct Window {
+        atu_window_opts opts{};
+        bool ready = false;          // a forward run whose checkpoints are held
+        int K = 0, nck = 0;          // checkpoint interval, checkpoints held
+        size_t tape_steps = 0, ck_slots = 0;
+        double* ckpt = nullptr;      // ck_slots x (19 prognostic + p_dyn) fields; the last slot is y_N
+        double* tape = nullptr;      // K x kTapeFields fields
+        double* tape_level = nullptr;// K x im level means
+        double *LY = nullptr, *LYN = nullptr, *LS = nullptr, *LK = nullptr, *LK2 = nullptr;   // 4 fields each
+        double *LAUX = nullptr;                                                            // 3 fields
+        double *LP = nullptr, *LPN = nullptr, *LPI = nullptr, *LPH = nullptr, *LNUE = nullptr, *Z = nullptr,
+               *B = nullptr, *LSRC = nullptr, *GREF = nullptr;                               // 1 field each
+        double *TGT = nullptr, *ROWS = nullptr, *PART = nullptr, *SC = nullptr;
+        double loss = 0.0;
+    } win;
 
     // profiling
     bool prof = false;
@@ -129,7 +147,11 @@ atu_ctx::~atu_ctx() {
                       BR, BAC, BAS})
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
-    for (float* p : {Y32, SA32, SB32, FROZEN32, PDYN32, PHYDRO32, AUX32, E32, R32})
+    for (double* p : {win.ckpt, win.tape, win.tape_level, win.LY, win.LYN, win.LS, win.LK, win.LK2, win.LAUX, win.LP,
+                      win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.B, win.LSRC, win.GREF, win.TGT, win.ROWS,
+                      win.PART, win.SC})
+        if (p) hipFree(p);
+    for (float* p : {Y32, SA32, SB32, FROZEN32, AUX32, E32, R32})
         if (p) hipFree(p);
     if (ev_bc) hipEventDestroy(ev_bc);
     if (ev_hydro) hipEventDestroy(ev_hydro);
@@ -286,6 +308,7 @@ StageArgs<double> stage_args(atu_ctx* c, int stage) {
     a.phydro = c->PHYDRO;
     a.aux = c->AUX;
     a.ymirror = nullptr;
+    a.xcd_order = c->rk_order;
     switch (stage) {
         case 0: a.xs = c->Y; a.xnext = c->SA; break;
         case 1: a.xs = c->SA; a.xnext = c->SB; break;
@@ -305,10 +328,11 @@ StageArgs<float> stage_args32(atu_ctx* c, int stage) {
     a.yn = c->Y;
     a.acc = c->ACC;
     a.frozen = c->FROZEN32;
-    a.pdyn = c->PDYN32;
-    a.phydro = c->PHYDRO32;
+    a.pdyn = c->PDYN;
+    a.phydro = c->PHYDRO;
     a.aux = c->AUX32;
     a.ymirror = nullptr;
+    a.xcd_order = c->rk_order;
     switch (stage) {
         case 0: a.xs = c->Y32; a.xnext = c->SA32; break;
         case 1: a.xs = c->SA32; a.xnext = c->SB32; break;
@@ -338,7 +362,7 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
                               [&] {
                                   return launch_hydro(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T),
                                                       c->prog(c->Y, ATU_V), c->prog(c->Y, ATU_W), c->PSTAT, c->LEVEL,
-                                                      c->PHYDRO, c->PHYDRO32, s);
+                                                      c->PHYDRO, s);
                               }),
                   "hydro");
 }
@@ -367,8 +391,8 @@ int do_residual(atu_ctx* c, double* rel, double* srcn) {
     return ATU_OK;
 }
 
-// Converging solve on the prepared source SRC: the consistent source b = src - (y.src / y.1) in
-// RES, then iterations until ||b - A p|| <= rel_tol ||b|| (or the iteration limit), then the corners,
+// Converging solve for P (p_dyn, or the adjoint's unknown) on the prepared source SRCP: the
+// consistent source b = src - (y.src / y.1) in RES, then iterations until ||b - A p|| <= rel_tol ||b|| (or the iteration limit), then the corners,
 // the seam and the held interior mean.
 //
 // FP64 context: each iteration is two line passes, the walls, the barotropic correction, the walls
@@ -376,17 +400,17 @@ int do_residual(atu_ctx* c, double* rel, double* srcn) {
 // Mixed context (iterative refinement): each outer iteration forms the residual r = b - A p in
 // FP64 (stored in FP32), solves A e = r approximately with `inner` FP32 line passes plus barotropic
 // corrections on an FP32 correction e, and accumulates p += e in FP64.
-int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
+int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, double* P, const double* SRCP) {
     const Shape& sh = c->sh;
     hipStream_t s = c->stream;
     auto H = [&](hipError_t e, const char* w) { return c->hip(e, w); };
     int e;
-    if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_YSRC, sh, c->geo, c->PDYN, c->SRC, c->SROWS,
+    if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_YSRC, sh, c->geo, P, SRCP, c->SROWS,
                                                                   c->SCAL + 0, s); }), "dsum")) ||
-        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_P, sh, c->geo, c->PDYN, nullptr, c->SROWS,
+        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_P, sh, c->geo, P, nullptr, c->SROWS,
                                                                   c->SCAL + 1, s); }), "dsum")) ||
-        (e = H(launch_make_b(sh, c->SRC, c->SCAL, c->dsol.yw, c->RES, s), "make_b")) ||
-        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_SQ, sh, c->geo, c->PDYN, c->RES, c->SROWS,
+        (e = H(launch_make_b(sh, SRCP, c->SCAL, c->dsol.yw, c->RES, s), "make_b")) ||
+        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_SQ, sh, c->geo, P, c->RES, c->SROWS,
                                                                   c->SCAL + 2, s); }), "dsum")))
         return e;
     const int max_it = o->max_sweeps > 0 ? o->max_sweeps : 1;
@@ -406,15 +430,15 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
         while (it < max_it) {
             for (int colour = 0; colour < 2; colour++)
                 if ((e = H(c->timed(PC_LINE, [&] { return launch_line<double, double, double>(sh, c->geo, c->dsol,
-                                                                                               c->PDYN, c->RES, colour,
+                                                                                               P, c->RES, colour,
                                                                                                s); }), "line")))
                     return e;
-            if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, c->PDYN, s); }), "walls")) ||
-                (e = H(c->timed(PC_BARO, [&] { return launch_baro<double, double>(sh, c->geo, c->dsol, c->PDYN, c->RES,
+            if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")) ||
+                (e = H(c->timed(PC_BARO, [&] { return launch_baro<double, double>(sh, c->geo, c->dsol, P, c->RES,
                                                                                    c->BR, c->BAC, c->BAS, s); }),
                        "baro")) ||
-                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, c->PDYN, s); }), "walls")) ||
-                (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, c->PDYN, c->RES,
+                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")) ||
+                (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES,
                                                                           c->SROWS, c->SCAL + 3, s); }), "dsum")))
                 return e;
             it++;
@@ -426,7 +450,7 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
         const int inner = o->sweeps > 0 ? o->sweeps : 4;
         while (true) {
             // FP64 residual of the current p, kept in FP32 as the right-hand side of the correction
-            if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, c->PDYN, c->RES,
+            if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES,
                                                                           c->SROWS, c->SCAL + 3, s, c->R32); }),
                        "dsum")) ||
                 (e = read_rel()))
@@ -447,16 +471,16 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
                     return e;
                 it++;
             }
-            if ((e = H(c->timed(PC_LINE, [&] { return launch_add_correction(sh, c->E32, c->PDYN, s); }), "add")) ||
-                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, c->PDYN, s); }), "walls")))
+            if ((e = H(c->timed(PC_LINE, [&] { return launch_add_correction(sh, c->E32, P, s); }), "add")) ||
+                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")))
                 return e;
             outer++;
         }
     }
-    if ((e = H(launch_p_corners<double>(sh, c->consts, c->PDYN, s), "corners")) ||
-        (e = H(launch_seam<double>(sh, c->PDYN, s), "seam")) ||
-        (e = H(launch_dsum<double>(DSUM_P, sh, c->geo, c->PDYN, nullptr, c->SROWS, c->SCAL + 4, s), "dsum")) ||
-        (e = H(launch_shift_all<double>(sh, c->SCAL, 1, 4, c->dsol.ncell, c->PDYN, s), "shift")))
+    if ((e = H(launch_p_corners<double>(sh, c->consts, P, s), "corners")) ||
+        (e = H(launch_seam<double>(sh, P, s), "seam")) ||
+        (e = H(launch_dsum<double>(DSUM_P, sh, c->geo, P, nullptr, c->SROWS, c->SCAL + 4, s), "dsum")) ||
+        (e = H(launch_shift_all<double>(sh, c->SCAL, 1, 4, c->dsol.ncell, P, s), "shift")))
         return e;
     if (st) {
         st->sweeps_done = it;
@@ -470,7 +494,7 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
 }
 
 int do_pressure_fp64(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
-    if (o->mode == ATU_SOLVE_CONVERGE) return do_converge(c, o, st);
+    if (o->mode == ATU_SOLVE_CONVERGE) return do_converge(c, o, st, c->PDYN, c->SRC);
     if (o->mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EINVAL, "unknown solve mode");
     const int sweeps = o->sweeps > 0 ? o->sweeps : 1;   // PressureSolver.h:174-175
     for (int s = 0; s < sweeps; s++) {
@@ -503,10 +527,209 @@ int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
         ? c->timed(PC_PROJ, [&] { return launch_proj_src<float>(c->sh, c->geo, c->consts, c->AUX32, c->SRC, c->stream); })
         : c->timed(PC_PROJ, [&] { return launch_proj_src<double>(c->sh, c->geo, c->consts, c->AUX, c->SRC, c->stream); });
     if (c->hip(pe, "proj_src")) return ATU_EHIP;
-    int e = do_pressure_fp64(c, o, st);
-    if (e) return e;
-    // the FP32 stencil of the next RK step reads p_dyn's mirror
-    if (c->mixed() && (e = mirror(c, c->PDYN, c->PDYN32))) return e;
+    return do_pressure_fp64(c, o, st);
+}
+
+// ---- gradients over a window of steps ----
+// Tape of one step: the four stage inputs X0..X3 (T, u, v, w each), the p_dyn the step read, the
+// clamped Poisson source it formed and the temperature of y_{n+1} (for the clamp mask).
+constexpr int kTapeFields = 19;
+enum { TP_X0 = 0, TP_X1 = 4, TP_X2 = 8, TP_X3 = 12, TP_P = 16, TP_SRC = 17, TP_TNEXT = 18 };
+constexpr int kCkFields = ATU_NPROG + 1;   // a checkpoint: the prognostic fields and p_dyn
+constexpr int kSumParts = 1024;
+
+double* tape_field(atu_ctx* c, int slot, int f) {
+    return c->win.tape + ((size_t)slot * kTapeFields + f) * c->sh.fs;
+}
+
+int dcopy(atu_ctx* c, double* dst, const double* src, size_t nfields) {
+    return c->hip(hipMemcpyAsync(dst, src, nfields * c->sh.fs * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                  "hipMemcpyAsync(D2D)");
+}
+
+atu_solver_opts window_solver(const atu_window_opts& w) {
+    atu_solver_opts o{ATU_SOLVE_CONVERGE, 0, 0, 0, 1.0, 0.0};
+    if (w.solve_iters > 0) {
+        o.max_sweeps = w.solve_iters;
+        o.rel_tol = 0.0;
+    } else {
+        o.max_sweeps = 200;
+        o.rel_tol = w.rel_tol;
+    }
+    return o;
+}
+
+int window_alloc(atu_ctx* c, int K, int slots) {
+    auto& W = c->win;
+    const size_t fs = c->sh.fs;
+    if (W.tape && W.tape_steps < (size_t)K) {
+        hipFree(W.tape);
+        hipFree(W.tape_level);
+        W.tape = W.tape_level = nullptr;
+    }
+    if (W.ckpt && W.ck_slots < (size_t)slots) {
+        hipFree(W.ckpt);
+        W.ckpt = nullptr;
+    }
+    auto grab = [&](double** p, size_t n) -> int {
+        if (*p) return ATU_OK;
+        if (c->hip(hipMalloc((void**)p, n * sizeof(double)), "hipMalloc(window)")) return ATU_ENOMEM;
+        return ATU_OK;
+    };
+    int e;
+    if ((e = grab(&W.tape, (size_t)K * kTapeFields * fs)) || (e = grab(&W.tape_level, (size_t)K * c->sh.im)) ||
+        (e = grab(&W.ckpt, (size_t)slots * kCkFields * fs)))
+        return e;
+    W.tape_steps = std::max(W.tape_steps, (size_t)K);
+    W.ck_slots = std::max(W.ck_slots, (size_t)slots);
+    for (double** p : {&W.LY, &W.LYN, &W.LS, &W.LK, &W.LK2})
+        if ((e = grab(p, 4 * fs))) return e;
+    if ((e = grab(&W.LAUX, 3 * fs))) return e;
+    for (double** p : {&W.LP, &W.LPN, &W.LPI, &W.LPH, &W.LNUE, &W.Z, &W.B, &W.LSRC, &W.GREF})
+        if ((e = grab(p, fs))) return e;
+    if ((e = grab(&W.TGT, c->sh.jm)) || (e = grab(&W.ROWS, (size_t)(c->sh.im - 2) * (c->sh.jm - 2))) ||
+        (e = grab(&W.PART, kSumParts)) || (e = grab(&W.SC, 8)))
+        return e;
+    return ATU_OK;
+}
+
+int ck_save(atu_ctx* c, int slot) {
+    double* d = c->win.ckpt + (size_t)slot * kCkFields * c->sh.fs;
+    int e = dcopy(c, d, c->Y, ATU_NPROG);
+    if (!e) e = dcopy(c, d + (size_t)ATU_NPROG * c->sh.fs, c->PDYN, 1);
+    return e;
+}
+
+// y_n into the state and both stage buffers (their boundary planes are the state's), p_dyn
+int ck_restore(atu_ctx* c, int slot) {
+    const double* d = c->win.ckpt + (size_t)slot * kCkFields * c->sh.fs;
+    int e = dcopy(c, c->Y, d, ATU_NPROG);
+    if (!e) e = dcopy(c, c->SA, d, ATU_NPROG);
+    if (!e) e = dcopy(c, c->SB, d, ATU_NPROG);
+    if (!e) e = dcopy(c, c->PDYN, d + (size_t)ATU_NPROG * c->sh.fs, 1);
+    return e;
+}
+
+// One step of the window in atu_step's order (the prep on the main stream). With slot >= 0 the
+// step's inputs and stage states go to that tape slot.
+int window_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
+    int e;
+    if (slot >= 0 && ((e = dcopy(c, tape_field(c, slot, TP_X0), c->Y, 4)) ||
+                      (e = dcopy(c, tape_field(c, slot, TP_P), c->PDYN, 1))))
+        return e;
+    if ((e = do_hydro(c, c->stream))) return e;
+    if (slot >= 0 && (e = c->hip(hipMemcpyAsync(c->win.tape_level + (size_t)slot * c->sh.im, c->LEVEL,
+                                                c->sh.im * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                                 "hipMemcpyAsync(level)")))
+        return e;
+    for (int s = 0; s < 4; s++) {
+        const StageArgs<double> a = stage_args(c, s);
+        if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")))
+            return e;
+        if (slot >= 0 && s < 3 && (e = dcopy(c, tape_field(c, slot, TP_X1 + 4 * s), a.xnext, 4))) return e;
+    }
+    if ((e = do_bc(c))) return e;
+    if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))) return e;
+    if ((e = do_pressure(c, &o, nullptr))) return e;
+    if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_SRC), c->SRC, 1))) return e;
+    return ATU_OK;
+}
+
+// The loss of the current state; LY receives scale * dL/dy (w only, interior rows).
+int window_loss(atu_ctx* c, double scale, double* loss) {
+    auto& W = c->win;
+    const Shape& sh = c->sh;
+    int e;
+    if ((e = c->hip(launch_zero(W.LY, 4 * sh.fs, c->stream), "zero")) ||
+        (e = c->hip(launch_loss(sh, c->prog(c->Y, ATU_W), W.TGT, scale, W.ROWS, W.LY + 3 * sh.fs, W.SC + 2, c->stream),
+                    "loss")))
+        return e;
+    double h = 0.0;
+    if ((e = c->hip(hipMemcpyAsync(&h, W.SC + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream), "hipMemcpy")) ||
+        (e = c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize")))
+        return e;
+    if (loss) *loss = h / ((double)(sh.im - 2) * (sh.jm - 2));
+    return ATU_OK;
+}
+
+// The adjoint of one step (taped in `slot`): LY, LP hold the lambdas of y_{n+1} and p_{n+1} on
+// entry and those of y_n and p_n on return.
+int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
+    auto& W = c->win;
+    const Shape& sh = c->sh;
+    const size_t fs = sh.fs;
+    hipStream_t s = c->stream;
+    auto H = [&](hipError_t e, const char* w) { return c->hip(e, w); };
+    int e;
+    // Poisson solve: walls folded, the held mean split off, A^T mu = lambda by the forward solver on
+    // Y^-1 lambda, the consistency shift and the source clamp, then div^T into lambda_aux
+    if ((e = H(c->timed(PC_ADJ, [&] {
+             hipError_t r = launch_pwall_adjoint(sh, W.LP, W.LPI, s);
+             if (r == hipSuccess) r = launch_dsum<double>(DSUM_P, sh, c->geo, W.LPI, nullptr, c->SROWS, W.SC + 0, s);
+             if (r == hipSuccess) r = launch_adj_rhs(sh, W.LPI, W.SC + 0, c->dsol.ncell, W.B, s);
+             if (r == hipSuccess) r = launch_zero(W.Z, fs, s);
+             return r;
+         }), "adjoint solve prep")))
+        return e;
+    if ((e = do_converge(c, &o, nullptr, W.Z, W.B))) return e;
+    if ((e = H(c->timed(PC_ADJ, [&] {
+             hipError_t r = launch_dsum<double>(DSUM_YSRC, sh, c->geo, W.Z, W.Z, c->SROWS, W.SC + 1, s);
+             if (r == hipSuccess)
+                 r = launch_adj_src(sh, c->geo, c->consts, W.Z, W.SC + 1, c->dsol.yw, tape_field(c, slot, TP_SRC),
+                                    W.LSRC, s);
+             if (r == hipSuccess) r = launch_adj_div(sh, c->geo, W.LSRC, W.LAUX, s);
+             // domain BC, then the final RK fold
+             if (r == hipSuccess) r = launch_bc_adjoint(sh, c->params, W.LY, s);
+             if (r == hipSuccess)
+                 r = launch_final_fold(sh, c->consts, tape_field(c, slot, TP_TNEXT), W.LY, W.LS, W.LYN, s);
+             if (r == hipSuccess) r = launch_zero(W.LPN, fs, s);
+             if (r == hipSuccess) r = launch_zero(W.LPH, fs, s);
+             return r;
+         }), "adjoint fold")))
+        return e;
+    // the stages in reverse: lambda_R3 = ls, lambda_R2 = 2 ls + dt lambda_X3, lambda_R1 = 2 ls +
+    // dt/2 lambda_X2, lambda_R0 = ls + dt/2 lambda_X1
+    AdjStageArgs a{};
+    a.sh = sh;
+    a.geo = c->geo;
+    a.c = c->consts;
+    a.pdyn = tape_field(c, slot, TP_P);
+    a.nue = c->FROZEN;   // the frozen block starts with nue
+    a.ls = W.LS;
+    a.ly = W.LYN;
+    a.lp = W.LPN;
+    a.lph = W.LPH;
+    a.lnue = W.LNUE;
+    struct StagePlan {
+        int x;
+        const double* lk;
+        const double* laux;
+        double* lk_prev;
+        double ws, cs;
+    };
+    const StagePlan plan[4] = {{TP_X3, W.LS, W.LAUX, W.LK, 2.0, c->consts.c_full},
+                               {TP_X2, W.LK, nullptr, W.LK2, 2.0, c->consts.c_half},
+                               {TP_X1, W.LK2, nullptr, W.LK, 1.0, c->consts.c_half},
+                               {TP_X0, W.LK, nullptr, nullptr, 0.0, 0.0}};
+    for (const StagePlan& q : plan) {
+        a.xs = tape_field(c, slot, q.x);
+        a.lk = q.lk;
+        a.laux = q.laux;
+        a.lk_prev = q.lk_prev;
+        a.ws = q.ws;
+        a.cs = q.cs;
+        if ((e = H(c->timed(PC_ADJ, [&] { return launch_rhs_adjoint(a, s); }), "rhs adjoint"))) return e;
+    }
+    // hydrostatic prep, then the held mean's share of p_n
+    if ((e = H(c->timed(PC_ADJ, [&] {
+             hipError_t r = launch_hydro_adjoint(sh, c->geo, c->consts, tape_field(c, slot, TP_X0), c->PSTAT,
+                                                 W.tape_level + (size_t)slot * sh.im, W.LPH, W.GREF, W.LYN, s);
+             if (r == hipSuccess) r = launch_add_mean(sh, W.SC + 0, c->dsol.ncell, W.LPN, s);
+             return r;
+         }), "adjoint hydro")))
+        return e;
+    std::swap(W.LY, W.LYN);
+    std::swap(W.LP, W.LPN);
     return ATU_OK;
 }
 
@@ -531,6 +754,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     c->grid = *g;
     c->params = *p;
     c->precision = precision;
+    if (const char* ord = std::getenv("ATU_RK_ORDER")) c->rk_order = std::atoi(ord) != 0 ? 1 : 0;
     c->sh.im = g->im;
     c->sh.jm = g->jm;
     c->sh.km = g->km;
@@ -570,8 +794,6 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         if (!e) e = alloc_fields(c.get(), &c->SA32, ATU_NPROG);
         if (!e) e = alloc_fields(c.get(), &c->SB32, ATU_NPROG);
         if (!e) e = alloc_fields(c.get(), &c->FROZEN32, ATU_NFROZEN);
-        if (!e) e = alloc_fields(c.get(), &c->PDYN32, 1);
-        if (!e) e = alloc_fields(c.get(), &c->PHYDRO32, 1);
         if (!e) e = alloc_fields(c.get(), &c->AUX32, 3);
         if (!e) e = alloc_fields(c.get(), &c->E32, 1);
         if (!e) e = alloc_fields(c.get(), &c->R32, 1);
@@ -649,10 +871,6 @@ int atu_set_field(atu_ctx* ctx, int f, const double* host) {
             if (!e) e = mirror(ctx, d, ctx->SB32 + (size_t)f * fs);
         } else if (f >= ATU_NUE && f < ATU_NUE + ATU_NFROZEN) {
             e = mirror(ctx, d, ctx->FROZEN32 + (size_t)(f - ATU_NUE) * fs);
-        } else if (f == ATU_P_DYN) {
-            e = mirror(ctx, d, ctx->PDYN32);
-        } else if (f == ATU_P_HYDRO) {
-            e = mirror(ctx, d, ctx->PHYDRO32);
         } else if (aux) {
             e = mirror(ctx, d, ctx->AUX32 + (size_t)(f - ATU_AUX_U) * fs);   // staged through RES
         }
@@ -728,6 +946,70 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
         if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;
     }
     return ATU_OK;
+}
+
+int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) {
+    if (!ctx || !opts || opts->nsteps < 1) return ATU_EINVAL;
+    if (ctx->mixed()) return ctx->fail(ATU_EUNSUPPORTED, "atu_window_forward: needs an FP64 context");
+    if (!ctx->params.rigid_lid) return ctx->fail(ATU_EUNSUPPORTED, "atu_window_forward: needs rigid_lid = 1");
+    if (opts->solve_iters <= 0 && !(opts->rel_tol > 0.0))
+        return ctx->fail(ATU_EINVAL, "atu_window_forward: give solve_iters > 0 or rel_tol > 0");
+    auto& W = ctx->win;
+    W.ready = false;
+    W.opts = *opts;
+    if (W.opts.checkpoint_every <= 0) W.opts.checkpoint_every = 10;
+    if (!(W.opts.target_amp > 0.0)) W.opts.target_amp = 1.2;
+    const int N = W.opts.nsteps;
+    W.K = std::min(W.opts.checkpoint_every, N);
+    W.nck = (N + W.K - 1) / W.K;
+    int e;
+    if ((e = sync_all(ctx)) || (e = window_alloc(ctx, W.K, W.nck + 1))) return e;
+    std::vector<double> tgt(ctx->sh.jm);
+    for (int j = 0; j < ctx->sh.jm; j++) tgt[j] = W.opts.target_amp * std::cos(3.0 * (0.5 * M_PI - ctx->hgeo.thez[j]));
+    if ((e = ctx->hip(hipMemcpyAsync(W.TGT, tgt.data(), tgt.size() * sizeof(double), hipMemcpyHostToDevice,
+                                     ctx->stream), "hipMemcpy(target)")))
+        return e;
+    const atu_solver_opts o = window_solver(W.opts);
+    for (int n = 0; n < N; n++) {
+        if (n % W.K == 0 && (e = ck_save(ctx, n / W.K))) return e;
+        if ((e = window_step(ctx, o, -1))) return e;
+    }
+    if ((e = ck_save(ctx, W.nck)) || (e = window_loss(ctx, 1.0, &W.loss))) return e;
+    if (loss) *loss = W.loss;
+    W.ready = true;
+    return ATU_OK;
+}
+
+int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* grad_nue) {
+    if (!ctx) return ATU_EINVAL;
+    auto& W = ctx->win;
+    if (!W.ready) return ctx->fail(ATU_EINVAL, "atu_window_backward: no atu_window_forward to differentiate");
+    const Shape& sh = ctx->sh;
+    const atu_solver_opts o = window_solver(W.opts);
+    const int N = W.opts.nsteps;
+    int e;
+    // seed: dloss * dL/dy_N from the final state
+    if ((e = ck_restore(ctx, W.nck)) || (e = window_loss(ctx, dloss, nullptr)) ||
+        (e = ctx->hip(launch_zero(W.LP, sh.fs, ctx->stream), "zero")) ||
+        (e = ctx->hip(launch_zero(W.LNUE, sh.fs, ctx->stream), "zero")))
+        return e;
+    for (int seg = W.nck - 1; seg >= 0; seg--) {
+        const int n0 = seg * W.K, n1 = std::min(n0 + W.K, N);
+        if ((e = ck_restore(ctx, seg))) return e;
+        for (int n = n0; n < n1; n++)
+            if ((e = window_step(ctx, o, n - n0))) return e;
+        for (int n = n1 - 1; n >= n0; n--)
+            if ((e = window_reverse_step(ctx, o, n - n0))) return e;
+    }
+    if (grad_nue) {
+        if ((e = ctx->hip(launch_sum(W.LNUE, sh.fs, W.PART, kSumParts, W.SC + 3, ctx->stream), "sum")) ||
+            (e = ctx->hip(hipMemcpyAsync(grad_nue, W.SC + 3, sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                          "hipMemcpy")))
+            return e;
+    }
+    if (grad_t0 && (e = get_field(ctx, W.LY + (size_t)ATU_T * sh.fs, grad_t0))) return e;
+    if ((e = ck_restore(ctx, W.nck))) return e;
+    return ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
 int atu_field_device_ptr(atu_ctx* ctx, int f, void** dptr, int* ld_k, int* elem_bytes) {

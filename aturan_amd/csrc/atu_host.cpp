@@ -137,8 +137,13 @@ void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo) {
     geo.pole_row.assign(jm, 0);
     std::vector<double> sin_solver(jm);
     for (int j = 0; j < jm; j++) {
-        const double s = std::max(p.sinthe_min, std::abs(std::sin(geo.thez[j])));
-        const double c = std::cos(geo.thez[j]);
+        // sin and cos of one angle: the reference's g++ build forms this pair (its loop at
+        // planet/RungeKutta_Uran_Turb.cpp:195-196) with one glibc sincos call, whose last bit can
+        // differ from separate sin and cos calls, so the pair is formed the same way here
+        double sj, cj;
+        sincos(geo.thez[j], &sj, &cj);
+        const double s = std::max(p.sinthe_min, std::abs(sj));
+        const double c = cj;
         geo.sinthe[j] = s;
         geo.sinthe2[j] = s * s;
         geo.costhe[j] = c;
@@ -219,8 +224,7 @@ void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& 
     for (int m = 0; m <= NM; m++)
         for (int k = 0; k < N; k++) {
             const double a = 2.0 * M_PI * (double)((long)m * k % N) / N;
-            t.ct[(size_t)m * N + k] = std::cos(a);
-            t.st[(size_t)m * N + k] = std::sin(a);
+            sincos(a, &t.st[(size_t)m * N + k], &t.ct[(size_t)m * N + k]);   // one call, as in the oracle
             t.ctT[(size_t)k * (NM + 1) + m] = t.ct[(size_t)m * N + k];
             t.stT[(size_t)k * (NM + 1) + m] = t.st[(size_t)m * N + k];
         }

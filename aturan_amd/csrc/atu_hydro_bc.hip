@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(64) k_level_serial(Shape sh, DevGeo g, StepCon
 // (METRIC_SPLIT + CORIOLIS_SPLIT branch, :104-161). One thread per (j, k) column.
 __global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepConsts c, const double* t,
                                                     const double* v, const double* w, const double* pstat,
-                                                    const double* level, double* phydro, float* phydro32) {
+                                                    const double* level, double* phydro) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (k >= sh.km) return;
@@ -79,13 +79,11 @@ __global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepCons
     double p = 0.0;
     double f_prev = f_at(0);
     phydro[(size_t)j * sh.ldk + k] = 0.0;
-    if (phydro32) phydro32[(size_t)j * sh.ldk + k] = 0.0f;
     for (int i = 1; i < sh.im; i++) {
         const double fi = f_at(i);
         p = p + 0.5 * (f_prev + fi) * c.dr;
         const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
         phydro[x] = p;
-        if (phydro32) phydro32[x] = (float)p;
         f_prev = fi;
     }
 }
@@ -244,11 +242,11 @@ __global__ void __launch_bounds__(256) k_species_clamp(BcCtx b) {
 }  // namespace
 
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
-                        const double* w, const double* pstat, double* level_out, double* phydro, float* phydro32,
+                        const double* w, const double* pstat, double* level_out, double* phydro,
                         hipStream_t s) {
     hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(64), 0, s, sh, g, c, t, pstat, level_out);
     hipLaunchKernelGGL(k_hydro_scan, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, sh, g, c, t, v, w, pstat,
-                       level_out, phydro, phydro32);
+                       level_out, phydro);
     return hipGetLastError();
 }
 

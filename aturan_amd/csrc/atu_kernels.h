@@ -37,9 +37,10 @@ struct StageArgs {
     S* ymirror;
     double* acc;
     const S* frozen;     // 21 fields
-    const S* pdyn;
-    const S* phydro;
+    const double* pdyn;  // FP64 in both precisions: the pressure gradients are differences of
+    const double* phydro;// nearby large values, formed in FP64 and rounded once
     S* aux;              // 3 fields, written at stage 3
+    int xcd_order;       // 1: XCD-aware, i-fastest tile order (1-D grid); 0: (k, j, i) 3-D grid
     Shape sh;
     DevGeo geo;
     StepConsts c;
@@ -53,7 +54,7 @@ hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream
 // then the column scan into `phydro`.
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t,
                         const double* v, const double* w, const double* pstat, double* level_out,
-                        double* phydro, float* phydro32, hipStream_t s);
+                        double* phydro, hipStream_t s);
 
 // Element-wise precision conversion of n values (FP32 mirrors of FP64 fields and back).
 hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s);
@@ -120,6 +121,43 @@ hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hip
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
                             hipStream_t s);
+
+// ---- the adjoint of the FP64 step (atu_adjoint.hip); 4-field blocks hold T, u, v, w ----
+struct AdjStageArgs {
+    Shape sh;
+    DevGeo geo;
+    StepConsts c;
+    const double* xs;      // stage input X_s (4 fields)
+    const double* pdyn;    // p_dyn read by the step
+    const double* nue;     // frozen eddy viscosity
+    const double* lk;      // lambda of the stage's tendencies (4 fields)
+    const double* laux;    // lambda of aux (3 fields) at stage 3, else null
+    const double* ls;      // lambda of the final accumulated sum (4 fields)
+    double* lk_prev;       // lambda of the previous stage's tendencies = ws * ls + cs * lambda_X, or null
+    double ws, cs;
+    double* ly;            // += lambda_X (4 fields)
+    double* lp;            // += lambda of p_dyn
+    double* lph;           // += lambda of p_hydro
+    double* lnue;          // += lambda of the eddy viscosity
+};
+hipError_t launch_rhs_adjoint(const AdjStageArgs& a, hipStream_t s);
+hipError_t launch_hydro_adjoint(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* x4,
+                                const double* pstat, const double* level, const double* lph, double* gref, double* ly,
+                                hipStream_t s);
+hipError_t launch_bc_adjoint(const Shape& sh, const atu_params& p, double* ly, hipStream_t s);
+hipError_t launch_final_fold(const Shape& sh, const StepConsts& c, const double* tnext, const double* ly, double* ls,
+                             double* lyn, hipStream_t s);
+hipError_t launch_pwall_adjoint(const Shape& sh, const double* lp, double* lpi, hipStream_t s);
+hipError_t launch_adj_rhs(const Shape& sh, const double* lpi, const double* sc, double ncell, double* b,
+                          hipStream_t s);
+hipError_t launch_adj_src(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* z, const double* sc,
+                          double yw, const double* src, double* lsrc, hipStream_t s);
+hipError_t launch_adj_div(const Shape& sh, const DevGeo& g, const double* lsrc, double* laux, hipStream_t s);
+hipError_t launch_add_mean(const Shape& sh, const double* sc, double ncell, double* lp, hipStream_t s);
+hipError_t launch_loss(const Shape& sh, const double* w, const double* target, double scale, double* rows, double* lw,
+                       double* out, hipStream_t s);
+hipError_t launch_sum(const double* v, size_t n, double* part, int nparts, double* out, hipStream_t s);
+hipError_t launch_zero(double* v, size_t n, hipStream_t s);
 
 // Residual r = src - Lc p over the interior: writes per-block partial sums of r^2 and src^2 into
 // `partials` (2 * nblocks doubles) and the fixed-order total into out[0] (r^2), out[1] (src^2).

@@ -45,6 +45,12 @@ __device__ __forceinline__ D7<R> stencil(const S* __restrict__ f, size_t c, size
     return d;
 }
 
+// (a - b) * s in FP64, rounded once to R. For R = double this is the reference's (a - b) * s.
+template <typename R>
+__device__ __forceinline__ R fp64_diff(double a, double b, double s) {
+    return (R)((a - b) * s);
+}
+
 // Fold one tendency into the accumulator and form the next input (pass B) or the step (final).
 template <int STAGE, int CLAMP, typename S>
 __device__ __forceinline__ void fold(const StageArgs<S>& a, int f, size_t c, double y0, double r) {
@@ -123,10 +129,12 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     // velocities and the pressure gradient
     const D7<R> U = ST(ATU_U), V = ST(ATU_V), W = ST(ATU_W);
     const R u0 = U.c, v0 = V.c, w0 = W.c;
-    const S* pd = a.pdyn;
-    const R dpdr = ((R)pd[c + plane] - (R)pd[c - plane]) * i2dr;
-    const R dpdthe = ((R)pd[c + ldk] - (R)pd[c - ldk]) * i2dt;
-    const R dpdphi = ((R)pd[ce] - (R)pd[cw]) * i2dp;
+    // p_dyn (|p| ~ 20) varies slowly: its differences are formed in FP64 and rounded once to R,
+    // since an FP32 difference of two such values keeps few significant bits
+    const double* pd = a.pdyn;
+    const R dpdr = fp64_diff<R>(pd[c + plane], pd[c - plane], G.inv_2dr);
+    const R dpdthe = fp64_diff<R>(pd[c + ldk], pd[c - ldk], G.inv_2dthe);
+    const R dpdphi = fp64_diff<R>(pd[ce], pd[cw], G.inv_2dphi);
     const R scale_p = (R)C.scale_p;
     const R v_invrm = v0 * inv_rm;
     const R w_invrs = w0 * inv_rmsinthe;
@@ -167,7 +175,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         if (STAGE == 3) a.aux[c] = (S)(rhs_u + dpdr_term);
     }
     const R cor_sc = (R)C.Coriolis * (R)C.scale_Cor;
-    const S* ph = a.phydro;
+    const double* ph = a.phydro;
     // ---- meridional velocity v (:305-306, :339-343, :719, :759-765) ----
     {
         const R transport_v = u0 * V.d1r + v_invrm * V.d1t + w_invrs * V.d1p
@@ -175,7 +183,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         const R diffusion_v = V.d2r + V.d1r * two_inv_rm + V.d2t * inv_rm2 + V.d1t * cirs - v_metric * v0
                               + V.d2p * inv_rm2sinthe2 + R(2.0) * U.d1t * inv_rm2
                               - W.d1p * R(2.0) * costhe * inv_rm2sinthe2;
-        const R dphdthe_term = ((R)ph[c + ldk] - (R)ph[c - ldk]) * i2dt * inv_rm;
+        const R dphdthe_term = fp64_diff<R>(ph[c + ldk], ph[c - ldk], G.inv_2dthe) * inv_rm;
         const R Coriolis_the = (R)C.m_two_omega * costhe * w0;
         const R rhs_v = -dpdthe_term - dphdthe_term - transport_v + diffusion_v / re + diffusion_v * nue_t
                         - cor_sc * Coriolis_the;
@@ -189,7 +197,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         const R diffusion_w = W.d2r + W.d1r * two_inv_rm + W.d2t * inv_rm2 + W.d1t * cirs - v_metric * w0
                               + W.d2p * inv_rm2sinthe2 + R(2.0) * U.d1p * inv_rm2sinthe
                               + V.d1p * R(2.0) * costhe * inv_rm2sinthe2;
-        const R dphdphi_term = ((R)ph[ce] - (R)ph[cw]) * i2dp * inv_rmsinthe;
+        const R dphdphi_term = fp64_diff<R>(ph[ce], ph[cw], G.inv_2dphi) * inv_rmsinthe;
         const R Coriolis_phi = (R)C.two_omega * (R(1.0) * costhe * v0 + sinthe_bf * u0);
         const R rhs_w = -dpdphi_term - dphdphi_term - transport_w + diffusion_w / re + diffusion_w * nue_t
                         - cor_sc * Coriolis_phi;
@@ -258,9 +266,27 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
 template <typename S, int STAGE>
 __global__ void __launch_bounds__(256) k_rk_stage(const StageArgs<S> a) {
     const int nk = a.sh.km - 1;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = 1 + blockIdx.z;
+    int kc, jt, il;
+    if (a.xcd_order) {
+        // 1-D grid. Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
+        // owns one contiguous range of tiles ordered i fastest, then j, then k: the i+-1 planes a
+        // tile's stencils read were just loaded into the same XCD's L2 by its neighbours in i.
+        const int ni = a.sh.im - 2, njt = (a.sh.jm - 2 + 3) / 4, nkc = (nk + 63) / 64;
+        const int nb = ni * njt * nkc;
+        const int per = (nb + 7) / 8;
+        const int L = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+        if (L >= nb) return;
+        il = L % ni;
+        jt = (L / ni) % njt;
+        kc = L / (ni * njt);
+    } else {
+        kc = blockIdx.x;
+        jt = blockIdx.y;
+        il = blockIdx.z;
+    }
+    const int k = kc * 64 + threadIdx.x;
+    const int j = 1 + jt * 4 + threadIdx.y;
+    const int i = 1 + il;
     if (k >= nk || j > a.sh.jm - 2) return;
     if constexpr (sizeof(S) == sizeof(double)) {
         rk_cell<double, S, STAGE>(a, i, j, k);
@@ -276,7 +302,8 @@ template <typename S>
 hipError_t launch_stage(int stage, const StageArgs<S>& a, hipStream_t s) {
     const int nk = a.sh.km - 1;
     const dim3 block(64, 4, 1);
-    const dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + 3) / 4, a.sh.im - 2);
+    dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + 3) / 4, a.sh.im - 2);
+    if (a.xcd_order) grid = dim3((grid.x * grid.y * grid.z + 7) / 8 * 8, 1, 1);
     switch (stage) {
         case 0: hipLaunchKernelGGL((k_rk_stage<S, 0>), grid, block, 0, s, a); break;
         case 1: hipLaunchKernelGGL((k_rk_stage<S, 1>), grid, block, 0, s, a); break;

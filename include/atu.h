@@ -230,6 +230,33 @@ int atu_field_device_ptr(atu_ctx* ctx, int field, void** dptr, int* ld_k, int* e
 /* Library build information string (arch, flags). */
 const char* atu_build_info(void);
 
+/* ---- gradients over a window of steps (no counterpart in the reference) ----
+ * The loss of a window of `nsteps` outer steps (atu_step's order, FP64 context, converging solve)
+ * is the misfit of the final zonal wind to a target jet,
+ *     L = 1/(ni nj) sum_{i,j interior} ( 1/N sum_{k<N} w(i,j,k) - target_amp cos(3 lat_j) )^2,
+ * lat_j = pi/2 - theta_j, N = km-1 meridians, ni = im-2, nj = jm-2 (w is non-dimensional: the
+ * default target_amp 1.2 is 120 m/s at u_0 = 100 m/s). atu_window_forward runs the window from
+ * the context's current state, keeping a checkpoint of the state (prognostic fields and p_dyn)
+ * every `checkpoint_every` steps, and returns L; atu_window_backward then runs the discrete
+ * adjoint of the same steps segment by segment (each segment recomputed from its checkpoint with
+ * its stage states taped) and returns dloss * dL/dT0 and dloss * dL/dnue, T0 being the initial
+ * temperature field and nue the eddy viscosity field (the derivative for one uniform change of
+ * every cell of nue). The solve is differentiated as exact; both the forward solves of the
+ * window and the adjoint solves run `solve_iters` iterations when it is > 0, else to rel_tol.
+ * The state after either call is the window's final state. Needs rigid_lid = 1. */
+typedef struct atu_window_opts {
+    int32_t nsteps;
+    int32_t checkpoint_every;   /* <= 0: 10                                                  */
+    int32_t solve_iters;        /* fixed iterations per solve (> 0), or 0: solve to rel_tol    */
+    int32_t pad_;
+    double rel_tol;             /* used when solve_iters == 0                                 */
+    double target_amp;          /* <= 0: 1.2                                                  */
+} atu_window_opts;
+
+int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss);
+/* grad_t0: dense [im][jm][km] host array or NULL; grad_nue: scalar or NULL. */
+int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* grad_nue);
+
 /* ---- per-kernel timing (no counterpart in the reference; used by the benchmark) ----
  * Between atu_profile_begin and atu_profile_query every kernel launch of the context is bracketed
  * by HIP events recorded on the context's stream. atu_profile_query synchronises and returns the

@@ -19,6 +19,7 @@
  *
  * Dense layout: fields[f * n3 + (i * jm + j) * km + k], f in the atu_field order, n3 = im*jm*km.
  */
+#define _GNU_SOURCE   /* sincos */
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -137,8 +138,12 @@ static void geo_init(orc_geo* G, const atu_grid* g, const atu_params* p) {
     G->thez[0] = g->the0;
     for (int l = 1; l < g->jm; l++) G->thez[l] = G->thez[l - 1] + g->dthe;
     for (int j = 0; j < g->jm; j++) {
-        G->sin_tbl[j] = dmax(p->sinthe_min, fabs(sin(G->thez[j])));
-        G->cos_tbl[j] = cos(G->thez[j]);
+        /* one sincos call for the pair, as the reference's g++ build forms it (its -O2 folds the
+         * sin and cos of one angle at planet/RungeKutta_Uran_Turb.cpp:195-196 into sincos) */
+        double sj, cj;
+        sincos(G->thez[j], &sj, &cj);
+        G->sin_tbl[j] = dmax(p->sinthe_min, fabs(sj));
+        G->cos_tbl[j] = cj;
     }
     G->inv_2dr = 1.0 / (2.0 * g->dr);
     G->inv_2dthe = 1.0 / (2.0 * g->dthe);
@@ -847,8 +852,7 @@ static void baro_init(orc_baro* B, const orc_geo* G, const orc_pgeo* q, const at
     for (int m = 0; m <= NM; m++)
         for (int k = 0; k < N; k++) {
             const double a = 2.0 * M_PI * (double)((long)m * k % N) / N;
-            B->ct[(size_t)m * N + k] = cos(a);
-            B->st[(size_t)m * N + k] = sin(a);
+            sincos(a, &B->st[(size_t)m * N + k], &B->ct[(size_t)m * N + k]);
         }
     double S2 = 0.0;
     for (int i = 1; i < im - 1; i++) {
