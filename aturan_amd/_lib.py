@@ -102,6 +102,7 @@ _SIGS = {
                                        C.POINTER(C.c_int)]),
     "atu_window_forward": (C.c_int, [C.c_void_p, C.POINTER(WindowOpts), _DP]),
     "atu_window_backward": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
+    "atu_window_rewind": (C.c_int, [C.c_void_p]),
     "atu_build_info": (C.c_char_p, []),
     "atu_profile_begin": (C.c_int, [C.c_void_p]),
     "atu_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.POINTER(C.c_int)]),

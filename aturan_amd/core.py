@@ -186,6 +186,10 @@ class UranusCore:
                self._ctx, "atu_window_backward")
         return g, gn.value
 
+    def window_rewind(self) -> None:
+        """State (prognostic fields, p_dyn) back to the start of the last window_forward."""
+        _check(self._lib.atu_window_rewind(self._ctx), self._ctx, "atu_window_rewind")
+
     # ---- per-kernel timing ----
     def profile_begin(self) -> None:
         _check(self._lib.atu_profile_begin(self._ctx), self._ctx, "atu_profile_begin")

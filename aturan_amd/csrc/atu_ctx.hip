@@ -77,7 +77,7 @@ struct atu_ctx {
     float* E32 = nullptr;     // refinement correction
     float* R32 = nullptr;     // refinement residual
     bool mixed() const { return precision == ATU_PREC_MIXED; }
-    int rk_order = 0;         // tile order of the stage kernel (ATU_RK_ORDER: 0 plain 3-D grid, 1 XCD-aware)
+    int rk_order = 0;         // stage kernel tile order (ATU_RK_ORDER): 0 3-D grid, 1 XCD ranges, 2 XCD j-bands
 
     // gradients over a window of steps (allocated by the first atu_window_forward)
     struct Window {
@@ -754,7 +754,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     c->grid = *g;
     c->params = *p;
     c->precision = precision;
-    if (const char* ord = std::getenv("ATU_RK_ORDER")) c->rk_order = std::atoi(ord) != 0 ? 1 : 0;
+    if (const char* ord = std::getenv("ATU_RK_ORDER")) c->rk_order = std::min(2, std::max(0, std::atoi(ord)));
     c->sh.im = g->im;
     c->sh.jm = g->jm;
     c->sh.km = g->km;
@@ -1010,6 +1010,14 @@ int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* gra
     if (grad_t0 && (e = get_field(ctx, W.LY + (size_t)ATU_T * sh.fs, grad_t0))) return e;
     if ((e = ck_restore(ctx, W.nck))) return e;
     return ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+int atu_window_rewind(atu_ctx* ctx) {
+    if (!ctx) return ATU_EINVAL;
+    if (!ctx->win.ready) return ctx->fail(ATU_EINVAL, "atu_window_rewind: no atu_window_forward to rewind to");
+    int e = ck_restore(ctx, 0);
+    if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return e;
 }
 
 int atu_field_device_ptr(atu_ctx* ctx, int f, void** dptr, int* ld_k, int* elem_bytes) {

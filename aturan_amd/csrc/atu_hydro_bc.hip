@@ -11,46 +11,65 @@ namespace {
 //
 // The reference adds the terms of a level one after another in (j, k) order, and the buoyancy
 // anomaly divides by this mean, which magnifies any rounding difference of the sum ~200-fold in
-// p_hydro. So the sum is formed in exactly that order: one wavefront per level computes 64
-// consecutive terms in parallel (a skipped term contributes an exact +0.0), parks them in LDS, and
-// lane 0 adds them in sequence. The chain is ~65k dependent adds per level; the step runs this
-// kernel on a side stream, concurrently with the previous step's Poisson solve, which it does not
-// depend on.
+// p_hydro. So the sum is formed in exactly that order: one wavefront per level. Its lanes load and
+// form 64 consecutive terms at a time (a skipped term contributes an exact +0.0); a group of
+// kLevelGroup such chunks is parked in LDS and added in sequence while the loads of the next group
+// are in flight. The chain is ~65k dependent adds per level; the step runs this kernel on a side
+// stream, concurrently with the previous step's Poisson solve, which it does not depend on.
+constexpr int kLevelGroup = 8;
 __global__ void __launch_bounds__(64) k_level_serial(Shape sh, DevGeo g, StepConsts c, const double* t,
                                                      const double* pstat, double* level) {
     const int i = blockIdx.x;
     const int n = sh.jm * sh.km;
-    __shared__ double tb[64], tw[64];
+    const int lane = threadIdx.x;
+    __shared__ double tb[kLevelGroup * 64], tw[kLevelGroup * 64];
+    double rt[kLevelGroup], rp[kLevelGroup];   // T and p_stat of the group in flight
+    int rj[kLevelGroup];
+    auto load_group = [&](int q0) {
+#pragma unroll
+        for (int u = 0; u < kLevelGroup; u++) {
+            const int q = q0 + u * 64 + lane;
+            rj[u] = -1;
+            rt[u] = 0.0;
+            rp[u] = 0.0;
+            if (q < n) {
+                const int j = q / sh.km, k = q - j * sh.km;
+                const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+                rj[u] = j;
+                rt[u] = t[x];
+                rp[u] = pstat[x];
+            }
+        }
+    };
     double s = 0.0, ws = 0.0;
-    for (int q0 = 0; q0 < n; q0 += 64) {
-        const int q = q0 + (int)threadIdx.x;
-        double term = 0.0, wt = 0.0;
-        if (q < n) {
-            const int j = q / sh.km, k = q - j * sh.km;
-            const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-            const double T = t[x];
-            if (T > 0.0) {
-                const double b = c.g * pstat[x] / (c.hyd_bnum * T * c.t_ref);
+    load_group(0);
+    for (int q0 = 0; q0 < n; q0 += kLevelGroup * 64) {
+#pragma unroll
+        for (int u = 0; u < kLevelGroup; u++) {
+            double term = 0.0, wt = 0.0;
+            const double T = rt[u];
+            if (rj[u] >= 0 && T > 0.0) {
+                const double b = c.g * rp[u] / (c.hyd_bnum * T * c.t_ref);
                 if (isfinite(b)) {
-                    const double wgt = g.sin_weight[j];
+                    const double wgt = g.sin_weight[rj[u]];
                     term = wgt * b;
                     wt = wgt;
                 }
             }
+            tb[u * 64 + lane] = term;
+            tw[u * 64 + lane] = wt;
         }
-        tb[threadIdx.x] = term;
-        tw[threadIdx.x] = wt;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const int m = min(64, n - q0);
-            for (int l = 0; l < m; l++) {
-                s += tb[l];
-                ws += tw[l];
-            }
+        load_group(q0 + kLevelGroup * 64);      // in flight while the chain below runs
+        const int m = min(kLevelGroup * 64, n - q0);
+#pragma unroll 16
+        for (int l = 0; l < m; l++) {
+            s += tb[l];
+            ws += tw[l];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) level[i] = (ws > 0.0) ? s / ws : 0.0;
+    if (lane == 0) level[i] = (ws > 0.0) ? s / ws : 0.0;
 }
 
 // ---- the column scan: p_hydro[0] = 0, p_hydro[i] = p_hydro[i-1] + 0.5 (f(i-1) + f(i)) dr with

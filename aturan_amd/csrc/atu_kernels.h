@@ -40,7 +40,7 @@ struct StageArgs {
     const double* pdyn;  // FP64 in both precisions: the pressure gradients are differences of
     const double* phydro;// nearby large values, formed in FP64 and rounded once
     S* aux;              // 3 fields, written at stage 3
-    int xcd_order;       // 1: XCD-aware, i-fastest tile order (1-D grid); 0: (k, j, i) 3-D grid
+    int xcd_order;       // 0: (k, j, i) 3-D grid; 1: per-XCD tile ranges, i fastest; 2: per-XCD j-bands, plane-major
     Shape sh;
     DevGeo geo;
     StepConsts c;

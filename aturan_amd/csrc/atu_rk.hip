@@ -267,7 +267,22 @@ template <typename S, int STAGE>
 __global__ void __launch_bounds__(256) k_rk_stage(const StageArgs<S> a) {
     const int nk = a.sh.km - 1;
     int kc, jt, il;
-    if (a.xcd_order) {
+    if (a.xcd_order == 2) {
+        // 1-D grid, plane-major. Workgroups go round-robin to the 8 XCDs, so slot r of a plane runs
+        // on XCD r & 7; each XCD is given the same band of j-tiles (all k-chunks) in every plane.
+        // A tile's i+-1 planes are then read through the same L2 a plane's worth of blocks apart,
+        // while the blocks in flight still spread over whole planes.
+        const int njt = (a.sh.jm - 2 + 3) / 4, nkc = (nk + 63) / 64;
+        const int band = (njt + 7) / 8;
+        const int slots = 8 * band * nkc;
+        const int L = (int)blockIdx.x;
+        il = L / slots;
+        const int r = L - il * slots, x = r & 7, m = r >> 3;
+        const int jb0 = (x * njt) / 8, jb1 = ((x + 1) * njt) / 8;
+        if (il >= a.sh.im - 2 || m >= (jb1 - jb0) * nkc) return;
+        jt = jb0 + m / nkc;
+        kc = m - (m / nkc) * nkc;
+    } else if (a.xcd_order) {
         // 1-D grid. Workgroups are dispatched round-robin over the 8 XCDs; remap so that each XCD
         // owns one contiguous range of tiles ordered i fastest, then j, then k: the i+-1 planes a
         // tile's stencils read were just loaded into the same XCD's L2 by its neighbours in i.
@@ -303,7 +318,8 @@ hipError_t launch_stage(int stage, const StageArgs<S>& a, hipStream_t s) {
     const int nk = a.sh.km - 1;
     const dim3 block(64, 4, 1);
     dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + 3) / 4, a.sh.im - 2);
-    if (a.xcd_order) grid = dim3((grid.x * grid.y * grid.z + 7) / 8 * 8, 1, 1);
+    if (a.xcd_order == 2) grid = dim3(8 * ((grid.y + 7) / 8) * grid.x * grid.z, 1, 1);
+    else if (a.xcd_order) grid = dim3((grid.x * grid.y * grid.z + 7) / 8 * 8, 1, 1);
     switch (stage) {
         case 0: hipLaunchKernelGGL((k_rk_stage<S, 0>), grid, block, 0, s, a); break;
         case 1: hipLaunchKernelGGL((k_rk_stage<S, 1>), grid, block, 0, s, a); break;

@@ -41,32 +41,67 @@ __global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P*
     const int kw = (k == 0) ? N - 1 : k - 1, ke = (k == N - 1) ? 0 : k + 1;
     const double* cp = t.line_cp + (size_t)j * ni;
     const double* dn = t.line_dn + (size_t)j * ni;
-    R dprev = R(0.0);
-    for (int s = 0; s < ni; s++) {
-        const int i = s + 1;
-        const size_t row = ((size_t)i * jm + j) * ldk;
-        const size_t x = row + k;
-        const R num1 = (R)g.num1, num2 = (R)g.num2[i], num3 = (R)g.num3[(size_t)i * jm + j];
-        R th;
-        if (j == 1)
-            th = (num2 - num2 * c13) * (R)p[x + ldk];
-        else if (j == jm - 2)
-            th = (num2 - num2 * c13) * (R)p[x - ldk];
-        else
-            th = ((R)p[x + ldk] + (R)p[x - ldk]) * num2;
-        const R rhs = (R)b[x] - th - ((R)p[row + ke] + (R)p[row + kw]) * num3;
-        const R a = (s == 0) ? R(0.0) : ((s == ni - 1) ? num1 - num1 * c13 : num1);
-        const R dp = (rhs - a * dprev) / (R)dn[s];
-        p[x] = (P)dp;
-        dprev = dp;
-    }
     const size_t stride = sh.plane;
-    size_t x = ((size_t)ni * jm + j) * ldk + k;
-    R xv = (R)p[x];
-    for (int s = ni - 2; s >= 0; s--) {
-        x -= stride;
-        xv = (R)p[x] - (R)cp[s] * xv;
-        p[x] = (P)xv;
+    const size_t x1 = ((size_t)jm + j) * ldk + k;   // the column's cell at i = 1
+    const size_t r1 = x1 - k;                        // and its row
+    const R num1 = (R)g.num1;
+    // Forward elimination in groups of LINE_GROUP planes: the right-hand sides of a group (which
+    // read only the other colour and b) are loaded and formed first, so their loads are in flight
+    // together, then the group's recurrence steps run and store.
+    constexpr int LINE_GROUP = 8;
+    R dprev = R(0.0);
+    for (int s0 = 0; s0 < ni; s0 += LINE_GROUP) {
+        R rhs[LINE_GROUP], dns[LINE_GROUP];
+#pragma unroll
+        for (int u = 0; u < LINE_GROUP; u++) {
+            const int s = s0 + u;
+            if (s < ni) {
+                const int i = s + 1;
+                const size_t x = x1 + (size_t)s * stride, row = r1 + (size_t)s * stride;
+                const R num2 = (R)g.num2[i], num3 = (R)g.num3[(size_t)i * jm + j];
+                R th;
+                if (j == 1)
+                    th = (num2 - num2 * c13) * (R)p[x + ldk];
+                else if (j == jm - 2)
+                    th = (num2 - num2 * c13) * (R)p[x - ldk];
+                else
+                    th = ((R)p[x + ldk] + (R)p[x - ldk]) * num2;
+                rhs[u] = (R)b[x] - th - ((R)p[row + ke] + (R)p[row + kw]) * num3;
+                dns[u] = (R)dn[s];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LINE_GROUP; u++) {
+            const int s = s0 + u;
+            if (s < ni) {
+                const R a = (s == 0) ? R(0.0) : ((s == ni - 1) ? num1 - num1 * c13 : num1);
+                const R dp = (rhs[u] - a * dprev) / dns[u];
+                p[x1 + (size_t)s * stride] = (P)dp;
+                dprev = dp;
+            }
+        }
+    }
+    // Back substitution x_s = d_s - cp_s x_{s+1}, from the last plane (whose value is dprev), in
+    // groups loaded ahead the same way.
+    R xv = dprev;
+    for (int s1 = ni - 2; s1 >= 0; s1 -= LINE_GROUP) {
+        R dv[LINE_GROUP], cv[LINE_GROUP];
+#pragma unroll
+        for (int u = 0; u < LINE_GROUP; u++) {
+            const int s = s1 - u;
+            if (s >= 0) {
+                dv[u] = (R)p[x1 + (size_t)s * stride];
+                cv[u] = (R)cp[s];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LINE_GROUP; u++) {
+            const int s = s1 - u;
+            if (s >= 0) {
+                xv = dv[u] - cv[u] * xv;
+                p[x1 + (size_t)s * stride] = (P)xv;
+            }
+        }
     }
 }
 
@@ -131,14 +166,17 @@ __global__ void __launch_bounds__(256) k_col_resid(Shape sh, DevGeo g, DevSolve 
     R[(size_t)(j - 1) * N + k] = s / t.S2;
 }
 
-// forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin
-__global__ void __launch_bounds__(256) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As) {
+// forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin. One
+// 64-lane block per (64 modes, jj); the k sum runs in order, its loads unrolled so that many are
+// in flight at once.
+__global__ void __launch_bounds__(64) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int jj = blockIdx.y * blockDim.y + threadIdx.y;
-    if (m > t.NM || jj >= t.J) return;
+    const int jj = blockIdx.y;
+    if (m > t.NM) return;
     const int N = t.N, M1 = t.NM + 1;
     const double* r = R + (size_t)jj * N;
     double sc = 0.0, ss = 0.0;
+#pragma unroll 8
     for (int k = 0; k < N; k++) {
         const double rv = r[k];
         sc += rv * t.ctT[(size_t)k * M1 + m];
@@ -148,35 +186,47 @@ __global__ void __launch_bounds__(256) k_dft_fwd(DevSolve t, const double* R, do
     As[(size_t)m * t.J + jj] = ss;
 }
 
-// per mode and part: the folded theta system, mode 0 pinned at j = 1
-__global__ void k_baro_tri(DevSolve t, double c13, double* Ac, double* As) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= 2 * (t.NM + 1)) return;
-    const int m = q >> 1, part = q & 1;
+// Per mode and part: the folded theta system, mode 0 pinned at j = 1. One 64-lane block per
+// (mode, part): the right-hand side and the mode's factors are staged in LDS with coalesced loads,
+// one lane runs the two sequential sweeps out of LDS, and the block writes the result back.
+constexpr int kBaroMaxJ = 1024;
+__global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double* Ac, double* As) {
+    const int m = blockIdx.x >> 1, part = blockIdx.x & 1;
     const int J = t.J;
     double* d = (part == 0 ? Ac : As) + (size_t)m * J;
     if (part == 1 && (m == 0 || 2 * m == t.N)) {
-        for (int jj = 0; jj < J; jj++) d[jj] = 0.0;
+        for (int jj = threadIdx.x; jj < J; jj += blockDim.x) d[jj] = 0.0;
         return;
     }
+    __shared__ double sd[kBaroMaxJ], scp[kBaroMaxJ], sdn[kBaroMaxJ];
     const double* cp = t.mode_cp + (size_t)m * J;
     const double* dn = t.mode_dn + (size_t)m * J;
-    const double e = t.e;
-    double prev = 0.0;
-    for (int jj = 0; jj < J; jj++) {
-        double a = (jj == 0) ? 0.0 : e;
-        if (jj == J - 1) a = e - c13 * e;
-        double rhs = d[jj];
-        if (m == 0 && jj == 0) rhs = 0.0;
-        const double dp = (rhs - a * prev) / dn[jj];
-        d[jj] = dp;
-        prev = dp;
+    for (int jj = threadIdx.x; jj < J; jj += blockDim.x) {
+        sd[jj] = d[jj];
+        scp[jj] = cp[jj];
+        sdn[jj] = dn[jj];
     }
-    double x = d[J - 1];
-    for (int jj = J - 2; jj >= 0; jj--) {
-        x = d[jj] - cp[jj] * x;
-        d[jj] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double e = t.e;
+        double prev = 0.0;
+        for (int jj = 0; jj < J; jj++) {
+            double a = (jj == 0) ? 0.0 : e;
+            if (jj == J - 1) a = e - c13 * e;
+            double rhs = sd[jj];
+            if (m == 0 && jj == 0) rhs = 0.0;
+            const double dp = (rhs - a * prev) / sdn[jj];
+            sd[jj] = dp;
+            prev = dp;
+        }
+        double x = sd[J - 1];
+        for (int jj = J - 2; jj >= 0; jj--) {
+            x = sd[jj] - scp[jj] * x;
+            sd[jj] = x;
+        }
     }
+    __syncthreads();
+    for (int jj = threadIdx.x; jj < J; jj += blockDim.x) d[jj] = sd[jj];
 }
 
 // inverse DFT of the correction and its addition to every interior level of the column
@@ -187,6 +237,7 @@ __global__ void __launch_bounds__(256) k_dft_inv_add(Shape sh, DevSolve t, const
     const int jj = blockIdx.y * blockDim.y + threadIdx.y;
     if (k >= N || jj >= J) return;
     double s = Ac[jj] * t.ct[k];
+#pragma unroll 8
     for (int m = 1; m < NM; m++)
         s += 2.0 * (Ac[(size_t)m * J + jj] * t.ct[(size_t)m * N + k] + As[(size_t)m * J + jj] * t.st[(size_t)m * N + k]);
     s += Ac[(size_t)NM * J + jj] * t.ct[(size_t)NM * N + k];
@@ -227,12 +278,27 @@ __global__ void __launch_bounds__(1024) k_dsum_rows(Shape sh, DevGeo g, const P*
     }
 }
 
-__global__ void __launch_bounds__(128) k_dsum_final(int ni, int nj, const double* rows, double* out) {
-    __shared__ double planes[256];
-    for (int i = threadIdx.x; i < ni; i += blockDim.x) {
+// The level sums (rows of a level in order) and their total (levels in order). Each wave takes
+// levels wave, wave + 16, ...: it stages 64 row sums at a time in LDS with one coalesced load and
+// adds them in order; every wave runs the same number of rounds so the block barriers are uniform.
+__global__ void __launch_bounds__(1024) k_dsum_final(int ni, int nj, const double* rows, double* out) {
+    __shared__ double planes[1024];
+    __shared__ double buf[16][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int base = 0; base < ni; base += 16) {
+        const int i = base + wave;
         double s = 0.0;
-        for (int j = 0; j < nj; j++) s += rows[(size_t)i * nj + j];
-        planes[i] = s;
+        for (int j0 = 0; j0 < nj; j0 += 64) {
+            const int j = j0 + lane;
+            buf[wave][lane] = (i < ni && j < nj) ? rows[(size_t)i * nj + j] : 0.0;
+            __syncthreads();
+            if (i < ni) {
+                const int m = min(64, nj - j0);
+                for (int l = 0; l < m; l++) s += buf[wave][l];
+            }
+            __syncthreads();
+        }
+        if (i < ni && lane == 0) planes[i] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -283,8 +349,7 @@ template <typename P, typename B, typename R>
 hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, int colour,
                        hipStream_t s) {
     const int half = (sh.km - 1 + 1) / 2;
-    hipLaunchKernelGGL((k_line<P, B, R>), dim3((half + 63) / 64, (sh.jm - 2 + 3) / 4), dim3(64, 4), 0, s, sh, g, t, p,
-                       b, colour);
+    hipLaunchKernelGGL((k_line<P, B, R>), dim3((half + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, p, b, colour);
     return hipGetLastError();
 }
 
@@ -304,14 +369,14 @@ hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s) {
 template <typename P, typename B>
 hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
                        double* As, hipStream_t s) {
+    if (t.J > kBaroMaxJ) return hipErrorInvalidValue;   // the theta solves stage a mode in LDS
     const int N = sh.km - 1;
-    hipLaunchKernelGGL((k_col_resid<P, B>), dim3((N + 63) / 64, (sh.jm - 2 + 3) / 4), dim3(64, 4), 0, s, sh, g, t,
-                       (const P*)p, b, R);
-    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, (t.J + 3) / 4), dim3(64, 4), 0, s, t, (const double*)R,
-                       Ac, As);
-    hipLaunchKernelGGL(k_baro_tri, dim3((2 * (t.NM + 1) + 63) / 64), dim3(64), 0, s, t, 1.0 / 3.0, Ac, As);
-    hipLaunchKernelGGL((k_dft_inv_add<P>), dim3((N + 63) / 64, (t.J + 3) / 4), dim3(64, 4), 0, s, sh, t,
-                       (const double*)Ac, (const double*)As, p);
+    hipLaunchKernelGGL((k_col_resid<P, B>), dim3((N + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, (const P*)p, b,
+                       R);
+    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, t.J), dim3(64), 0, s, t, (const double*)R, Ac, As);
+    hipLaunchKernelGGL(k_baro_tri, dim3(2 * (t.NM + 1)), dim3(64), 0, s, t, 1.0 / 3.0, Ac, As);
+    hipLaunchKernelGGL((k_dft_inv_add<P>), dim3((N + 63) / 64, t.J), dim3(64), 0, s, sh, t, (const double*)Ac,
+                       (const double*)As, p);
     return hipGetLastError();
 }
 
@@ -335,7 +400,8 @@ hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, c
             hipLaunchKernelGGL((k_dsum_rows<DSUM_RESID2, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, r_out);
             break;
     }
-    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(128), 0, s, sh.im - 2, sh.jm - 2, (const double*)rows, out);
+    if (sh.im - 2 > 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(1024), 0, s, sh.im - 2, sh.jm - 2, (const double*)rows, out);
     return hipGetLastError();
 }
 

@@ -25,12 +25,13 @@ N_INT = 39 * 179 * 360         # integrated interior cells
 # Algorithmic HBM bytes of the four fused RK stages per interior cell: array passes (stage 0: 80,
 # stages 1-2: 118 each, stage 3: 102) x bytes per value of each array.
 RK_PASSES_FP64 = (80, 118, 118, 102)
-# mixed: stencil states, frozen inputs, p_dyn, p_hydro and aux in FP32; y_n and acc in FP64
+# mixed: stencil states, frozen inputs and aux in FP32; p_dyn, p_hydro, y_n and acc in FP64
+_MX_READ = 40 * 4 + 2 * 8 + 19 * 8                               # 19 states + 21 frozen (4 B), p_dyn, p_hydro, y_n
 RK_BYTES_MIXED = (
-    42 * 4 + 19 * 8 + 19 * 4 + 19 * 8,                         # stage 0: reads 19+2+21 (4 B) + y_n; writes acc, next
-    42 * 4 + 19 * 8 + 19 * 8 + 19 * 8 + 19 * 4,                # stages 1-2: + acc read
-    42 * 4 + 19 * 8 + 19 * 8 + 19 * 8 + 19 * 4,
-    42 * 4 + 19 * 8 + 19 * 8 + 19 * 8 + 19 * 4 + 3 * 4,        # stage 3: writes y (FP64) + mirror (FP32) + aux
+    _MX_READ + 19 * 8 + 19 * 4,                                  # stage 0: writes acc, next input
+    _MX_READ + 19 * 8 + 19 * 8 + 19 * 4,                         # stages 1-2: + acc read
+    _MX_READ + 19 * 8 + 19 * 8 + 19 * 4,
+    _MX_READ + 19 * 8 + 19 * 8 + 19 * 4 + 3 * 4,                 # stage 3: reads acc; writes y, its mirror, aux
 )
 ERR_FIELDS = {"u_zonal(w)": "w", "v": "v", "w_radial(u)": "u", "T": "t", "CH4": "ch4", "p": "p_dyn"}
 
@@ -46,6 +47,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true", help="skip the 1/10/100-step mixed-vs-FP64 comparison")
+    ap.add_argument("--adjoint-steps", type=int, default=50, help="window of the adjoint timing (0: skip)")
+    ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of known size (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -129,6 +132,40 @@ def mixed_errors(g, p, F, opts64, optsmx):
     return res
 
 
+def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s):
+    """Forward window (with checkpoints) and hand-written adjoint (recompute + reverse) timings."""
+    import aturan_amd as A
+    with A.UranusCore(g, p, "fp64") as core:
+        core.set_state(F)
+        core.window_forward(min(2, nsteps), checkpoint_every=min(2, K), rel_tol=tol)   # allocations, warm-up
+        core.window_backward(1.0)
+        core.window_rewind()
+        core.synchronize()
+        t0 = time.perf_counter()
+        loss = core.window_forward(nsteps, checkpoint_every=K, rel_tol=tol)
+        core.synchronize()
+        t1 = time.perf_counter()
+        g_t, g_nue = core.window_backward(1.0)
+        core.synchronize()
+        t2 = time.perf_counter()
+        prof_ms = None
+        core.window_rewind()
+        core.window_forward(nsteps, checkpoint_every=K, rel_tol=tol)
+        core.profile_begin()
+        core.window_backward(1.0)
+        prof_ms = core.profile_query("adjoint")[0]
+        core.profile_end()
+    fwd, bwd = t1 - t0, t2 - t1
+    return {"window_steps": nsteps, "checkpoint_every": K, "loss": loss, "forward_s": round(fwd, 4),
+            "backward_s": round(bwd, 4), "backward_over_forward": round(bwd / fwd, 3),
+            "backward_over_plain_steps": round(bwd / (nsteps * fwd_step_s), 3) if fwd_step_s else None,
+            "adjoint_kernels_ms": round(prof_ms, 2) if prof_ms is not None else None,
+            "fwd_plus_adjoint_steps_per_s": round(nsteps / (fwd + bwd), 3),
+            "grad_t0_norm": float((g_t * g_t).sum() ** 0.5), "grad_nue": g_nue,
+            "memory": f"checkpoints {(nsteps + K - 1) // K + 1} x 20 fields + tape {K} x 19 fields of "
+                      f"{g.im}x{g.jm}x384 FP64"}
+
+
 def cpu_baseline(tol, budget_s):
     import aturan_amd as A
     from oracle import binding as O
@@ -210,6 +247,9 @@ def main():
         if not args.no_errors and args.mode == "both":
             mixed["rel_error_vs_fp64"] = mixed_errors(g, p, F, opts64, optsmx)
         line["mixed"] = mixed
+    if args.adjoint_steps > 0 and args.mode in ("both", "fp64"):
+        line["adjoint"] = adjoint_bench(g, p, F, args.adjoint_steps, args.checkpoint_every, args.tol,
+                                        line["ms_per_step"] / 1e3 if line["ms_per_step"] else None)
     line["config"] = cfg
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds)

@@ -256,6 +256,10 @@ typedef struct atu_window_opts {
 int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss);
 /* grad_t0: dense [im][jm][km] host array or NULL; grad_nue: scalar or NULL. */
 int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* grad_nue);
+/* Put the state (prognostic fields and p_dyn) back to the start of the last atu_window_forward,
+ * from its first checkpoint on the device (for repeated evaluations such as finite differences:
+ * rewind, change a field with atu_set_field, run the window again). */
+int atu_window_rewind(atu_ctx* ctx);
 
 /* ---- per-kernel timing (no counterpart in the reference; used by the benchmark) ----
  * Between atu_profile_begin and atu_profile_query every kernel launch of the context is bracketed

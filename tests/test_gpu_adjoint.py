@@ -7,6 +7,8 @@ uniform eddy viscosity nue (relative step 1e-6). The solves run a fixed iteratio
 forward is a smooth map; the adjoint differentiates the solve as exact, which the fixed count
 reaches to the solver's rounding floor. Bar: relative error <= 1e-4.
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -28,34 +30,35 @@ def _fd_check(g, p, F, opts, ndir, seed):
     rng = np.random.default_rng(seed)
     T = A.FIELD["t"]
     nue = A.FIELD["nue"]
+    t0 = time.time()
     with A.UranusCore(g, p) as core:
         core.set_state(F)
         L0 = core.window_forward(**opts)
+        t1 = time.time()
         gT, gN = core.window_backward(1.0)
+        t2 = time.time()
+        print(f"[adjoint] forward {t1 - t0:.2f} s (with upload), backward {t2 - t1:.2f} s", flush=True)
 
-        def loss(G):
-            core.set_state(G)
+        def loss(t_field, nu_field):
+            core.window_rewind()                 # the window's initial state, on the device
+            core.set_field("t", t_field)
+            core.set_field("nue", nu_field)
             return core.window_forward(**opts)
 
         worst = 0.0
         for q in range(ndir):
             d = rng.standard_normal(F[T].shape)
-            Fp, Fm = F.copy(), F.copy()
-            Fp[T] = F[T] + EPS * d
-            Fm[T] = F[T] - EPS * d
-            fd = (loss(Fp) - loss(Fm)) / (2 * EPS)
+            fd = (loss(F[T] + EPS * d, F[nue]) - loss(F[T] - EPS * d, F[nue])) / (2 * EPS)
             ad = float(np.sum(gT * d))
             e = abs(fd - ad) / max(abs(ad), 1e-300)
             worst = max(worst, e)
             print(f"[adjoint] dir {q}: AD {ad:+.12e} FD {fd:+.12e} rel {e:.2e}", flush=True)
         nu = float(F[nue].flat[0])
         h = EPS * nu
-        Fp, Fm = F.copy(), F.copy()
-        Fp[nue] = nu + h
-        Fm[nue] = nu - h
-        fdn = (loss(Fp) - loss(Fm)) / (2 * h)
+        fdn = (loss(F[T], F[nue] + h) - loss(F[T], F[nue] - h)) / (2 * h)
         en = abs(fdn - gN) / max(abs(gN), 1e-300)
-        print(f"[adjoint] nue: AD {gN:+.12e} FD {fdn:+.12e} rel {en:.2e}; L0 {L0:.12e}", flush=True)
+        print(f"[adjoint] nue: AD {gN:+.12e} FD {fdn:+.12e} rel {en:.2e}; L0 {L0:.12e}; "
+              f"{time.time() - t0:.1f} s in all", flush=True)
     return worst, en, gT, gN
 
 
@@ -83,8 +86,13 @@ def test_adjoint_backward_is_repeatable_and_linear(oracle):
         g1, n1 = core.window_backward(1.0)
         g2, n2 = core.window_backward(1.0)
         g3, n3 = core.window_backward(-2.5)
+    print(f"[adjoint] repeat max diff {np.max(np.abs(g1 - g2)):.3e} nue {n1!r} {n2!r}; linearity "
+          f"{np.max(np.abs(g3 + 2.5 * g1)) / np.max(np.abs(g1)):.3e}", flush=True)
     assert np.array_equal(g1, g2) and n1 == n2          # gather-form kernels: no atomics
-    assert np.allclose(g3, -2.5 * g1, rtol=1e-13, atol=0) and abs(n3 + 2.5 * n1) <= 1e-13 * abs(n1)
+    # linear in the seed up to rounding: the scaled seed rounds differently along the way, and the
+    # adjoint solves stop at the solver's rounding floor (~1e-11 relative residual)
+    assert np.max(np.abs(g3 + 2.5 * g1)) <= 1e-9 * np.max(np.abs(g1))
+    assert abs(n3 + 2.5 * n1) <= 1e-9 * abs(n1)
 
 
 @pytest.mark.slow
