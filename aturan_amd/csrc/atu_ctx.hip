@@ -77,7 +77,6 @@ struct atu_ctx {
     float* E32 = nullptr;     // refinement correction
     float* R32 = nullptr;     // refinement residual
     bool mixed() const { return precision == ATU_PREC_MIXED; }
-    int rk_order = 0;         // stage kernel tile order (ATU_RK_ORDER): 0 3-D grid, 1 XCD ranges, 2 XCD j-bands
 
     // gradients over a window of steps (allocated by the first atu_window_forward)
     struct Window {
@@ -308,7 +307,6 @@ StageArgs<double> stage_args(atu_ctx* c, int stage) {
     a.phydro = c->PHYDRO;
     a.aux = c->AUX;
     a.ymirror = nullptr;
-    a.xcd_order = c->rk_order;
     switch (stage) {
         case 0: a.xs = c->Y; a.xnext = c->SA; break;
         case 1: a.xs = c->SA; a.xnext = c->SB; break;
@@ -332,7 +330,6 @@ StageArgs<float> stage_args32(atu_ctx* c, int stage) {
     a.phydro = c->PHYDRO;
     a.aux = c->AUX32;
     a.ymirror = nullptr;
-    a.xcd_order = c->rk_order;
     switch (stage) {
         case 0: a.xs = c->Y32; a.xnext = c->SA32; break;
         case 1: a.xs = c->SA32; a.xnext = c->SB32; break;
@@ -610,14 +607,20 @@ int ck_restore(atu_ctx* c, int slot) {
     return e;
 }
 
-// One step of the window in atu_step's order (the prep on the main stream). With slot >= 0 the
-// step's inputs and stage states go to that tape slot.
-int window_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
+// One step of the window in atu_step's order. With slot >= 0 the step's inputs and stage states go
+// to that tape slot. As in atu_step, the hydrostatic prep of the next step runs on the side stream
+// during this step's Poisson solve: `first` (the step after a state change) prepares its own on the
+// main stream, `last` launches none and leaves no side-stream work behind.
+int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool last) {
     int e;
     if (slot >= 0 && ((e = dcopy(c, tape_field(c, slot, TP_X0), c->Y, 4)) ||
                       (e = dcopy(c, tape_field(c, slot, TP_P), c->PDYN, 1))))
         return e;
-    if ((e = do_hydro(c, c->stream))) return e;
+    if (first) {
+        if ((e = do_hydro(c, c->stream))) return e;
+    } else if ((e = c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent"))) {
+        return e;
+    }
     if (slot >= 0 && (e = c->hip(hipMemcpyAsync(c->win.tape_level + (size_t)slot * c->sh.im, c->LEVEL,
                                                 c->sh.im * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
                                  "hipMemcpyAsync(level)")))
@@ -630,6 +633,11 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
     }
     if ((e = do_bc(c))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))) return e;
+    if (!last && ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
+                  (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")) ||
+                  (e = do_hydro(c, c->side)) ||
+                  (e = c->hip(hipEventRecord(c->ev_hydro, c->side), "hipEventRecord"))))
+        return e;
     if ((e = do_pressure(c, &o, nullptr))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_SRC), c->SRC, 1))) return e;
     return ATU_OK;
@@ -754,7 +762,6 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     c->grid = *g;
     c->params = *p;
     c->precision = precision;
-    if (const char* ord = std::getenv("ATU_RK_ORDER")) c->rk_order = std::min(2, std::max(0, std::atoi(ord)));
     c->sh.im = g->im;
     c->sh.jm = g->jm;
     c->sh.km = g->km;
@@ -861,8 +868,12 @@ int atu_set_field(atu_ctx* ctx, int f, const double* host) {
     int e = put_field(ctx, d, host);
     if (!e && !ctx->mixed() && f < ATU_NPROG) {
         // restoreVar's role: every stage buffer starts from the same state, boundary planes included
-        e = put_field(ctx, ctx->prog(ctx->SA, f), host);
-        if (!e) e = put_field(ctx, ctx->prog(ctx->SB, f), host);
+        // (copied on the device from the uploaded field)
+        e = ctx->hip(hipMemcpyAsync(ctx->prog(ctx->SA, f), d, fs * sizeof(double), hipMemcpyDeviceToDevice,
+                                    ctx->stream), "hipMemcpyAsync(D2D)");
+        if (!e)
+            e = ctx->hip(hipMemcpyAsync(ctx->prog(ctx->SB, f), d, fs * sizeof(double), hipMemcpyDeviceToDevice,
+                                        ctx->stream), "hipMemcpyAsync(D2D)");
     }
     if (!e && ctx->mixed()) {
         if (f < ATU_NPROG) {
@@ -972,7 +983,7 @@ int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) 
     const atu_solver_opts o = window_solver(W.opts);
     for (int n = 0; n < N; n++) {
         if (n % W.K == 0 && (e = ck_save(ctx, n / W.K))) return e;
-        if ((e = window_step(ctx, o, -1))) return e;
+        if ((e = window_step(ctx, o, -1, n % W.K == 0, n == N - 1 || (n + 1) % W.K == 0))) return e;
     }
     if ((e = ck_save(ctx, W.nck)) || (e = window_loss(ctx, 1.0, &W.loss))) return e;
     if (loss) *loss = W.loss;
@@ -997,7 +1008,7 @@ int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* gra
         const int n0 = seg * W.K, n1 = std::min(n0 + W.K, N);
         if ((e = ck_restore(ctx, seg))) return e;
         for (int n = n0; n < n1; n++)
-            if ((e = window_step(ctx, o, n - n0))) return e;
+            if ((e = window_step(ctx, o, n - n0, n == n0, n == n1 - 1))) return e;
         for (int n = n1 - 1; n >= n0; n--)
             if ((e = window_reverse_step(ctx, o, n - n0))) return e;
     }

@@ -40,7 +40,6 @@ struct StageArgs {
     const double* pdyn;  // FP64 in both precisions: the pressure gradients are differences of
     const double* phydro;// nearby large values, formed in FP64 and rounded once
     S* aux;              // 3 fields, written at stage 3
-    int xcd_order;       // 0: (k, j, i) 3-D grid; 1: per-XCD tile ranges, i fastest; 2: per-XCD j-bands, plane-major
     Shape sh;
     DevGeo geo;
     StepConsts c;
