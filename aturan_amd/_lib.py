@@ -34,6 +34,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 SOLVE_REFERENCE = 0
 SOLVE_CONVERGE = 1
+SOLVE_URAN = 2
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "non-finite value",
           -5: "no convergence", -6: "unsupported"}
@@ -97,6 +98,7 @@ _SIGS = {
     "atu_apply_bc": (C.c_int, [C.c_void_p]),
     "atu_pressure_solve": (C.c_int, [C.c_void_p, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "atu_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
+    "atu_keep_stage_rhs": (C.c_int, [C.c_void_p, C.c_int]),
     "atu_pressure_residual": (C.c_int, [C.c_void_p, _DP, _DP]),
     "atu_field_device_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),

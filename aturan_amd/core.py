@@ -67,7 +67,7 @@ class SolveResult:
 def solver_opts(mode: str = "reference", sweeps: int = 1, omega: float = 1.0, rel_tol: float = 1e-8,
                 max_sweeps: int = 200) -> L.SolverOpts:
     o = L.SolverOpts()
-    o.mode = L.SOLVE_REFERENCE if mode == "reference" else L.SOLVE_CONVERGE
+    o.mode = {"reference": L.SOLVE_REFERENCE, "converge": L.SOLVE_CONVERGE, "uran": L.SOLVE_URAN}[mode]
     o.sweeps = sweeps
     o.max_sweeps = max_sweeps
     o.omega = omega
@@ -137,6 +137,10 @@ class UranusCore:
 
     def RungeKuttaUran(self) -> None:
         _check(self._lib.atu_rk4(self._ctx), self._ctx, "atu_rk4")
+
+    def keep_stage_rhs(self, on: bool = True) -> None:
+        """Keep rhs_u, rhs_v, rhs_w of the last RK stage (read by the "uran" solve)."""
+        _check(self._lib.atu_keep_stage_rhs(self._ctx, int(on)), self._ctx, "atu_keep_stage_rhs")
 
     def boundary_conditions(self) -> None:
         """BC_Uran(m).bcRadius(); bcTheta(); bcPhi() on the prognostic fields."""

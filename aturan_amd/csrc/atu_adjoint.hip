@@ -1,4 +1,4 @@
-// Discrete adjoint of the FP64 step (hydrostatic prep, RK4, domain BC, converging Poisson solve)
+// Discrete adjoint of the step (hydrostatic prep, RK4, domain BC, converging Poisson solve), in FP64,
 // for gradients of a zonal-wind misfit. There is no counterpart in the reference; the forward it
 // differentiates is the one of atu_rk.hip, atu_hydro_bc.hip, atu_poisson.hip and atu_solve.hip.
 //

@@ -68,6 +68,16 @@ struct atu_ctx {
     double* BR = nullptr;     // J*N
     double* BAC = nullptr;    // (NM+1)*J
     double* BAS = nullptr;
+    int* CTL = nullptr;       // device-side convergence control (CTL_N ints)
+    // PressureSolverUran: its weight tables, the last stage's rhs_u, rhs_v, rhs_w
+    UranTables huran;
+    UranTabs duran{};
+    void* uran_blob = nullptr;
+    double* RHS3 = nullptr;   // 3 fields, allocated when first kept
+    bool keep_rhs = false;    // stage 3 writes RHS3
+    bool rhs_valid = false;   // RHS3 holds the tendencies of the last RK4
+    int pred_iters = 4;       // iterations (FP64) or refinements (mixed) the last solve needed: the
+    int pred_outer = 2;       // number queued before the host first reads the outcome
     // mixed path: FP32 stencil-side mirrors and stage states, the FP32 correction and residual
     float* Y32 = nullptr;     // 19, FP32 mirror of y_n (the stencil of stage 0)
     float* SA32 = nullptr;    // 19
@@ -146,6 +156,9 @@ atu_ctx::~atu_ctx() {
                       BR, BAC, BAS})
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
+    if (CTL) hipFree(CTL);
+    if (uran_blob) hipFree(uran_blob);
+    if (RHS3) hipFree(RHS3);
     for (double* p : {win.ckpt, win.tape, win.tape_level, win.LY, win.LYN, win.LS, win.LK, win.LK2, win.LAUX, win.LP,
                       win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.B, win.LSRC, win.GREF, win.TGT, win.ROWS,
                       win.PART, win.SC})
@@ -210,8 +223,8 @@ int upload_geometry(atu_ctx* c) {
 
 int upload_solve_tables(atu_ctx* c) {
     const SolveTables& t = c->htab;
-    const std::vector<const std::vector<double>*> parts = {&t.line_cp, &t.line_dn, &t.mode_cp, &t.mode_dn,
-                                                           &t.ct, &t.st, &t.ctT, &t.stT};
+    const std::vector<const std::vector<double>*> parts = {&t.line_cp, &t.line_dn, &t.line_inv_dn, &t.mode_cp,
+                                                           &t.mode_dn, &t.ct, &t.st, &t.ctT, &t.stT};
     std::vector<double> blob;
     std::vector<size_t> off;
     for (auto* v : parts) {
@@ -224,7 +237,8 @@ int upload_solve_tables(atu_ctx* c) {
         return ATU_EHIP;
     const double* d = (const double*)c->solve_blob;
     DevSolve& s = c->dsol;
-    const double** dst[] = {&s.line_cp, &s.line_dn, &s.mode_cp, &s.mode_dn, &s.ct, &s.st, &s.ctT, &s.stT};
+    const double** dst[] = {&s.line_cp, &s.line_dn, &s.line_inv_dn, &s.mode_cp, &s.mode_dn,
+                            &s.ct, &s.st, &s.ctT, &s.stT};
     for (size_t q = 0; q < parts.size(); q++) *dst[q] = d + off[q];
     s.ni = t.ni;
     s.N = t.N;
@@ -239,8 +253,37 @@ int upload_solve_tables(atu_ctx* c) {
         c->hip(hipMalloc((void**)&c->SCAL, 8 * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->BR, (size_t)t.J * t.N * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->BAC, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc") ||
-        c->hip(hipMalloc((void**)&c->BAS, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc"))
+        c->hip(hipMalloc((void**)&c->BAS, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc") ||
+        c->hip(hipMalloc((void**)&c->CTL, CTL_N * sizeof(int)), "hipMalloc"))
         return ATU_ENOMEM;
+    return ATU_OK;
+}
+
+int upload_uran_tables(atu_ctx* c) {
+    const UranTables& u = c->huran;
+    const std::vector<const std::vector<double>*> parts = {&u.a_inv_rm, &u.a_num2, &u.m_num2, &u.a_inv_rmsinthe,
+                                                           &u.a_num3, &u.a_denom, &u.m_num3, &u.m_inv_denom};
+    std::vector<double> blob;
+    std::vector<size_t> off;
+    for (auto* v : parts) {
+        off.push_back(blob.size());
+        blob.insert(blob.end(), v->begin(), v->end());
+    }
+    if (c->hip(hipMalloc(&c->uran_blob, blob.size() * sizeof(double)), "hipMalloc(uran tables)") ||
+        c->hip(hipMemcpy(c->uran_blob, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice),
+               "hipMemcpy(uran tables)"))
+        return ATU_EHIP;
+    const double* d = (const double*)c->uran_blob;
+    UranTabs& t = c->duran;
+    const double** dst[] = {&t.a_inv_rm, &t.a_num2, &t.m_num2, &t.a_inv_rmsinthe,
+                            &t.a_num3, &t.a_denom, &t.m_num3, &t.m_inv_denom};
+    for (size_t q = 0; q < parts.size(); q++) *dst[q] = d + off[q];
+    t.a_num1 = u.a_num1;
+    t.m_num1 = u.m_num1;
+    t.two_dr = u.two_dr;
+    t.two_dthe = u.two_dthe;
+    t.two_dphi = u.two_dphi;
+    t.src_scale = u.src_scale;
     return ATU_OK;
 }
 
@@ -306,6 +349,7 @@ StageArgs<double> stage_args(atu_ctx* c, int stage) {
     a.pdyn = c->PDYN;
     a.phydro = c->PHYDRO;
     a.aux = c->AUX;
+    a.rhs3 = c->keep_rhs ? c->RHS3 : nullptr;
     a.ymirror = nullptr;
     switch (stage) {
         case 0: a.xs = c->Y; a.xnext = c->SA; break;
@@ -340,6 +384,7 @@ StageArgs<float> stage_args32(atu_ctx* c, int stage) {
 }
 
 int do_rk4(atu_ctx* c) {
+    c->rhs_valid = c->keep_rhs && !c->mixed();
     for (int s = 0; s < 4; s++) {
         hipError_t e;
         if (c->mixed()) {
@@ -411,11 +456,18 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
                                                                   c->SCAL + 2, s); }), "dsum")))
         return e;
     const int max_it = o->max_sweeps > 0 ? o->max_sweeps : 1;
+    // Convergence is decided on the device (SolveCtl): the iterations' kernels return at once after
+    // the residual norm has met the tolerance, so the host queues as many iterations as the last
+    // solve needed and reads the outcome once, then one at a time if more are needed.
+    int* const F = c->CTL;
+    const int* done = F + CTL_DONE;
+    int flags[CTL_N] = {0, 0, 0, 0};
     int it = 0, outer = 0;
     double rel = INFINITY, bnorm = 0.0;
-    auto read_rel = [&]() -> int {
+    auto read_ctl = [&]() -> int {
         double h[2];
         if (H(hipMemcpyAsync(h, c->SCAL + 2, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpy(norms)") ||
+            H(hipMemcpyAsync(flags, F, sizeof(flags), hipMemcpyDeviceToHost, s), "hipMemcpy(ctl)") ||
             H(hipStreamSynchronize(s), "hipStreamSynchronize"))
             return ATU_EHIP;
         bnorm = std::sqrt(h[0]);
@@ -423,56 +475,83 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         if (!std::isfinite(rel)) return c->fail(ATU_ENONFINITE, "converging solve: non-finite residual");
         return ATU_OK;
     };
+    if ((e = H(hipMemsetAsync(F, 0, CTL_N * sizeof(int), s), "hipMemsetAsync(ctl)"))) return e;
+    const bool fixed = !(o->rel_tol > 0.0);   // no tolerance: run max_it iterations in one go
     if (!c->mixed()) {
-        while (it < max_it) {
-            for (int colour = 0; colour < 2; colour++)
-                if ((e = H(c->timed(PC_LINE, [&] { return launch_line<double, double, double>(sh, c->geo, c->dsol,
-                                                                                               P, c->RES, colour,
-                                                                                               s); }), "line")))
-                    return e;
-            if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")) ||
-                (e = H(c->timed(PC_BARO, [&] { return launch_baro<double, double>(sh, c->geo, c->dsol, P, c->RES,
-                                                                                   c->BR, c->BAC, c->BAS, s); }),
-                       "baro")) ||
-                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")) ||
-                (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES,
-                                                                          c->SROWS, c->SCAL + 3, s); }), "dsum")))
-                return e;
-            it++;
-            if ((e = read_rel())) return e;
-            if (rel <= o->rel_tol) break;
-        }
-        outer = 1;
-    } else {
-        const int inner = o->sweeps > 0 ? o->sweeps : 4;
-        while (true) {
-            // FP64 residual of the current p, kept in FP32 as the right-hand side of the correction
-            if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES,
-                                                                          c->SROWS, c->SCAL + 3, s, c->R32); }),
-                       "dsum")) ||
-                (e = read_rel()))
-                return e;
-            if (rel <= o->rel_tol || it + inner > max_it) break;
-            if ((e = H(launch_fill(c->E32, 0.0f, sh.fs, s), "fill"))) return e;
-            for (int q = 0; q < inner; q++) {
+        const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, 1, 0};
+        int launched = 0, batch = fixed ? max_it : std::max(1, c->pred_iters);
+        while (launched < max_it) {
+            const int nb = std::min(batch, max_it - launched);
+            for (int q = 0; q < nb; q++) {
                 for (int colour = 0; colour < 2; colour++)
-                    if ((e = H(c->timed(PC_LINE, [&] { return launch_line<float, float, float>(sh, c->geo, c->dsol,
-                                                                                                c->E32, c->R32, colour,
-                                                                                                s); }), "line")))
+                    if ((e = H(c->timed(PC_LINE, [&] { return launch_line<double, double, double>(
+                                                           sh, c->geo, c->dsol, P, c->RES, colour, s, done); }),
+                               "line")))
                         return e;
-                if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s); }), "walls")) ||
-                    (e = H(c->timed(PC_BARO, [&] { return launch_baro<float, float>(sh, c->geo, c->dsol, c->E32,
-                                                                                     c->R32, c->BR, c->BAC, c->BAS,
-                                                                                     s); }), "baro")) ||
-                    (e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s); }), "walls")))
+                if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, done); }), "walls")) ||
+                    (e = H(c->timed(PC_BARO, [&] { return launch_baro<double, double>(sh, c->geo, c->dsol, P, c->RES,
+                                                                                       c->BR, c->BAC, c->BAS, s,
+                                                                                       done); }),
+                           "baro")) ||
+                    (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, done); }), "walls")) ||
+                    (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES,
+                                                                              c->SROWS, c->SCAL + 3, s, nullptr,
+                                                                              chk); }), "dsum")))
                     return e;
-                it++;
             }
-            if ((e = H(c->timed(PC_LINE, [&] { return launch_add_correction(sh, c->E32, P, s); }), "add")) ||
-                (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")))
-                return e;
-            outer++;
+            launched += nb;
+            if ((e = read_ctl())) return e;
+            if (flags[CTL_DONE]) break;
+            batch = 1;
         }
+        it = flags[CTL_ITERS];
+        outer = 1;
+        c->pred_iters = it;
+    } else {
+        // The FP64 residual of p, kept in FP32 as the right-hand side of the correction, and its
+        // test; then refinements, each `inner` FP32 iterations on the correction e, p += e in FP64,
+        // and the next residual with its test.
+        const int inner = o->sweeps > 0 ? o->sweeps : 2;
+        const int max_ref = max_it / inner;
+        const SolveCtl first{F, c->SCAL + 2, o->rel_tol, 0, 0};
+        const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, inner, 1};
+        auto residual = [&](const SolveCtl& k) {
+            return H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES, c->SROWS,
+                                                                        c->SCAL + 3, s, c->R32, k); }), "dsum");
+        };
+        if ((e = residual(first))) return e;
+        int launched = 0, batch = fixed ? max_ref : std::max(1, c->pred_outer);
+        while (launched < max_ref) {
+            const int nb = std::min(batch, max_ref - launched);
+            for (int q = 0; q < nb; q++) {
+                if ((e = H(launch_fill(c->E32, 0.0f, sh.fs, s, done), "fill"))) return e;
+                for (int r = 0; r < inner; r++) {
+                    for (int colour = 0; colour < 2; colour++)
+                        if ((e = H(c->timed(PC_LINE, [&] { return launch_line<float, float, float>(
+                                                               sh, c->geo, c->dsol, c->E32, c->R32, colour, s, done); }),
+                                   "line")))
+                            return e;
+                    if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s, done); }), "walls")) ||
+                        (e = H(c->timed(PC_BARO, [&] { return launch_baro<float, float>(sh, c->geo, c->dsol, c->E32,
+                                                                                         c->R32, c->BR, c->BAC, c->BAS,
+                                                                                         s, done); }), "baro")) ||
+                        (e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s, done); }), "walls")))
+                        return e;
+                }
+                if ((e = H(c->timed(PC_LINE, [&] { return launch_add_correction(sh, c->E32, P, s, done); }), "add")) ||
+                    (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, done); }), "walls")) ||
+                    (e = residual(chk)))
+                    return e;
+            }
+            launched += nb;
+            if ((e = read_ctl())) return e;
+            if (flags[CTL_DONE]) break;
+            batch = 1;
+        }
+        if (max_ref == 0 && (e = read_ctl())) return e;
+        it = flags[CTL_ITERS];
+        outer = flags[CTL_OUTER];
+        c->pred_outer = std::max(1, outer);
     }
     if ((e = H(launch_p_corners<double>(sh, c->consts, P, s), "corners")) ||
         (e = H(launch_seam<double>(sh, P, s), "seam")) ||
@@ -490,8 +569,31 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     return ATU_OK;
 }
 
+// PressureSolverUran::run, `sweeps` calls; SRC receives the defect-correction source of the last
+int do_uran(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
+    if (c->mixed()) return c->fail(ATU_EUNSUPPORTED, "ATU_SOLVE_URAN runs on FP64 contexts only");
+    if (!c->rhs_valid)
+        return c->fail(ATU_EINVAL, "ATU_SOLVE_URAN reads the last stage's rhs_u/v/w: call atu_keep_stage_rhs(ctx, 1) "
+                                   "before atu_rk4 (atu_step does this itself)");
+    if (c->sh.im - 2 < 1 || c->sh.jm - 2 < 1) return c->fail(ATU_EINVAL, "grid too small");
+    const int calls = o->sweeps > 0 ? o->sweeps : 1;
+    for (int q = 0; q < calls; q++)
+        if (c->hip(c->timed(PC_RB, [&] { return launch_uran_call(c->sh, c->duran, c->AUX, c->RHS3, c->PDYN, c->SRC,
+                                                                 c->stream); }),
+                   "uran"))
+            return ATU_EHIP;
+    if (st) {
+        st->sweeps_done = calls;
+        st->outer_iters = 1;
+        int e = do_residual(c, &st->rel_resid, &st->src_norm);
+        if (e) return e;
+    }
+    return ATU_OK;
+}
+
 int do_pressure_fp64(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
     if (o->mode == ATU_SOLVE_CONVERGE) return do_converge(c, o, st, c->PDYN, c->SRC);
+    if (o->mode == ATU_SOLVE_URAN) return do_uran(c, o, st);
     if (o->mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EINVAL, "unknown solve mode");
     const int sweeps = o->sweeps > 0 ? o->sweeps : 1;   // PressureSolver.h:174-175
     for (int s = 0; s < sweeps; s++) {
@@ -520,6 +622,7 @@ int do_pressure_fp64(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) 
 int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
     atu_solver_opts def{ATU_SOLVE_REFERENCE, 1, 0, 0, 1.0, 0.0};
     if (!o) o = &def;
+    if (o->mode == ATU_SOLVE_URAN) return do_uran(c, o, st);   // its own aux walls and source
     const hipError_t pe = c->mixed()
         ? c->timed(PC_PROJ, [&] { return launch_proj_src<float>(c->sh, c->geo, c->consts, c->AUX32, c->SRC, c->stream); })
         : c->timed(PC_PROJ, [&] { return launch_proj_src<double>(c->sh, c->geo, c->consts, c->AUX, c->SRC, c->stream); });
@@ -600,10 +703,17 @@ int ck_save(atu_ctx* c, int slot) {
 // y_n into the state and both stage buffers (their boundary planes are the state's), p_dyn
 int ck_restore(atu_ctx* c, int slot) {
     const double* d = c->win.ckpt + (size_t)slot * kCkFields * c->sh.fs;
+    const size_t n = (size_t)ATU_NPROG * c->sh.fs;
     int e = dcopy(c, c->Y, d, ATU_NPROG);
-    if (!e) e = dcopy(c, c->SA, d, ATU_NPROG);
-    if (!e) e = dcopy(c, c->SB, d, ATU_NPROG);
-    if (!e) e = dcopy(c, c->PDYN, d + (size_t)ATU_NPROG * c->sh.fs, 1);
+    if (!c->mixed()) {
+        if (!e) e = dcopy(c, c->SA, d, ATU_NPROG);
+        if (!e) e = dcopy(c, c->SB, d, ATU_NPROG);
+    } else {   // the FP32 stencil states start from the FP64 state rounded once
+        if (!e) e = c->hip(launch_convert(c->Y, c->Y32, n, c->stream), "convert");
+        if (!e) e = c->hip(launch_convert(c->Y, c->SA32, n, c->stream), "convert");
+        if (!e) e = c->hip(launch_convert(c->Y, c->SB32, n, c->stream), "convert");
+    }
+    if (!e) e = dcopy(c, c->PDYN, d + n, 1);
     return e;
 }
 
@@ -626,10 +736,20 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
                                  "hipMemcpyAsync(level)")))
         return e;
     for (int s = 0; s < 4; s++) {
-        const StageArgs<double> a = stage_args(c, s);
-        if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")))
-            return e;
-        if (slot >= 0 && s < 3 && (e = dcopy(c, tape_field(c, slot, TP_X1 + 4 * s), a.xnext, 4))) return e;
+        if (!c->mixed()) {
+            const StageArgs<double> a = stage_args(c, s);
+            if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")))
+                return e;
+            if (slot >= 0 && s < 3 && (e = dcopy(c, tape_field(c, slot, TP_X1 + 4 * s), a.xnext, 4))) return e;
+        } else {   // mixed: the FP32 stage states, taped widened to FP64
+            const StageArgs<float> a = stage_args32(c, s);
+            if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_mixed(s, a, c->stream); }), "rk_stage")))
+                return e;
+            if (slot >= 0 && s < 3 &&
+                (e = c->hip(launch_convert(a.xnext, tape_field(c, slot, TP_X1 + 4 * s), 4 * c->sh.fs, c->stream),
+                            "convert")))
+                return e;
+        }
     }
     if ((e = do_bc(c))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))) return e;
@@ -771,6 +891,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     build_geometry(*g, *p, c->hgeo);
     build_step_consts(*g, *p, c->consts);
     build_solve_tables(*g, *p, c->hgeo, c->htab);
+    build_uran_tables(*g, *p, c->hgeo, c->huran);
     int e;
     if ((e = c->hip(hipStreamCreate(&c->stream), "hipStreamCreate"))) {
         g_create_error = c->err;
@@ -783,7 +904,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         g_create_error = c->err;
         return e;
     }
-    if ((e = upload_geometry(c.get())) || (e = upload_solve_tables(c.get())) ||
+    if ((e = upload_geometry(c.get())) || (e = upload_solve_tables(c.get())) || (e = upload_uran_tables(c.get())) ||
         (e = alloc_fields(c.get(), &c->Y, ATU_NPROG)) ||
         (e = alloc_fields(c.get(), &c->ACC, ATU_NPROG)) || (e = alloc_fields(c.get(), &c->FROZEN, ATU_NFROZEN)) ||
         (e = alloc_fields(c.get(), &c->PSTAT, 1)) || (e = alloc_fields(c.get(), &c->PDYN, 1)) ||
@@ -933,8 +1054,25 @@ int atu_pressure_residual(atu_ctx* ctx, double* rel, double* srcn) {
     return do_residual(ctx, rel, srcn);
 }
 
+int atu_keep_stage_rhs(atu_ctx* ctx, int on) {
+    if (!ctx) return ATU_EINVAL;
+    if (on && ctx->mixed()) return ctx->fail(ATU_EUNSUPPORTED, "atu_keep_stage_rhs: FP64 contexts only");
+    if (on && !ctx->RHS3) {
+        if (sync_all(ctx)) return ATU_EHIP;
+        int e = alloc_fields(ctx, &ctx->RHS3, 3);   // zero: the polar rows of rhs_v, rhs_w stay 0
+        if (e) return e;
+    }
+    ctx->keep_rhs = on != 0;
+    if (!ctx->keep_rhs) ctx->rhs_valid = false;
+    return ATU_OK;
+}
+
 int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_stats* stats) {
     if (!ctx || nsteps < 0) return ATU_EINVAL;
+    if (opts && opts->mode == ATU_SOLVE_URAN && !ctx->keep_rhs) {
+        const int e = atu_keep_stage_rhs(ctx, 1);
+        if (e) return e;
+    }
     if (opts && !(opts->omega > 0.0 && opts->omega < 2.0))
         return ctx->fail(ATU_EINVAL, "atu_step: omega must lie in (0, 2)");
     // The hydrostatic prep of step n+1 reads only t, v, w after the BC of step n, and the Poisson
@@ -961,7 +1099,6 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
 
 int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) {
     if (!ctx || !opts || opts->nsteps < 1) return ATU_EINVAL;
-    if (ctx->mixed()) return ctx->fail(ATU_EUNSUPPORTED, "atu_window_forward: needs an FP64 context");
     if (!ctx->params.rigid_lid) return ctx->fail(ATU_EUNSUPPORTED, "atu_window_forward: needs rigid_lid = 1");
     if (opts->solve_iters <= 0 && !(opts->rel_tol > 0.0))
         return ctx->fail(ATU_EINVAL, "atu_window_forward: give solve_iters > 0 or rel_tol > 0");

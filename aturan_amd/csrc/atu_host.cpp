@@ -198,6 +198,7 @@ void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& 
     t.ni = n;
     t.line_cp.assign((size_t)jm * n, 0.0);
     t.line_dn.assign((size_t)jm * n, 0.0);
+    t.line_inv_dn.assign((size_t)jm * n, 0.0);
     for (int j = 1; j < jm - 1; j++) {
         double* cp = &t.line_cp[(size_t)j * n];
         double* dn = &t.line_dn[(size_t)j * n];
@@ -213,6 +214,7 @@ void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& 
             const double d = (s == 0) ? diag : diag - a * cp[s - 1];
             dn[s] = d;
             cp[s] = c / d;
+            t.line_inv_dn[(size_t)j * n + s] = 1.0 / d;
         }
     }
     const int N = km - 1, NM = N / 2, J = jm - 2;
@@ -264,6 +266,53 @@ void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& 
             t.mode_dn[(size_t)m * J + jj] = d;
             t.mode_cp[(size_t)m * J + jj] = c / d;
             prev_cp = t.mode_cp[(size_t)m * J + jj];
+        }
+    }
+}
+
+// PressureSolverUran::run's weights at PRESS_CONSISTENT (planet/PressureSolverUran.h:40-46, 67-68,
+// 146-158 for the source, 180-195 for the sweep). Its sine table is a lone sin() per row, floored.
+void build_uran_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, UranTables& u) {
+    const int im = g.im, jm = g.jm;
+    std::vector<double> sinthe(jm);
+    for (int j = 0; j < jm; j++) {
+        sinthe[j] = std::sin(geo.thez[j]);
+        if (sinthe[j] < p.sinthe_min) sinthe[j] = p.sinthe_min;
+    }
+    const double dr = g.dr, dthe = g.dthe, dphi = g.dphi;
+    const double inv_dr2 = 1.0 / (dr * dr), inv_dthe2 = 1.0 / (dthe * dthe), inv_dphi2 = 1.0 / (dphi * dphi);
+    u.a_inv_rm.assign(im, 0); u.a_num2.assign(im, 0); u.m_num2.assign(im, 0);
+    const size_t nij = (size_t)im * jm;
+    u.a_inv_rmsinthe.assign(nij, 0); u.a_num3.assign(nij, 0); u.a_denom.assign(nij, 0);
+    u.m_num3.assign(nij, 0); u.m_inv_denom.assign(nij, 0);
+    u.a_num1 = 1.0 / (dr * dr);
+    u.m_num1 = inv_dr2;
+    u.two_dr = 2.0 * dr;
+    u.two_dthe = 2.0 * dthe;
+    u.two_dphi = 2.0 * dphi;
+    u.src_scale = (p.r_mix * p.u_0 * p.u_0) / 1.0e5;
+    for (int i = 0; i < im; i++) {
+        const double rm = geo.rm[i];
+        // the source's set
+        const double inv_rm = 1.0 / rm;
+        const double inv_rm2 = inv_rm * inv_rm;
+        u.a_inv_rm[i] = inv_rm;
+        u.a_num2[i] = inv_rm2 / (dthe * dthe);
+        // the sweep's set
+        const double rm2 = rm * rm;
+        const double inv_rm2_m = 1.0 / rm2;
+        u.m_num2[i] = inv_rm2_m * inv_dthe2;
+        for (int j = 0; j < jm; j++) {
+            const size_t q = (size_t)i * jm + j;
+            const double s = sinthe[j];
+            u.a_inv_rmsinthe[q] = 1.0 / (rm * s);
+            const double inv_rm2sinthe2 = inv_rm2 / (s * s);
+            u.a_num3[q] = inv_rm2sinthe2 / (dphi * dphi);
+            u.a_denom[q] = 2.0 * (u.a_num1 + u.a_num2[i] + u.a_num3[q]);
+            const double inv_rm2sinthe2_m = inv_rm2_m / (s * s);
+            const double denom = 2.0 * inv_dr2 + 2.0 * inv_rm2_m * inv_dthe2 + 2.0 * inv_rm2sinthe2_m * inv_dphi2;
+            u.m_inv_denom[q] = 1.0 / denom;
+            u.m_num3[q] = inv_rm2sinthe2_m * inv_dphi2;
         }
     }
 }

@@ -113,7 +113,8 @@ __global__ void __launch_bounds__(256) k_convert(const A* src, B* dst, size_t n)
     if (q < n) dst[q] = (B)src[q];
 }
 
-__global__ void __launch_bounds__(256) k_fill(float* dst, float v, size_t n) {
+__global__ void __launch_bounds__(256) k_fill(float* dst, float v, size_t n, const int* done) {
+    if (done && *done) return;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) dst[q] = v;
 }
@@ -279,8 +280,8 @@ hipError_t launch_convert(const float* src, double* dst, size_t n, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, v, n);
+hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s, const int* done) {
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, v, n, done);
     return hipGetLastError();
 }
 

@@ -38,12 +38,22 @@ void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo);
 struct SolveTables {
     int ni = 0, N = 0, NM = 0, J = 0;
     std::vector<double> line_cp, line_dn;   // [jm][ni]
+    std::vector<double> line_inv_dn;        // [jm][ni], 1 / line_dn (the FP32 line pass multiplies)
     std::vector<double> mode_cp, mode_dn;   // [NM+1][J]
     std::vector<double> ct, st;             // [NM+1][N]
     std::vector<double> ctT, stT;           // [N][NM+1]
     double S2 = 0.0, yw = 0.0, ncell = 0.0, e = 0.0;
 };
 void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, SolveTables& t);
+
+// Weights of PressureSolverUran (planet/PressureSolverUran.h): the defect-correction source's set
+// (a_*) and the sweep's set (m_*), each formed with the reference's own expressions.
+struct UranTables {
+    std::vector<double> a_inv_rm, a_num2, m_num2;                                  // [im]
+    std::vector<double> a_inv_rmsinthe, a_num3, a_denom, m_num3, m_inv_denom;     // [im][jm]
+    double a_num1 = 0, m_num1 = 0, two_dr = 0, two_dthe = 0, two_dphi = 0, src_scale = 0;
+};
+void build_uran_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, UranTables& u);
 
 // cp table of H2 for a CpH2 mode (1 normal, 2 equilibrium), J/(kg K).
 const std::vector<double>& cp_h2_table(int mode);

@@ -40,6 +40,7 @@ struct StageArgs {
     const double* pdyn;  // FP64 in both precisions: the pressure gradients are differences of
     const double* phydro;// nearby large values, formed in FP64 and rounded once
     S* aux;              // 3 fields, written at stage 3
+    double* rhs3;        // when non-null, stage 3 also writes rhs_u, rhs_v, rhs_w (FP64 storage)
     Shape sh;
     DevGeo geo;
     StepConsts c;
@@ -58,7 +59,7 @@ hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, c
 // Element-wise precision conversion of n values (FP32 mirrors of FP64 fields and back).
 hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s);
 hipError_t launch_convert(const float* src, double* dst, size_t n, hipStream_t s);
-hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s);
+hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s, const int* done = nullptr);
 
 // Domain BC on the fields of the path that the reference's lists name: the 17 prognostic fields of
 // `y` (FP64; every value written also goes to each non-null mirror, FP64 `m64a`, `m64b`, FP32
@@ -93,33 +94,55 @@ hipError_t launch_p_corners(const Shape& sh, const StepConsts& c, P* p, hipStrea
 
 // ---- the converging solve (atu_solve.hip) ----
 struct DevSolve {
-    const double *line_cp, *line_dn, *mode_cp, *mode_dn, *ct, *st, *ctT, *stT;
+    const double *line_cp, *line_dn, *line_inv_dn, *mode_cp, *mode_dn, *ct, *st, *ctT, *stT;
     int ni, N, NM, J;
     double S2, yw, ncell, e;
 };
 enum { DSUM_P = 0, DSUM_YSRC = 1, DSUM_SQ = 2, DSUM_RESID2 = 3 };
+// Device-side control of the converging solve. flags[CTL_DONE] set means converged: every kernel of
+// an iteration given that flag returns at once, so the host can queue several iterations and read
+// the outcome once. The residual norm's final sum updates the flags (see k_dsum_final).
+enum { CTL_DONE = 0, CTL_ITERS = 1, CTL_OUTER = 2, CTL_N = 4 };
+struct SolveCtl {
+    int* flags = nullptr;       // CTL_N ints, or null: no control
+    const double* bb = nullptr; // |b|^2
+    double tol = 0.0;
+    int iters = 0, outer = 0;   // added to the counters before the test (the work just done)
+};
 // P: storage of the unknown, B: of the right-hand side, R: arithmetic of the line pass
 template <typename P, typename B, typename R>
 hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, int colour,
-                       hipStream_t s);
+                       hipStream_t s, const int* done = nullptr);
 template <typename P>
-hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s);
+hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s, const int* done = nullptr);
 template <typename P>
 hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s);
 template <typename P, typename B>
 hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
-                       double* As, hipStream_t s);
+                       double* As, hipStream_t s, const int* done = nullptr);
 // canonical-order interior sum of p (DSUM_P), y*v (DSUM_YSRC), v^2 (DSUM_SQ) or (v - A p)^2
 // (DSUM_RESID2, which can also store the residual in FP32 into r_out); `rows` holds
 // (im-2)*(jm-2) doubles; the total lands in *out (device)
 template <typename P>
 hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, const double* v, double* rows,
-                       double* out, hipStream_t s, float* r_out = nullptr);
+                       double* out, hipStream_t s, float* r_out = nullptr, const SolveCtl& ctl = SolveCtl());
 hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, double yw, double* b, hipStream_t s);
-hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s);
+hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done = nullptr);
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
                             hipStream_t s);
+
+// ---- PressureSolverUran (atu_uran.hip), the reference's default solver ----
+struct UranTabs {
+    const double *a_inv_rm, *a_num2, *m_num2;                                  // [im]
+    const double *a_inv_rmsinthe, *a_num3, *a_denom, *m_num3, *m_inv_denom;    // [im][jm]
+    double a_num1, m_num1, two_dr, two_dthe, two_dphi, src_scale;
+};
+// One PressureSolverUran::run: the radial walls of aux, the defect-correction source into `incr`
+// (from rhs3 = rhs_u, rhs_v, rhs_w of the last stage and the current p), one lexicographic
+// Gauss-Seidel sweep of p, and p's walls and seam.
+hipError_t launch_uran_call(const Shape& sh, const UranTabs& t, double* aux, const double* rhs3, double* p,
+                            double* incr, hipStream_t s);
 
 // ---- the adjoint of the FP64 step (atu_adjoint.hip); 4-field blocks hold T, u, v, w ----
 struct AdjStageArgs {

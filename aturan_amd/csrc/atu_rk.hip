@@ -172,7 +172,10 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
                               + U.d1t * cirs + U.d2p * inv_rm2sinthe2;
         const R rhs_u = -dpdr_term - transport_u + diffusion_u / re + diffusion_u * nue_t - sponge * u0;
         fold<STAGE, CL_NONE>(a, ATU_U, c, Y0(ATU_U, u0), (double)rhs_u);
-        if (STAGE == 3) a.aux[c] = (S)(rhs_u + dpdr_term);
+        if (STAGE == 3) {
+            a.aux[c] = (S)(rhs_u + dpdr_term);
+            if (a.rhs3) a.rhs3[c] = (double)rhs_u;
+        }
     }
     const R cor_sc = (R)C.Coriolis * (R)C.scale_Cor;
     const double* ph = a.phydro;
@@ -188,7 +191,10 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         const R rhs_v = -dpdthe_term - dphdthe_term - transport_v + diffusion_v / re + diffusion_v * nue_t
                         - cor_sc * Coriolis_the;
         fold<STAGE, CL_NONE>(a, ATU_V, c, Y0(ATU_V, v0), (double)rhs_v);
-        if (STAGE == 3) a.aux[fs + c] = (S)(rhs_v + dpdthe_term);
+        if (STAGE == 3) {
+            a.aux[fs + c] = (S)(rhs_v + dpdthe_term);
+            if (a.rhs3) a.rhs3[fs + c] = (double)rhs_v;
+        }
     }
     // ---- zonal velocity w (:307-308, :344-348, :720, :780-788) ----
     {
@@ -202,7 +208,10 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         const R rhs_w = -dpdphi_term - dphdphi_term - transport_w + diffusion_w / re + diffusion_w * nue_t
                         - cor_sc * Coriolis_phi;
         fold<STAGE, CL_NONE>(a, ATU_W, c, Y0(ATU_W, w0), (double)rhs_w);
-        if (STAGE == 3) a.aux[2 * fs + c] = (S)(rhs_w + dpdphi_term);
+        if (STAGE == 3) {
+            a.aux[2 * fs + c] = (S)(rhs_w + dpdphi_term);
+            if (a.rhs3) a.rhs3[2 * fs + c] = (double)rhs_w;
+        }
     }
 
     // ---- species (:310-326, :350-379, :790-913) ----

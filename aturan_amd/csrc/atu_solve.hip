@@ -30,7 +30,9 @@ namespace {
 __device__ __forceinline__ double endw(int x, int lo, int hi) { return (x == lo || x == hi) ? 1.5 : 1.0; }
 
 template <typename P, typename B, typename R>
-__global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P* p, const B* b, int colour) {
+__global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P* p, const B* b, int colour,
+                                              const int* done) {
+    if (done && *done) return;
     const int N = sh.km - 1;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
     if (j > sh.jm - 2) return;
@@ -40,7 +42,10 @@ __global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P*
     const R c13 = R(1.0) / R(3.0);
     const int kw = (k == 0) ? N - 1 : k - 1, ke = (k == N - 1) ? 0 : k + 1;
     const double* cp = t.line_cp + (size_t)j * ni;
-    const double* dn = t.line_dn + (size_t)j * ni;
+    // FP64: divide by the Thomas pivots, as the oracle; FP32 (the refinement's correction):
+    // multiply by their reciprocals, off the division's latency
+    constexpr bool kDiv = sizeof(R) == sizeof(double);
+    const double* dn = (kDiv ? t.line_dn : t.line_inv_dn) + (size_t)j * ni;
     const size_t stride = sh.plane;
     const size_t x1 = ((size_t)jm + j) * ldk + k;   // the column's cell at i = 1
     const size_t r1 = x1 - k;                        // and its row
@@ -75,7 +80,7 @@ __global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P*
             const int s = s0 + u;
             if (s < ni) {
                 const R a = (s == 0) ? R(0.0) : ((s == ni - 1) ? num1 - num1 * c13 : num1);
-                const R dp = (rhs[u] - a * dprev) / dns[u];
+                const R dp = kDiv ? (rhs[u] - a * dprev) / dns[u] : (rhs[u] - a * dprev) * dns[u];
                 p[x1 + (size_t)s * stride] = (P)dp;
                 dprev = dp;
             }
@@ -107,7 +112,8 @@ __global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P*
 
 // radial walls (j = 1..jm-2) and polar walls (i = 1..im-2), k = 0..N-1
 template <typename P>
-__global__ void __launch_bounds__(256) k_walls(Shape sh, double c43, double c13, P* p) {
+__global__ void __launch_bounds__(256) k_walls(Shape sh, double c43, double c13, P* p, const int* done) {
+    if (done && *done) return;
     const int N = sh.km - 1;
     const int n1 = (sh.jm - 2) * N, n2 = (sh.im - 2) * N;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,7 +162,8 @@ __device__ __forceinline__ double resid_at(const Shape& sh, const DevGeo& g, con
 // R(jj, k) = sum_i a_i r_i / S2 over each interior column
 template <typename P, typename B>
 __global__ void __launch_bounds__(256) k_col_resid(Shape sh, DevGeo g, DevSolve t, const P* p, const B* b,
-                                                   double* R) {
+                                                   double* R, const int* done) {
+    if (done && *done) return;
     const int N = sh.km - 1;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -169,7 +176,9 @@ __global__ void __launch_bounds__(256) k_col_resid(Shape sh, DevGeo g, DevSolve 
 // forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin. One
 // 64-lane block per (64 modes, jj); the k sum runs in order, its loads unrolled so that many are
 // in flight at once.
-__global__ void __launch_bounds__(64) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As) {
+__global__ void __launch_bounds__(64) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As,
+                                                const int* done) {
+    if (done && *done) return;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int jj = blockIdx.y;
     if (m > t.NM) return;
@@ -190,7 +199,8 @@ __global__ void __launch_bounds__(64) k_dft_fwd(DevSolve t, const double* R, dou
 // (mode, part): the right-hand side and the mode's factors are staged in LDS with coalesced loads,
 // one lane runs the two sequential sweeps out of LDS, and the block writes the result back.
 constexpr int kBaroMaxJ = 1024;
-__global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double* Ac, double* As) {
+__global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double* Ac, double* As, const int* done) {
+    if (done && *done) return;
     const int m = blockIdx.x >> 1, part = blockIdx.x & 1;
     const int J = t.J;
     double* d = (part == 0 ? Ac : As) + (size_t)m * J;
@@ -231,7 +241,9 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
 
 // inverse DFT of the correction and its addition to every interior level of the column
 template <typename P>
-__global__ void __launch_bounds__(256) k_dft_inv_add(Shape sh, DevSolve t, const double* Ac, const double* As, P* p) {
+__global__ void __launch_bounds__(256) k_dft_inv_add(Shape sh, DevSolve t, const double* Ac, const double* As, P* p,
+                                                     const int* done) {
+    if (done && *done) return;
     const int N = t.N, NM = t.NM, J = t.J;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int jj = blockIdx.y * blockDim.y + threadIdx.y;
@@ -251,7 +263,8 @@ __global__ void __launch_bounds__(256) k_dft_inv_add(Shape sh, DevSolve t, const
 // ---- canonical interior sums: per row, 64-wide trees, chunks, rows and levels in sequence ----
 template <int MODE, typename P>
 __global__ void __launch_bounds__(1024) k_dsum_rows(Shape sh, DevGeo g, const P* p, const double* v, double* rows,
-                                                    float* r_out) {
+                                                    float* r_out, const int* done) {
+    if (done && *done) return;
     const int N = sh.km - 1;
     const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
     const int k = threadIdx.x;
@@ -281,7 +294,11 @@ __global__ void __launch_bounds__(1024) k_dsum_rows(Shape sh, DevGeo g, const P*
 // The level sums (rows of a level in order) and their total (levels in order). Each wave takes
 // levels wave, wave + 16, ...: it stages 64 row sums at a time in LDS with one coalesced load and
 // adds them in order; every wave runs the same number of rounds so the block barriers are uniform.
-__global__ void __launch_bounds__(1024) k_dsum_final(int ni, int nj, const double* rows, double* out) {
+// With a control block (the residual norm of the converging solve) it also decides convergence on
+// the device: the counters take the work just done, then rel = |r| / |b| (|b|^2 in *bb) <= tol
+// sets done.
+__global__ void __launch_bounds__(1024) k_dsum_final(int ni, int nj, const double* rows, double* out, SolveCtl ctl) {
+    if (ctl.flags && ctl.flags[CTL_DONE]) return;
     __shared__ double planes[1024];
     __shared__ double buf[16][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -305,6 +322,14 @@ __global__ void __launch_bounds__(1024) k_dsum_final(int ni, int nj, const doubl
         double total = 0.0;
         for (int i = 0; i < ni; i++) total += planes[i];
         *out = total;
+        if (ctl.flags) {
+            int* f = ctl.flags;
+            f[CTL_ITERS] += ctl.iters;
+            f[CTL_OUTER] += ctl.outer;
+            const double b2 = *ctl.bb;
+            const double rel = b2 > 0.0 ? sqrt(total) / sqrt(b2) : sqrt(total);
+            if (rel <= ctl.tol) f[CTL_DONE] = 1;
+        }
     }
 }
 
@@ -332,7 +357,8 @@ __global__ void __launch_bounds__(256) k_shift_all(Shape sh, const double* sc, i
 }
 
 // refinement: p += e on the interior (FP64 accumulation of the FP32 correction)
-__global__ void __launch_bounds__(256) k_add_correction(Shape sh, const float* e, double* p) {
+__global__ void __launch_bounds__(256) k_add_correction(Shape sh, const float* e, double* p, const int* done) {
+    if (done && *done) return;
     const int N = sh.km - 1;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -347,16 +373,16 @@ __global__ void __launch_bounds__(256) k_add_correction(Shape sh, const float* e
 // ---- launchers ----
 template <typename P, typename B, typename R>
 hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, int colour,
-                       hipStream_t s) {
+                       hipStream_t s, const int* done) {
     const int half = (sh.km - 1 + 1) / 2;
-    hipLaunchKernelGGL((k_line<P, B, R>), dim3((half + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, p, b, colour);
+    hipLaunchKernelGGL((k_line<P, B, R>), dim3((half + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, p, b, colour, done);
     return hipGetLastError();
 }
 
 template <typename P>
-hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s) {
+hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s, const int* done) {
     const int n = (sh.jm - 2 + sh.im - 2) * (sh.km - 1);
-    hipLaunchKernelGGL((k_walls<P>), dim3((n + 255) / 256), dim3(256), 0, s, sh, 4.0 / 3.0, 1.0 / 3.0, p);
+    hipLaunchKernelGGL((k_walls<P>), dim3((n + 255) / 256), dim3(256), 0, s, sh, 4.0 / 3.0, 1.0 / 3.0, p, done);
     return hipGetLastError();
 }
 
@@ -368,40 +394,41 @@ hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s) {
 
 template <typename P, typename B>
 hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
-                       double* As, hipStream_t s) {
+                       double* As, hipStream_t s, const int* done) {
     if (t.J > kBaroMaxJ) return hipErrorInvalidValue;   // the theta solves stage a mode in LDS
     const int N = sh.km - 1;
     hipLaunchKernelGGL((k_col_resid<P, B>), dim3((N + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, (const P*)p, b,
-                       R);
-    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, t.J), dim3(64), 0, s, t, (const double*)R, Ac, As);
-    hipLaunchKernelGGL(k_baro_tri, dim3(2 * (t.NM + 1)), dim3(64), 0, s, t, 1.0 / 3.0, Ac, As);
+                       R, done);
+    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, t.J), dim3(64), 0, s, t, (const double*)R, Ac, As, done);
+    hipLaunchKernelGGL(k_baro_tri, dim3(2 * (t.NM + 1)), dim3(64), 0, s, t, 1.0 / 3.0, Ac, As, done);
     hipLaunchKernelGGL((k_dft_inv_add<P>), dim3((N + 63) / 64, t.J), dim3(64), 0, s, sh, t, (const double*)Ac,
-                       (const double*)As, p);
+                       (const double*)As, p, done);
     return hipGetLastError();
 }
 
 template <typename P>
 hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, const double* v, double* rows,
-                       double* out, hipStream_t s, float* r_out) {
+                       double* out, hipStream_t s, float* r_out, const SolveCtl& ctl) {
+    const int* done = ctl.flags ? ctl.flags + CTL_DONE : nullptr;
     const int N = sh.km - 1;
     const int threads = ((N + 63) / 64) * 64;
     const dim3 grid(sh.jm - 2, sh.im - 2);
     switch (mode) {
         case DSUM_P:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_P, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_P, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr, done);
             break;
         case DSUM_YSRC:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_YSRC, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_YSRC, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr, done);
             break;
         case DSUM_SQ:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_SQ, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_SQ, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr, done);
             break;
         default:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_RESID2, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, r_out);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_RESID2, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, r_out, done);
             break;
     }
     if (sh.im - 2 > 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(1024), 0, s, sh.im - 2, sh.jm - 2, (const double*)rows, out);
+    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(1024), 0, s, sh.im - 2, sh.jm - 2, (const double*)rows, out, ctl);
     return hipGetLastError();
 }
 
@@ -411,9 +438,9 @@ hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, d
     return hipGetLastError();
 }
 
-hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s) {
+hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done) {
     hipLaunchKernelGGL(k_add_correction, dim3((sh.km - 1 + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0,
-                       s, sh, e, p);
+                       s, sh, e, p, done);
     return hipGetLastError();
 }
 
@@ -426,19 +453,19 @@ hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, 
 }
 
 template hipError_t launch_line<double, double, double>(const Shape&, const DevGeo&, const DevSolve&, double*,
-                                                        const double*, int, hipStream_t);
+                                                        const double*, int, hipStream_t, const int*);
 template hipError_t launch_line<float, float, float>(const Shape&, const DevGeo&, const DevSolve&, float*,
-                                                     const float*, int, hipStream_t);
+                                                     const float*, int, hipStream_t, const int*);
 template hipError_t launch_baro<double, double>(const Shape&, const DevGeo&, const DevSolve&, double*, const double*,
-                                                double*, double*, double*, hipStream_t);
+                                                double*, double*, double*, hipStream_t, const int*);
 template hipError_t launch_baro<float, float>(const Shape&, const DevGeo&, const DevSolve&, float*, const float*,
-                                              double*, double*, double*, hipStream_t);
-template hipError_t launch_walls<double>(const Shape&, double*, hipStream_t);
-template hipError_t launch_walls<float>(const Shape&, float*, hipStream_t);
+                                              double*, double*, double*, hipStream_t, const int*);
+template hipError_t launch_walls<double>(const Shape&, double*, hipStream_t, const int*);
+template hipError_t launch_walls<float>(const Shape&, float*, hipStream_t, const int*);
 template hipError_t launch_seam<double>(const Shape&, double*, hipStream_t);
 template hipError_t launch_seam<float>(const Shape&, float*, hipStream_t);
 template hipError_t launch_dsum<double>(int, const Shape&, const DevGeo&, const double*, const double*, double*,
-                                        double*, hipStream_t, float*);
+                                        double*, hipStream_t, float*, const SolveCtl&);
 template hipError_t launch_shift_all<double>(const Shape&, const double*, int, int, double, double*, hipStream_t);
 
 }  // namespace atu

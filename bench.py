@@ -42,13 +42,18 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tol", type=float, default=1e-8, help="relative residual of the Poisson solve")
-    ap.add_argument("--inner", type=int, default=3, help="FP32 smoothing iterations per refinement (mixed)")
+    ap.add_argument("--inner", type=int, default=2, help="FP32 smoothing iterations per refinement (mixed)")
     ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true", help="skip the 1/10/100-step mixed-vs-FP64 comparison")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="window of the adjoint timing (0: skip)")
     ap.add_argument("--checkpoint-every", type=int, default=10)
+    ap.add_argument("--refined", action="store_true",
+                    help="the 81x361x721 workload instead: mixed vs FP64 over --refined-steps steps and the "
+                         "mixed vs FP64 adjoint over --refined-window steps")
+    ap.add_argument("--refined-steps", type=int, default=200)
+    ap.add_argument("--refined-window", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of known size (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -77,13 +82,16 @@ def traffic_from_profiles():
     return None
 
 
-def initial_state():
+def initial_state(refine=1):
+    """The seeded synthetic state with the domain BC applied once (by the library, on the GPU)."""
     import aturan_amd as A
-    from oracle import binding as O   # the oracle's BC of the seeded state: identical to the GPU's (tested)
-    g = A.default_grid(1)
+    g = A.default_grid(refine)
     p = A.default_params(g)
     F = A.synthetic_state(g, p, 20261203)
-    O.bc(g, p, F)
+    with A.UranusCore(g, p) as core:
+        core.set_state(F)
+        core.boundary_conditions()
+        F = core.get_state()
     return g, p, F
 
 
@@ -166,10 +174,72 @@ def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s):
                       f"{g.im}x{g.jm}x384 FP64"}
 
 
+def refined_bench(args):
+    """81x361x721 (dr/2, 0.5 deg): mixed vs FP64 forward over `refined_steps` steps, and the window
+    adjoint (checkpoint every 10) in mixed vs FP64 over `refined_window` steps."""
+    import numpy as np
+    import aturan_amd as A
+    g, p, F = initial_state(2)
+    n_int = (g.im - 2) * (g.jm - 2) * (g.km - 1)
+    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60)
+    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner)
+    out, fields = {}, {}
+    for prec, opts in (("fp64", opts64), ("mixed", optsmx)):
+        with A.UranusCore(g, p, prec) as core:
+            core.set_state(F)
+            core.step(2, **opts)
+            core.set_state(F)
+            core.synchronize()
+            t0 = time.perf_counter()
+            st = core.step(args.refined_steps, **opts)
+            core.synchronize()
+            el = time.perf_counter() - t0
+            fields[prec] = {f: core.get_field(f) for f in set(ERR_FIELDS.values())}
+        out[prec] = {"steps_per_s": round(args.refined_steps / el, 3), "ms_per_step": round(1e3 * el / args.refined_steps, 3),
+                     "cell_updates_per_s": round(n_int * args.refined_steps / el, 1),
+                     "solve_iterations_last_step": st.sweeps_done, "solve_rel_resid_last_step": st.rel_resid}
+    out["mixed"]["speedup_vs_fp64"] = round(out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"], 4)
+    out["mixed"]["rel_error_vs_fp64"] = rel_errors(fields["mixed"], fields["fp64"])
+    del fields
+    grads = {}
+    for prec in ("fp64", "mixed"):
+        with A.UranusCore(g, p, prec) as core:
+            core.set_state(F)
+            core.synchronize()
+            t0 = time.perf_counter()
+            loss = core.window_forward(args.refined_window, checkpoint_every=10, rel_tol=1e-10)
+            core.synchronize()
+            t1 = time.perf_counter()
+            gt, gn = core.window_backward(1.0)
+            core.synchronize()
+            t2 = time.perf_counter()
+        grads[prec] = (gt[1:-1, 1:-1, :-1], gn)
+        out[prec]["adjoint"] = {"window_steps": args.refined_window, "checkpoint_every": 10, "loss": loss,
+                                "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
+                                "backward_over_forward": round((t2 - t1) / (t1 - t0), 3), "grad_nue": gn}
+    a, b = grads["mixed"][0], grads["fp64"][0]
+    out["mixed"]["adjoint"]["grad_t0_rel_error_vs_fp64"] = {
+        "max": float(np.max(np.abs(a - b)) / np.max(np.abs(b))),
+        "l2": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
+    out["mixed"]["adjoint"]["grad_nue_rel_error_vs_fp64"] = abs(grads["mixed"][1] - grads["fp64"][1]) / abs(grads["fp64"][1])
+    line = {"metric": "RK4 steps/s at 81x361x721 (mixed FP32/FP64 GPU step, Poisson to 1e-8)",
+            "value": out["mixed"]["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.refined_steps,
+            "warmup": 2, "ms_per_step": out["mixed"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+            "config": {"workload": f"81x361x721 shell (dr 0.0125, 0.5 deg), seeded synthetic state: "
+                                   f"{args.refined_steps} steps, Poisson to {args.tol:g}; window adjoint over "
+                                   f"{args.refined_window} steps, checkpoint every 10"},
+            "fp64": out["fp64"], "mixed": out["mixed"]}
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(tol, budget_s):
     import aturan_amd as A
     from oracle import binding as O
-    g, p, F = initial_state()
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F = A.synthetic_state(g, p, 20261203)
+    O.bc(g, p, F)
 
     def one():
         O.hydro(g, p, F)
@@ -203,6 +273,9 @@ def main():
     import torch
     import aturan_amd as A
 
+    if args.refined:
+        refined_bench(args)
+        return
     g, p, F = initial_state()
     opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60)
     optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner)

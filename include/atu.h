@@ -119,7 +119,7 @@ typedef enum atu_solve_mode {
     ATU_SOLVE_REFERENCE = 0, /* `sweeps` red-black sweeps with the wall BCs inside each sweep,
                                 exactly planet/PressureSolver.h:318-757 (omega = 1 is the
                                 reference's Gauss-Seidel; other omega is SOR)                 */
-    ATU_SOLVE_CONVERGE = 1   /* the same system with every wall folded in, iterated to
+    ATU_SOLVE_CONVERGE = 1,  /* the same system with every wall folded in, iterated to
                                 ||b - A p||_2 / ||b||_2 <= rel_tol, where b is the source less
                                 its component along the all-Neumann left null vector; zebra
                                 red-black radial line relaxation (the reference's PRESS_LINE)
@@ -127,12 +127,20 @@ typedef enum atu_solve_mode {
                                 iteration; the interior mean of p is held. On a mixed context:
                                 FP64 residual and accumulation, FP32 smoothing (iterative
                                 refinement)                                                    */
+    ATU_SOLVE_URAN = 2       /* PressureSolverUran::run (planet/PressureSolverUran.h:33-250), the
+                                reference's default solver, `sweeps` calls: the radial walls of
+                                aux, the defect-correction source src_scale div(rhs) + Lc(p_old)
+                                from the last stage's rhs_u, rhs_v, rhs_w (kept by the context
+                                when atu_keep_stage_rhs is on; atu_step turns it on for this
+                                mode), one serial lexicographic Gauss-Seidel sweep (run as
+                                hyperplanes i+j+k = const, which reproduces it exactly), and the
+                                walls of p. FP64 contexts only.                                */
 } atu_solve_mode;
 
 typedef struct atu_solver_opts {
     int32_t mode;            /* atu_solve_mode                                                */
     int32_t sweeps;          /* REFERENCE: sweeps to run. CONVERGE on a mixed context: FP32
-                                smoothing iterations per FP64 refinement (<= 0: 4)            */
+                                smoothing iterations per FP64 refinement (<= 0: 2)            */
     int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing iterations                 */
     int32_t pad_;
     double omega;            /* relaxation factor (1 = Gauss-Seidel)                          */
@@ -215,6 +223,11 @@ int atu_apply_bc(atu_ctx* ctx);
  * of PressureSolver<Planet>::run (planet/PressureSolver.h:145-766) for p_dyn. */
 int atu_pressure_solve(atu_ctx* ctx, const atu_solver_opts* opts, atu_solve_stats* stats);
 
+/* Keep (on != 0) or stop keeping rhs_u, rhs_v, rhs_w of the last RK stage in the context: the
+ * reference's rhs_* arrays that its default solver reads (planet/cUranusModel.h:189-208). Off by
+ * default; the stage kernel then does not write them. */
+int atu_keep_stage_rhs(atu_ctx* ctx, int on);
+
 /* One outer step as the benchmark loop orders it: hydro prep, RK4, BC, Poisson, `nsteps` times.
  * `stats` (optional) receives the last solve's statistics. */
 int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_stats* stats);
@@ -243,7 +256,9 @@ const char* atu_build_info(void);
  * temperature field and nue the eddy viscosity field (the derivative for one uniform change of
  * every cell of nue). The solve is differentiated as exact; both the forward solves of the
  * window and the adjoint solves run `solve_iters` iterations when it is > 0, else to rel_tol.
- * The state after either call is the window's final state. Needs rigid_lid = 1. */
+ * The state after either call is the window's final state. Needs rigid_lid = 1. On a mixed
+ * context the window's steps (forward and recompute) are the mixed step and the adjoint solves the
+ * mixed refinement; the stage states are taped widened to FP64 and the adjoint kernels are FP64. */
 typedef struct atu_window_opts {
     int32_t nsteps;
     int32_t checkpoint_every;   /* <= 0: 10                                                  */

@@ -37,6 +37,8 @@ def load() -> C.CDLL:
         "orc_cp_lapse": (C.c_double, [V, C.c_double]),
         "orc_rhs": (C.c_int, [V, V, _DP, _DP]),
         "orc_rk4": (C.c_int, [V, V, _DP]),
+        "orc_rk4_rhs": (C.c_int, [V, V, _DP, _DP]),
+        "orc_pressure_uran": (C.c_int, [V, V, _DP, _DP, C.c_int]),
         "orc_hydro": (C.c_int, [V, V, _DP, _DP]),
         "orc_bc": (C.c_int, [V, V, _DP]),
         "orc_pressure": (C.c_int, [V, V, _DP, C.c_int, C.c_double, _DP]),
@@ -87,6 +89,19 @@ def rhs(grid, params, fields: np.ndarray) -> np.ndarray:
 
 def rk4(grid, params, fields: np.ndarray) -> None:
     _ok(load().orc_rk4(_p(grid), _p(params), _d(fields)), "orc_rk4")
+
+
+def rk4_rhs(grid, params, fields: np.ndarray) -> np.ndarray:
+    """RK4 as rk4(); returns rhs_u, rhs_v, rhs_w of the last stage (3 dense fields)."""
+    out = np.zeros((3,) + fields.shape[1:])
+    _ok(load().orc_rk4_rhs(_p(grid), _p(params), _d(fields), _d(out)), "orc_rk4_rhs")
+    return out
+
+
+def pressure_uran(grid, params, fields: np.ndarray, rhs3: np.ndarray, calls: int = 1) -> None:
+    """PressureSolverUran::run `calls` times (the reference's default solver)."""
+    _ok(load().orc_pressure_uran(_p(grid), _p(params), _d(fields), _d(np.ascontiguousarray(rhs3)), calls),
+        "orc_pressure_uran")
 
 
 def hydro(grid, params, fields: np.ndarray) -> np.ndarray:

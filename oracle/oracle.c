@@ -354,9 +354,10 @@ int orc_rhs(const atu_grid* g, const atu_params* p, double* fields, double* out)
 }
 
 /* ------------------------------------------------------------------------------------------- */
-/* RK4 (planet/RungeKutta_Uran_Turb.cpp:179-404); aux_* of the last stage land in fields.       */
+/* RK4 (planet/RungeKutta_Uran_Turb.cpp:179-404); aux_* of the last stage land in fields, and, when */
+/* rhs3 is given, rhs_u, rhs_v, rhs_w of the last stage (3 dense fields, 0 off the interior).   */
 /* ------------------------------------------------------------------------------------------- */
-int orc_rk4(const atu_grid* g, const atu_params* p, double* fields) {
+int orc_rk4_rhs(const atu_grid* g, const atu_params* p, double* fields, double* rhs3) {
     orc_geo G;
     geo_init(&G, g, p);
     const int im = g->im, jm = g->jm, km = g->km;
@@ -422,10 +423,15 @@ int orc_rk4(const atu_grid* g, const atu_params* p, double* fields) {
             }
     for (int q = 0; q < 3; q++)
         memcpy(fields + (size_t)(ATU_AUX_U + q) * n3, rhs + (size_t)(ATU_NPROG + q) * n3, sizeof(double) * n3);
+    if (rhs3)
+        for (int q = 0; q < 3; q++)
+            memcpy(rhs3 + (size_t)q * n3, rhs + (size_t)(ATU_U + q) * n3, sizeof(double) * n3);
     free(yn); free(acc); free(rhs);
     geo_free(&G);
     return 0;
 }
+
+int orc_rk4(const atu_grid* g, const atu_params* p, double* fields) { return orc_rk4_rhs(g, p, fields, NULL); }
 
 /* ------------------------------------------------------------------------------------------- */
 /* hydrostatic prep (planet/RungeKutta_Uran_Turb.cpp:18-177)                                    */
@@ -1022,6 +1028,112 @@ int orc_pressure_converge(const atu_grid* g, const atu_params* p, double* fields
     if (!src_out) free(src);
     baro_free(&B);
     pgeo_free(&q);
+    geo_free(&G);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------- */
+/* PressureSolverUran::run (planet/PressureSolverUran.h:33-250), the reference's default solver,   */
+/* at its defaults: periodic phi (k = 0..km-2 relaxed, km-1 the copy), PRESS_CONSISTENT (metric    */
+/* radius, src_scale = r_mix u0^2 / 1e5) and PRESS_INCR (the defect-correction source              */
+/* src_scale div(rhs) + Lc(p_old) over the last stage's rhs_*). One call: the radial walls of aux, */
+/* the source, one serial lexicographic Gauss-Seidel sweep, the walls of p. `calls` calls in a row. */
+/* ------------------------------------------------------------------------------------------- */
+int orc_pressure_uran(const atu_grid* g, const atu_params* p, double* fields, const double* rhs3, int calls) {
+    orc_geo G;
+    geo_init(&G, g, p);
+    const int im = g->im, jm = g->jm, km = g->km, n = km - 1;
+    const size_t n3 = (size_t)im * jm * km;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    double* au = fields + (size_t)ATU_AUX_U * n3;
+    double* av = fields + (size_t)ATU_AUX_V * n3;
+    double* aw = fields + (size_t)ATU_AUX_W * n3;
+    double* pd = fields + (size_t)ATU_P_DYN * n3;
+    const double* ru_ = rhs3;
+    const double* rv_ = rhs3 + n3;
+    const double* rw_ = rhs3 + 2 * n3;
+    double* sinthe = malloc(sizeof(double) * jm);
+    double* incr = malloc(sizeof(double) * n3);
+    for (int j = 0; j < jm; j++) {
+        sinthe[j] = sin(G.thez[j]);
+        if (sinthe[j] < p->sinthe_min) sinthe[j] = p->sinthe_min;
+    }
+    const double src_scale = (p->r_mix * p->u_0 * p->u_0) / 1.0e5;
+    const double dr = g->dr, dthe = g->dthe, dphi = g->dphi;
+    const double inv_2dr = 1.0 / (2.0 * dr), inv_2dthe = 1.0 / (2.0 * dthe), inv_2dphi = 1.0 / (2.0 * dphi);
+    const double inv_dr2 = 1.0 / (dr * dr), inv_dthe2 = 1.0 / (dthe * dthe), inv_dphi2 = 1.0 / (dphi * dphi);
+    (void)inv_2dr; (void)inv_2dthe; (void)inv_2dphi;
+    for (int call = 0; call < calls; call++) {
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < km - 1; k++) {
+                double* A[3] = {au, av, aw};
+                for (int q = 0; q < 3; q++) {
+                    A[q][IDX(0, j, k)] = c43 * A[q][IDX(1, j, k)] - c13 * A[q][IDX(2, j, k)];
+                    A[q][IDX(im - 1, j, k)] = c43 * A[q][IDX(im - 2, j, k)] - c13 * A[q][IDX(im - 3, j, k)];
+                }
+            }
+        /* the defect-correction source, from the p_dyn before this call's sweep */
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int i = 1; i < im - 1; i++)
+            for (int j = 1; j < jm - 1; j++) {
+                const double rm = metric_radius(p, G.radz[i]);
+                const double inv_rm = 1.0 / rm;
+                const double inv_rmsinthe = 1.0 / (rm * sinthe[j]);
+                const double inv_rm2 = inv_rm * inv_rm;
+                const double inv_rm2sinthe2 = inv_rm2 / (sinthe[j] * sinthe[j]);
+                const double num1 = 1.0 / (dr * dr), num2 = inv_rm2 / (dthe * dthe),
+                             num3 = inv_rm2sinthe2 / (dphi * dphi);
+                const double denom = 2.0 * (num1 + num2 + num3);
+                for (int k = 0; k < n; k++) {
+                    const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+                    const double up = (i + 1 == im - 1) ? c43 * ru_[IDX(im - 2, j, k)] - c13 * ru_[IDX(im - 3, j, k)]
+                                                        : ru_[IDX(i + 1, j, k)];
+                    const double um = (i - 1 == 0) ? c43 * ru_[IDX(1, j, k)] - c13 * ru_[IDX(2, j, k)]
+                                                   : ru_[IDX(i - 1, j, k)];
+                    const double div_rhs = (up - um) / (2.0 * dr)
+                                           + (rv_[IDX(i, j + 1, k)] - rv_[IDX(i, j - 1, k)]) / (2.0 * dthe) * inv_rm
+                                           + (rw_[IDX(i, j, ke)] - rw_[IDX(i, j, kw)]) / (2.0 * dphi) * inv_rmsinthe;
+                    const double lc_old = (pd[IDX(i + 1, j, k)] + pd[IDX(i - 1, j, k)]) * num1
+                                          + (pd[IDX(i, j + 1, k)] + pd[IDX(i, j - 1, k)]) * num2
+                                          + (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)]) * num3
+                                          - denom * pd[IDX(i, j, k)];
+                    incr[IDX(i, j, k)] = src_scale * div_rhs + lc_old;
+                }
+            }
+        /* one lexicographic Gauss-Seidel sweep, in place */
+        for (int i = 1; i < im - 1; i++)
+            for (int j = 1; j < jm - 1; j++) {
+                const double rm = metric_radius(p, G.radz[i]);
+                const double rm2 = rm * rm;
+                const double s = sinthe[j];
+                const double inv_rm2 = 1.0 / rm2;
+                const double inv_rm2sinthe2 = inv_rm2 / (s * s);
+                const double denom = 2.0 * inv_dr2 + 2.0 * inv_rm2 * inv_dthe2 + 2.0 * inv_rm2sinthe2 * inv_dphi2;
+                const double inv_denom = 1.0 / denom;
+                const double num1 = inv_dr2, num2 = inv_rm2 * inv_dthe2, num3 = inv_rm2sinthe2 * inv_dphi2;
+                for (int k = 0; k < n; k++) {
+                    const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+                    pd[IDX(i, j, k)] = ((pd[IDX(i + 1, j, k)] + pd[IDX(i - 1, j, k)]) * num1
+                                        + (pd[IDX(i, j + 1, k)] + pd[IDX(i, j - 1, k)]) * num2
+                                        + (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)]) * num3
+                                        - incr[IDX(i, j, k)]) * inv_denom;
+                }
+            }
+        for (int k = 0; k < km; k++)
+            for (int j = 0; j < jm; j++) {
+                pd[IDX(0, j, k)] = c43 * pd[IDX(1, j, k)] - c13 * pd[IDX(2, j, k)];
+                pd[IDX(im - 1, j, k)] = c43 * pd[IDX(im - 2, j, k)] - c13 * pd[IDX(im - 3, j, k)];
+            }
+        for (int k = 0; k < km; k++)
+            for (int i = 0; i < im; i++) {
+                pd[IDX(i, 0, k)] = c43 * pd[IDX(i, 1, k)] - c13 * pd[IDX(i, 2, k)];
+                pd[IDX(i, jm - 1, k)] = c43 * pd[IDX(i, jm - 2, k)] - c13 * pd[IDX(i, jm - 3, k)];
+            }
+        for (int i = 0; i < im; i++)
+            for (int j = 0; j < jm; j++) pd[IDX(i, j, km - 1)] = pd[IDX(i, j, 0)];
+    }
+    free(sinthe);
+    free(incr);
     geo_free(&G);
     return 0;
 }

@@ -8,6 +8,8 @@
 //   mode rk4      restoreVar(1) + RungeKuttaUran (hydro prep + 4 stages) (planet/RungeKutta_Uran_Turb.cpp)
 //   mode pressure PressureSolver<cUranusModel>::run with ATURAN_PRESS_SWEEPS=sweeps
 //   mode step     `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolver)
+//   mode rk4_uran restoreVar(1) + RungeKuttaUran + PressureSolverUran::run (the default solver)
+//   mode step_uran `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolverUran)
 // in/out: int32 im, jm, km, nfields, then nfields dense [im][jm][km] FP64 arrays in the atu_field
 // order of include/atu.h; out appends im doubles of buoy_ref_level.
 
@@ -42,6 +44,7 @@
 #include "BC_Uran.h"
 #include "ChemistryUran.h"
 #include "PressureSolver.h"
+#include "PressureSolverUran.h"
 #undef private
 #undef protected
 
@@ -136,6 +139,17 @@ int main(int argc, char** argv) {
             m.RungeKuttaUran();
             bc();
             PressureSolver<cUranusModel>(m).run();
+        }
+    } else if (mode == "rk4_uran") {
+        m.restoreVar(1.0);
+        m.RungeKuttaUran();
+        PressureSolverUran(m).run();
+    } else if (mode == "step_uran") {
+        for (int s = 0; s < steps; s++) {
+            m.restoreVar(1.0);
+            m.RungeKuttaUran();
+            bc();
+            PressureSolverUran(m).run();
         }
     } else {
         std::fprintf(stderr, "unknown mode %s\n", mode.c_str());

@@ -40,6 +40,8 @@ CASES = {
     "pressure_1": ("pressure", 1, 1, True),
     "pressure_20": ("pressure", 1, 20, True),
     "step_2x3": ("step", 2, 3, True),
+    "rk4_uran": ("rk4_uran", 1, 1, True),
+    "step_uran_2": ("step_uran", 2, 1, True),
 }
 
 
@@ -116,7 +118,9 @@ def main() -> None:
     p = A.default_params(g)
     shape = (g.im, g.jm, g.km)
     with tempfile.TemporaryDirectory() as td:
-        for name, (mode, steps, sweeps, _) in CASES.items():
+        names = sys.argv[1:] or list(CASES)
+        for name in names:
+            mode, steps, sweeps, _ = CASES[name]
             F = case_input(name, g, p)
             fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
             write_bin(fin, F)

@@ -1,6 +1,7 @@
 """Pins the CPU oracle to golden vectors written by the compiled reference (tests/golden/ref_*.npz,
 from tests/golden/make_golden.py): the domain BC, one RungeKuttaUran (hydro prep + 4 stages), the
-red-black PressureSolver at 1 and 20 sweeps, and two full steps, on an input in which every term of
+red-black PressureSolver at 1 and 20 sweeps, two full steps, and the default PressureSolverUran after
+one RungeKuttaUran and in two full steps, on an input in which every term of
 the path is non-zero. Per field: the sampled cells within 1e-12 of the field's largest sample, and
 the exact sum and sum of squares within 1e-12 relative."""
 import math
@@ -30,6 +31,16 @@ def run_oracle(oracle, name, g, p):
         oracle.rk4(g, p, F)
     elif mode == "pressure":
         oracle.pressure(g, p, F, sweeps, 1.0)
+    elif mode == "rk4_uran":
+        level = oracle.hydro(g, p, F)
+        rhs3 = oracle.rk4_rhs(g, p, F)
+        oracle.pressure_uran(g, p, F, rhs3, 1)
+    elif mode == "step_uran":
+        for _ in range(steps):
+            level = oracle.hydro(g, p, F)
+            rhs3 = oracle.rk4_rhs(g, p, F)
+            oracle.bc(g, p, F)
+            oracle.pressure_uran(g, p, F, rhs3, 1)
     else:
         for _ in range(steps):
             level = oracle.hydro(g, p, F)
@@ -58,7 +69,7 @@ def test_oracle_matches_compiled_reference(oracle, name):
     bad = {k: v for k, v in worst.items() if v > TOL}
     print(f"[golden] {name}: worst sampled difference {max(worst.values()):.3e}")
     assert not bad, bad
-    if MG.CASES[name][0] in ("rk4", "step"):
+    if MG.CASES[name][0] in ("rk4", "step", "rk4_uran", "step_uran"):
         assert np.max(np.abs(got["level"] - ref["level"])) <= TOL * np.max(np.abs(ref["level"]))
 
 

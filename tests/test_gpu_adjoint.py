@@ -95,6 +95,26 @@ def test_adjoint_backward_is_repeatable_and_linear(oracle):
     assert abs(n3 + 2.5 * n1) <= 1e-9 * abs(n1)
 
 
+def test_mixed_window_adjoint_tracks_fp64(oracle):
+    """The window on a mixed context (FP32 stencils and stage states, refined solves) against FP64."""
+    print("[case] mixed vs FP64 window adjoint, 12x19x37 shell, 6 steps", flush=True)
+    g = A.default_grid(1)
+    g.im, g.jm, g.km = 12, 19, 37
+    p = A.default_params(g)
+    F = _state(g, p, oracle)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with A.UranusCore(g, p, prec) as core:
+            core.set_state(F)
+            loss = core.window_forward(nsteps=6, checkpoint_every=4, rel_tol=1e-10)
+            res[prec] = (loss,) + core.window_backward(1.0)
+    (l64, g64, n64), (lmx, gmx, nmx) = res["fp64"], res["mixed"]
+    eg = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
+    en = abs(nmx - n64) / abs(n64)
+    print(f"[adjoint] mixed vs fp64: loss {abs(lmx - l64) / l64:.2e}, grad T0 {eg:.2e}, grad nue {en:.2e}", flush=True)
+    assert abs(lmx - l64) <= 1e-6 * l64 and eg <= 1e-4 and en <= 1e-4
+
+
 @pytest.mark.slow
 def test_adjoint_full_shell_50_steps(oracle):
     print("[case] adjoint vs FD, 41x181x361, 50 steps, checkpoint every 10, 16 directions", flush=True)

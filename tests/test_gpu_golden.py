@@ -1,6 +1,7 @@
 """The HIP path against the compiled reference's golden vectors (tests/golden/ref_*.npz) on the
 enriched input in which every term of the step is non-zero: domain BC, one RungeKuttaUran (hydro
-prep + 4 stages), PressureSolver at 1 and 20 sweeps, and two full steps. FP64 bar: 1e-12 of each
+prep + 4 stages), PressureSolver at 1 and 20 sweeps, two full steps, and the default
+PressureSolverUran after one RungeKuttaUran and in two full steps. FP64 bar: 1e-12 of each
 field's largest sampled magnitude, and 1e-12 relative on the exact field sums."""
 import sys
 from pathlib import Path
@@ -36,6 +37,13 @@ def test_gpu_matches_compiled_reference(oracle, name):
             core.RungeKuttaUran()
         elif mode == "pressure":
             core.PressureSolver("reference", sweeps=sweeps, omega=1.0)
+        elif mode == "rk4_uran":
+            core.keep_stage_rhs(True)
+            level = core.computeHydrostaticPressure()
+            core.RungeKuttaUran()
+            core.PressureSolver("uran", sweeps=1)
+        elif mode == "step_uran":
+            core.step(steps, "uran", sweeps=1)
         else:
             # p_hydro after the loop is the last step's prep, in the reference and here alike
             core.step(steps, "reference", sweeps=sweeps, omega=1.0)
@@ -51,6 +59,6 @@ def test_gpu_matches_compiled_reference(oracle, name):
         for key in ("sums", "sumsq"):
             x, y = got[key][f], ref[key][f]
             assert x == y or abs(x - y) <= TOL * abs(y), (name, fname, key, x, y)
-    if mode == "rk4":
+    if mode in ("rk4", "rk4_uran"):
         assert np.array_equal(got["level"], ref["level"])
     print(f"[golden-gpu] {name}: worst sampled difference {worst:.3e}", flush=True)

@@ -90,6 +90,37 @@ def test_rk4_matches_oracle(setup, oracle):
     assert report("rk4", errs) <= TOL
 
 
+@pytest.mark.parametrize("calls", [1, 3])
+def test_pressure_uran_matches_oracle(setup, oracle, calls):
+    """The reference's default solver: defect-correction source and lexicographic sweep."""
+    print(f"[case] PressureSolverUran x{calls}", flush=True)
+    g, p, F0 = setup
+    F = F0.copy()
+    oracle.hydro(g, p, F)
+    F[A.FIELD["p_dyn"]] = 1e-3 * np.sin(np.arange(F[0].size).reshape(F[0].shape) * 1e-3)
+    with A.UranusCore(g, p) as core:
+        upload(core, F)
+        core.keep_stage_rhs(True)
+        core.RungeKuttaUran()
+        st = core.PressureSolver("uran", sweeps=calls)
+        G = core.get_state()
+    rhs3 = oracle.rk4_rhs(g, p, F)
+    oracle.pressure_uran(g, p, F, rhs3, calls)
+    errs = {n: rel_err(G[A.FIELD[n]], F[A.FIELD[n]]) for n in ["p_dyn", "aux_u", "aux_v", "aux_w", "t", "w"]}
+    print(f"[parity] uran x{calls}: rel resid {st.rel_resid:.3e}", flush=True)
+    assert report(f"uran x{calls}", errs) == 0.0
+
+
+def test_pressure_uran_needs_kept_rhs(setup):
+    g, p, F0 = setup
+    with A.UranusCore(g, p) as core:
+        upload(core, F0)
+        core.computeHydrostaticPressure()
+        core.RungeKuttaUran()                 # rhs of the stage not kept
+        with pytest.raises(A.AtuError):
+            core.PressureSolver("uran")
+
+
 @pytest.mark.parametrize("sweeps,omega", [(1, 1.0), (20, 1.0), (20, 1.5)])
 def test_pressure_reference_matches_oracle(setup, oracle, sweeps, omega):
     print(f"[case] red-black sweeps={sweeps} omega={omega}", flush=True)
