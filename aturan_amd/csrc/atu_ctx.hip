@@ -374,6 +374,18 @@ StageArgs<float> stage_args32(atu_ctx* c, int stage) {
     a.phydro = c->PHYDRO;
     a.aux = c->AUX32;
     a.ymirror = nullptr;
+    a.yn_lo = c->Y32;
+    // rows [jb0, jb1) run in FP32 when the FP64 rows are the two bands next to the poles (sin(theta)
+    // is monotone on either side of the equator); otherwise one launch picks per row (jb0 = jb1 = 0)
+    const std::vector<double>& pr = c->hgeo.pole_row;
+    const int jm = c->sh.jm;
+    int j0 = 1, j1 = jm - 1;
+    while (j0 < jm - 1 && pr[j0] != 0.0) j0++;
+    while (j1 > j0 && pr[j1 - 1] != 0.0) j1--;
+    bool banded = true;
+    for (int j = j0; j < j1; j++) banded = banded && pr[j] == 0.0;
+    a.jb0 = banded ? j0 : 0;
+    a.jb1 = banded ? j1 : 0;
     switch (stage) {
         case 0: a.xs = c->Y32; a.xnext = c->SA32; break;
         case 1: a.xs = c->SA32; a.xnext = c->SB32; break;

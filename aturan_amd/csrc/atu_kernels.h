@@ -41,6 +41,12 @@ struct StageArgs {
     const double* phydro;// nearby large values, formed in FP64 and rounded once
     S* aux;              // 3 fields, written at stage 3
     double* rhs3;        // when non-null, stage 3 also writes rhs_u, rhs_v, rhs_w (FP64 storage)
+    // FP32 storage only: y_n rounded to FP32 (its mirror), from which stages 1-2 form the next
+    // stage input (stage 0 reads it as its own stencil centre); the final update reads FP64 y_n
+    const S* yn_lo;
+    // FP32 storage only: rows [jb0, jb1) are away from the poles and run in FP32 arithmetic, the
+    // two bands outside them in FP64 (one launch each)
+    int jb0, jb1;
     Shape sh;
     DevGeo geo;
     StepConsts c;

@@ -120,8 +120,11 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     auto ST = [&](int f) {
         return stencil<R, S>(a.xs + (size_t)f * fs, c, cw, ce, plane, ldk, i2dr, idr2, i2dt, idt2, i2dp, idp2);
     };
+    // y_n of the fold: the next stage input of the FP32 path is rounded to FP32 anyway, so it is
+    // formed from y_n's FP32 mirror (at stage 0 the stencil centre itself); FP64 y_n elsewhere
     auto Y0 = [&](int f, R center) -> double {
-        if (STAGE == 0 && sizeof(S) == sizeof(double)) return (double)center;
+        if (STAGE == 0) return (double)center;
+        if (STAGE < 3 && sizeof(S) == sizeof(float)) return (double)a.yn_lo[(size_t)f * fs + c];
         return a.yn[(size_t)f * fs + c];
     };
     auto FZ = [&](int f) -> R { return (R)a.frozen[(size_t)(f - ATU_NUE) * fs + c]; };
@@ -286,33 +289,59 @@ __device__ __forceinline__ void cell(const StageArgs<S>& a, int i, int j, int k)
 
 // One thread per interior cell; a block is 64 k x TJ j x TI i (launched as 64 x 4 x 1: taller
 // blocks, 4x4, 2x4, 4x2 and 8x2, cut the fetched bytes by up to 11% but ran 1-6% slower).
-template <typename S, int STAGE, int TJ, int TI>
+// BAND 0: every interior row, the arithmetic picked per row; BAND 1: rows [jb0, jb1) in FP32;
+// BAND 2: the rows outside [jb0, jb1) in FP64 (blockIdx.y counts the rows of both bands).
+template <typename S, int STAGE, int TJ, int TI, int BAND>
 __global__ void __launch_bounds__(64 * TJ * TI) k_rk_stage(const StageArgs<S> a) {
     const int nk = a.sh.km - 1;
     const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = 1 + blockIdx.y * TJ + threadIdx.y;
+    const int r = blockIdx.y * TJ + threadIdx.y;
     const int i = 1 + blockIdx.z * TI + threadIdx.z;
-    if (k >= nk || j > a.sh.jm - 2 || i > a.sh.im - 2) return;
-    cell<S, STAGE>(a, i, j, k);
+    if (k >= nk || i > a.sh.im - 2) return;
+    if constexpr (BAND == 0) {
+        const int j = 1 + r;
+        if (j > a.sh.jm - 2) return;
+        cell<S, STAGE>(a, i, j, k);
+    } else if constexpr (BAND == 1) {
+        const int j = a.jb0 + r;
+        if (j >= a.jb1) return;
+        rk_cell<float, S, STAGE>(a, i, j, k);
+    } else {
+        const int nlo = a.jb0 - 1;
+        const int j = r < nlo ? 1 + r : a.jb1 + (r - nlo);
+        if (j > a.sh.jm - 2) return;
+        rk_cell<double, S, STAGE>(a, i, j, k);
+    }
 }
 
-template <typename S, int TJ, int TI>
-hipError_t launch_tiled(int stage, const StageArgs<S>& a, hipStream_t s) {
+template <typename S, int TJ, int TI, int BAND>
+hipError_t launch_rows(int stage, int rows, const StageArgs<S>& a, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
     const int nk = a.sh.km - 1;
     const dim3 block(64, TJ, TI);
-    const dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + TJ - 1) / TJ, (a.sh.im - 2 + TI - 1) / TI);
+    const dim3 grid((nk + 63) / 64, (rows + TJ - 1) / TJ, (a.sh.im - 2 + TI - 1) / TI);
     switch (stage) {
-        case 0: hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_rk_stage<S, 1, TJ, TI>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_rk_stage<S, 2, TJ, TI>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_rk_stage<S, 3, TJ, TI>), grid, block, 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI, BAND>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_rk_stage<S, 1, TJ, TI, BAND>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_rk_stage<S, 2, TJ, TI, BAND>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_rk_stage<S, 3, TJ, TI, BAND>), grid, block, 0, s, a); break;
     }
     return hipGetLastError();
 }
 
+// FP64 storage: one launch. FP32 storage with banded pole rows: the FP32 rows and the FP64 bands as
+// two launches (each kernel holds one arithmetic path, so neither carries the other's registers).
 template <typename S>
 hipError_t launch_stage(int stage, const StageArgs<S>& a, hipStream_t s) {
-    return launch_tiled<S, 4, 1>(stage, a, s);
+    const int interior = a.sh.jm - 2;
+    if constexpr (sizeof(S) == sizeof(double)) {
+        return launch_rows<S, 4, 1, 0>(stage, interior, a, s);
+    } else {
+        if (a.jb0 < 1 || a.jb1 < a.jb0 || a.jb1 > a.sh.jm - 1) return launch_rows<S, 4, 1, 0>(stage, interior, a, s);
+        hipError_t e = launch_rows<S, 4, 1, 1>(stage, a.jb1 - a.jb0, a, s);
+        if (e != hipSuccess) return e;
+        return launch_rows<S, 4, 1, 2>(stage, interior - (a.jb1 - a.jb0), a, s);
+    }
 }
 
 }  // namespace

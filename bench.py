@@ -184,6 +184,11 @@ def refined_bench(args):
     opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60)
     optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner)
     out, fields = {}, {}
+
+    def progress(msg):
+        print(f"[refined] {msg}", file=sys.stderr, flush=True)
+
+    progress("initial state ready")
     for prec, opts in (("fp64", opts64), ("mixed", optsmx)):
         with A.UranusCore(g, p, prec) as core:
             core.set_state(F)
@@ -195,6 +200,7 @@ def refined_bench(args):
             core.synchronize()
             el = time.perf_counter() - t0
             fields[prec] = {f: core.get_field(f) for f in set(ERR_FIELDS.values())}
+            progress(f"{prec}: {args.refined_steps} steps in {el:.2f} s")
         out[prec] = {"steps_per_s": round(args.refined_steps / el, 3), "ms_per_step": round(1e3 * el / args.refined_steps, 3),
                      "cell_updates_per_s": round(n_int * args.refined_steps / el, 1),
                      "solve_iterations_last_step": st.sweeps_done, "solve_rel_resid_last_step": st.rel_resid}
@@ -213,6 +219,7 @@ def refined_bench(args):
             gt, gn = core.window_backward(1.0)
             core.synchronize()
             t2 = time.perf_counter()
+            progress(f"{prec} window: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s")
         grads[prec] = (gt[1:-1, 1:-1, :-1], gn)
         out[prec]["adjoint"] = {"window_steps": args.refined_window, "checkpoint_every": 10, "loss": loss,
                                 "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
