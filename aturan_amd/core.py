@@ -195,8 +195,13 @@ class UranusCore:
         _check(self._lib.atu_window_rewind(self._ctx), self._ctx, "atu_window_rewind")
 
     # ---- per-kernel timing ----
-    def profile_begin(self) -> None:
-        _check(self._lib.atu_profile_begin(self._ctx), self._ctx, "atu_profile_begin")
+    def profile_begin(self, classes: str | None = None) -> None:
+        """Bracket kernel launches with HIP events: every class, or the comma-separated `classes`."""
+        if classes is None:
+            _check(self._lib.atu_profile_begin(self._ctx), self._ctx, "atu_profile_begin")
+        else:
+            _check(self._lib.atu_profile_begin_classes(self._ctx, classes.encode()), self._ctx,
+                   "atu_profile_begin_classes")
 
     def profile_query(self, cls: str) -> tuple[float, int]:
         ms, n = C.c_double(), C.c_int()

@@ -107,6 +107,7 @@ struct atu_ctx {
 
     // profiling
     bool prof = false;
+    unsigned prof_mask = ~0u;   // the kernel classes bracketed while profiling
     std::vector<ProfEvent> events;
     std::vector<hipEvent_t> pool;
 
@@ -137,7 +138,7 @@ struct atu_ctx {
     }
     template <typename F>
     hipError_t timed_on(int cls, hipStream_t s, F&& launch) {
-        if (!prof) return launch();
+        if (!prof || !(prof_mask & (1u << cls))) return launch();
         ProfEvent pe{cls, take_event(), take_event()};
         hipEventRecord(pe.a, s);
         const hipError_t e = launch();
@@ -1194,14 +1195,34 @@ const char* atu_build_info(void) {
     return "libaturan_amd abi " "1" " gfx950 hipcc -O3 -ffp-contract=off";
 }
 
-int atu_profile_begin(atu_ctx* ctx) {
+int atu_profile_begin(atu_ctx* ctx) { return atu_profile_begin_classes(ctx, nullptr); }
+
+int atu_profile_begin_classes(atu_ctx* ctx, const char* classes) {
     if (!ctx) return ATU_EINVAL;
+    unsigned mask = ~0u;
+    if (classes) {
+        mask = 0;
+        const std::string list(classes);
+        size_t a = 0;
+        while (a <= list.size()) {
+            size_t b = list.find(',', a);
+            if (b == std::string::npos) b = list.size();
+            const std::string name = list.substr(a, b - a);
+            int id = -1;
+            for (int q = 0; q < PC_N; q++)
+                if (name == kClassNames[q]) id = q;
+            if (id < 0) return ctx->fail(ATU_EINVAL, "atu_profile_begin_classes: unknown kernel class '" + name + "'");
+            mask |= 1u << id;
+            a = b + 1;
+        }
+    }
     if (sync_all(ctx)) return ATU_EHIP;
     for (auto& e : ctx->events) {
         ctx->pool.push_back(e.a);
         ctx->pool.push_back(e.b);
     }
     ctx->events.clear();
+    ctx->prof_mask = mask;
     ctx->prof = true;
     return ATU_OK;
 }

@@ -11,18 +11,22 @@ namespace {
 //
 // The reference adds the terms of a level one after another in (j, k) order, and the buoyancy
 // anomaly divides by this mean, which magnifies any rounding difference of the sum ~200-fold in
-// p_hydro. So the sum is formed in exactly that order: one wavefront per level. Its lanes load and
+// p_hydro. So the sum is formed in exactly that order: one workgroup per level. Its lanes load and
 // form 64 consecutive terms at a time (a skipped term contributes an exact +0.0); a group of
 // kLevelGroup such chunks is parked in LDS and added in sequence while the loads of the next group
-// are in flight. The chain is ~65k dependent adds per level; the step runs this kernel on a side
+// are in flight, the LDS reads of each batch of the chain issued while the previous batch adds.
+// Two waves per level: wave 0 runs the chain of the weighted terms, wave 1 that of the weights, each
+// on its own SIMD. The chain is ~65k dependent adds per level; the step runs this kernel on a side
 // stream, concurrently with the previous step's Poisson solve, which it does not depend on.
 constexpr int kLevelGroup = 8;
-__global__ void __launch_bounds__(64) k_level_serial(Shape sh, DevGeo g, StepConsts c, const double* t,
-                                                     const double* pstat, double* level) {
+__global__ void __launch_bounds__(128) k_level_serial(Shape sh, DevGeo g, StepConsts c, const double* t,
+                                                      const double* pstat, double* level) {
     const int i = blockIdx.x;
     const int n = sh.jm * sh.km;
-    const int lane = threadIdx.x;
-    __shared__ double tb[kLevelGroup * 64], tw[kLevelGroup * 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;   // wave 0: the sum, wave 1: the weights
+    __shared__ double tv[2][kLevelGroup * 64];
+    __shared__ double wsum;
+    double* mine = tv[wv];
     double rt[kLevelGroup], rp[kLevelGroup];   // T and p_stat of the group in flight
     int rj[kLevelGroup];
     auto load_group = [&](int q0) {
@@ -41,7 +45,7 @@ __global__ void __launch_bounds__(64) k_level_serial(Shape sh, DevGeo g, StepCon
             }
         }
     };
-    double s = 0.0, ws = 0.0;
+    double acc = 0.0;
     load_group(0);
     for (int q0 = 0; q0 < n; q0 += kLevelGroup * 64) {
 #pragma unroll
@@ -56,20 +60,34 @@ __global__ void __launch_bounds__(64) k_level_serial(Shape sh, DevGeo g, StepCon
                     wt = wgt;
                 }
             }
-            tb[u * 64 + lane] = term;
-            tw[u * 64 + lane] = wt;
+            mine[u * 64 + lane] = (wv == 0) ? term : wt;
         }
         __syncthreads();
         load_group(q0 + kLevelGroup * 64);      // in flight while the chain below runs
-        const int m = min(kLevelGroup * 64, n - q0);
-#pragma unroll 16
-        for (int l = 0; l < m; l++) {
-            s += tb[l];
-            ws += tw[l];
+        // the whole group (terms past the plane are +0.0, which leaves the sum unchanged) in
+        // batches held in two register sets: the LDS reads of one batch are in flight while the
+        // other batch adds
+        constexpr int kBatch = 8, kG = kLevelGroup * 64;
+        double xa[kBatch], xb[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) xa[u] = mine[u];
+#pragma unroll
+        for (int l0 = 0; l0 < kG; l0 += 2 * kBatch) {
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) xb[u] = mine[l0 + kBatch + u];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) acc += xa[u];
+            const int l2 = (l0 + 2 * kBatch < kG) ? l0 + 2 * kBatch : 0;
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) xa[u] = mine[l2 + u];
+#pragma unroll
+            for (int u = 0; u < kBatch; u++) acc += xb[u];
         }
         __syncthreads();
     }
-    if (lane == 0) level[i] = (ws > 0.0) ? s / ws : 0.0;
+    if (wv == 1 && lane == 0) wsum = acc;
+    __syncthreads();
+    if (wv == 0 && lane == 0) level[i] = (wsum > 0.0) ? acc / wsum : 0.0;
 }
 
 // ---- the column scan: p_hydro[0] = 0, p_hydro[i] = p_hydro[i-1] + 0.5 (f(i-1) + f(i)) dr with
@@ -264,7 +282,7 @@ __global__ void __launch_bounds__(256) k_species_clamp(BcCtx b) {
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
                         const double* w, const double* pstat, double* level_out, double* phydro,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(64), 0, s, sh, g, c, t, pstat, level_out);
+    hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(128), 0, s, sh, g, c, t, pstat, level_out);
     hipLaunchKernelGGL(k_hydro_scan, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, sh, g, c, t, v, w, pstat,
                        level_out, phydro);
     return hipGetLastError();

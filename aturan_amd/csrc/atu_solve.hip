@@ -110,6 +110,70 @@ __global__ void __launch_bounds__(256) k_line(Shape sh, DevGeo g, DevSolve t, P*
     }
 }
 
+// The same line pass with the column's work split three ways. A block is 64 columns of one colour
+// in one row j times kLineWaves waves: the waves form the right-hand sides of all planes at once
+// (their loads all in flight) into LDS, wave 0 runs the two recurrences out of LDS with the same
+// operations in the same order as k_line, and all waves store the column. The dynamic LDS holds
+// ni x 64 values of R.
+constexpr int kLineWaves = 8;
+template <typename P, typename B, typename R>
+__global__ void __launch_bounds__(64 * kLineWaves) k_line_par(Shape sh, DevGeo g, DevSolve t, P* p, const B* b,
+                                                              int colour, const int* done) {
+    if (done && *done) return;
+    extern __shared__ unsigned char line_lds[];
+    R* sv = reinterpret_cast<R*>(line_lds);   // [ni][64]
+    const int N = sh.km - 1;
+    const int j = 1 + blockIdx.y;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int k = 2 * (int)(blockIdx.x * 64 + lane) + ((colour + j) & 1);
+    const bool on = k < N;
+    const int ldk = sh.ldk, jm = sh.jm, ni = t.ni;
+    const R c13 = R(1.0) / R(3.0);
+    const int kw = (k == 0) ? N - 1 : k - 1, ke = (k == N - 1) ? 0 : k + 1;
+    const size_t stride = sh.plane;
+    const size_t x1 = ((size_t)jm + j) * ldk + k;
+    const size_t r1 = x1 - k;
+    if (on) {
+        for (int s = w; s < ni; s += kLineWaves) {
+            const int i = s + 1;
+            const size_t x = x1 + (size_t)s * stride, row = r1 + (size_t)s * stride;
+            const R num2 = (R)g.num2[i], num3 = (R)g.num3[(size_t)i * jm + j];
+            R th;
+            if (j == 1)
+                th = (num2 - num2 * c13) * (R)p[x + ldk];
+            else if (j == jm - 2)
+                th = (num2 - num2 * c13) * (R)p[x - ldk];
+            else
+                th = ((R)p[x + ldk] + (R)p[x - ldk]) * num2;
+            sv[s * 64 + lane] = (R)b[x] - th - ((R)p[row + ke] + (R)p[row + kw]) * num3;
+        }
+    }
+    __syncthreads();
+    if (w == 0 && on) {
+        constexpr bool kDiv = sizeof(R) == sizeof(double);
+        const double* cp = t.line_cp + (size_t)j * ni;
+        const double* dn = (kDiv ? t.line_dn : t.line_inv_dn) + (size_t)j * ni;
+        const R num1 = (R)g.num1;
+        R dprev = R(0.0);
+#pragma unroll 4
+        for (int s = 0; s < ni; s++) {
+            const R a = (s == 0) ? R(0.0) : ((s == ni - 1) ? num1 - num1 * c13 : num1);
+            const R dp = kDiv ? (sv[s * 64 + lane] - a * dprev) / (R)dn[s] : (sv[s * 64 + lane] - a * dprev) * (R)dn[s];
+            sv[s * 64 + lane] = dp;
+            dprev = dp;
+        }
+        R xv = dprev;
+#pragma unroll 4
+        for (int s = ni - 2; s >= 0; s--) {
+            xv = sv[s * 64 + lane] - (R)cp[s] * xv;
+            sv[s * 64 + lane] = xv;
+        }
+    }
+    __syncthreads();
+    if (on)
+        for (int s = w; s < ni; s += kLineWaves) p[x1 + (size_t)s * stride] = (P)sv[s * 64 + lane];
+}
+
 // radial walls (j = 1..jm-2) and polar walls (i = 1..im-2), k = 0..N-1
 template <typename P>
 __global__ void __launch_bounds__(256) k_walls(Shape sh, double c43, double c13, P* p, const int* done) {
@@ -173,15 +237,42 @@ __global__ void __launch_bounds__(256) k_col_resid(Shape sh, DevGeo g, DevSolve 
     R[(size_t)(j - 1) * N + k] = s / t.S2;
 }
 
-// forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin. One
-// 64-lane block per (64 modes, jj); the k sum runs in order, its loads unrolled so that many are
-// in flight at once.
-__global__ void __launch_bounds__(64) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As,
-                                                const int* done) {
+// The same column residual with the levels' terms formed by all waves at once into LDS and summed
+// in order by wave 0 (dynamic LDS: ni x 64 doubles).
+template <typename P, typename B>
+__global__ void __launch_bounds__(64 * kLineWaves) k_col_resid_par(Shape sh, DevGeo g, DevSolve t, const P* p,
+                                                                   const B* b, double* R, const int* done) {
     if (done && *done) return;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int jj = blockIdx.y;
-    if (m > t.NM) return;
+    extern __shared__ unsigned char resid_lds[];
+    double* sv = reinterpret_cast<double*>(resid_lds);   // [ni][64]
+    const int N = sh.km - 1;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int k = blockIdx.x * 64 + lane;
+    const int j = 1 + blockIdx.y;
+    const int ni = sh.im - 2;
+    const bool on = k < N;
+    if (on)
+        for (int s = w; s < ni; s += kLineWaves)
+            sv[s * 64 + lane] = endw(s + 1, 1, sh.im - 2) * resid_at(sh, g, p, b, s + 1, j, k);
+    __syncthreads();
+    if (w == 0 && on) {
+        double acc = 0.0;
+#pragma unroll 8
+        for (int s = 0; s < ni; s++) acc += sv[s * 64 + lane];
+        R[(size_t)(j - 1) * N + k] = acc / t.S2;
+    }
+}
+
+// forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin, each sum
+// over k in order. A block is 64 modes x kDftRows rows, one wave per row: the waves read the same
+// table values, which the CU's cache serves after the first.
+constexpr int kDftRows = 4;
+__global__ void __launch_bounds__(64 * kDftRows) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As,
+                                                           const int* done) {
+    if (done && *done) return;
+    const int m = blockIdx.x * 64 + threadIdx.x;
+    const int jj = blockIdx.y * kDftRows + __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (m > t.NM || jj >= t.J) return;
     const int N = t.N, M1 = t.NM + 1;
     const double* r = R + (size_t)jj * N;
     double sc = 0.0, ss = 0.0;
@@ -196,8 +287,10 @@ __global__ void __launch_bounds__(64) k_dft_fwd(DevSolve t, const double* R, dou
 }
 
 // Per mode and part: the folded theta system, mode 0 pinned at j = 1. One 64-lane block per
-// (mode, part): the right-hand side and the mode's factors are staged in LDS with coalesced loads,
-// one lane runs the two sequential sweeps out of LDS, and the block writes the result back.
+// (mode, part): the right-hand side, the mode's factors and the reciprocals of its pivots are
+// staged in LDS with coalesced loads, one lane runs the two sequential sweeps out of LDS (the
+// forward sweep multiplies by the reciprocals, off the division's latency), and the block writes
+// the result back.
 constexpr int kBaroMaxJ = 1024;
 __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double* Ac, double* As, const int* done) {
     if (done && *done) return;
@@ -214,7 +307,7 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
     for (int jj = threadIdx.x; jj < J; jj += blockDim.x) {
         sd[jj] = d[jj];
         scp[jj] = cp[jj];
-        sdn[jj] = dn[jj];
+        sdn[jj] = 1.0 / dn[jj];   // the sweep multiplies by the pivots' reciprocals
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -225,7 +318,7 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
             if (jj == J - 1) a = e - c13 * e;
             double rhs = sd[jj];
             if (m == 0 && jj == 0) rhs = 0.0;
-            const double dp = (rhs - a * prev) / sdn[jj];
+            const double dp = (rhs - a * prev) * sdn[jj];
             sd[jj] = dp;
             prev = dp;
         }
@@ -239,25 +332,35 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
     for (int jj = threadIdx.x; jj < J; jj += blockDim.x) d[jj] = sd[jj];
 }
 
-// inverse DFT of the correction and its addition to every interior level of the column
-template <typename P>
-__global__ void __launch_bounds__(256) k_dft_inv_add(Shape sh, DevSolve t, const double* Ac, const double* As, P* p,
-                                                     const int* done) {
+// inverse DFT of the correction: Q[jj][k] = (sum over modes in order) / N; a block is 64 k x
+// kDftRows rows, one wave per row, the waves sharing the table values through the CU's cache. Q may
+// overwrite the column residual.
+__global__ void __launch_bounds__(64 * kDftRows) k_dft_inv(DevSolve t, const double* Ac, const double* As, double* Q,
+                                                           const int* done) {
     if (done && *done) return;
     const int N = t.N, NM = t.NM, J = t.J;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int jj = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int jj = blockIdx.y * kDftRows + __builtin_amdgcn_readfirstlane(threadIdx.y);
     if (k >= N || jj >= J) return;
     double s = Ac[jj] * t.ct[k];
 #pragma unroll 8
     for (int m = 1; m < NM; m++)
         s += 2.0 * (Ac[(size_t)m * J + jj] * t.ct[(size_t)m * N + k] + As[(size_t)m * J + jj] * t.st[(size_t)m * N + k]);
     s += Ac[(size_t)NM * J + jj] * t.ct[(size_t)NM * N + k];
-    const double qv = s / N;
-    for (int i = 1; i < sh.im - 1; i++) {
-        const size_t x = ((size_t)i * sh.jm + jj + 1) * sh.ldk + k;
-        p[x] = (P)((double)p[x] + qv);
-    }
+    Q[(size_t)jj * N + k] = s / N;
+}
+
+// the correction added to every interior level: p += Q on the interior
+template <typename P>
+__global__ void __launch_bounds__(256) k_add_q(Shape sh, const double* Q, P* p, const int* done) {
+    if (done && *done) return;
+    const int N = sh.km - 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = 1 + blockIdx.z;
+    if (k >= N || j > sh.jm - 2) return;
+    const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+    p[x] = (P)((double)p[x] + Q[(size_t)(j - 1) * N + k]);
 }
 
 // ---- canonical interior sums: per row, 64-wide trees, chunks, rows and levels in sequence ----
@@ -291,36 +394,23 @@ __global__ void __launch_bounds__(1024) k_dsum_rows(Shape sh, DevGeo g, const P*
     }
 }
 
-// The level sums (rows of a level in order) and their total (levels in order). Each wave takes
-// levels wave, wave + 16, ...: it stages 64 row sums at a time in LDS with one coalesced load and
-// adds them in order; every wave runs the same number of rounds so the block barriers are uniform.
-// With a control block (the residual norm of the converging solve) it also decides convergence on
-// the device: the counters take the work just done, then rel = |r| / |b| (|b|^2 in *bb) <= tol
-// sets done.
-__global__ void __launch_bounds__(1024) k_dsum_final(int ni, int nj, const double* rows, double* out, SolveCtl ctl) {
+// The total of the row sums: partial t (one thread of 256) adds rows t, t + 256, ... in sequence,
+// each wave reduces its 64 partials by the halving tree, and the 4 wave results are reduced by the
+// halving tree. With a control block (the residual norm of the converging solve) it also decides
+// convergence on the device: the counters take the work just done, then rel = |r| / |b| (|b|^2 in
+// *bb) <= tol sets done.
+constexpr int kDsumParts = 256;
+__global__ void __launch_bounds__(kDsumParts) k_dsum_final(int nrows, const double* rows, double* out, SolveCtl ctl) {
     if (ctl.flags && ctl.flags[CTL_DONE]) return;
-    __shared__ double planes[1024];
-    __shared__ double buf[16][64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int base = 0; base < ni; base += 16) {
-        const int i = base + wave;
-        double s = 0.0;
-        for (int j0 = 0; j0 < nj; j0 += 64) {
-            const int j = j0 + lane;
-            buf[wave][lane] = (i < ni && j < nj) ? rows[(size_t)i * nj + j] : 0.0;
-            __syncthreads();
-            if (i < ni) {
-                const int m = min(64, nj - j0);
-                for (int l = 0; l < m; l++) s += buf[wave][l];
-            }
-            __syncthreads();
-        }
-        if (i < ni && lane == 0) planes[i] = s;
-    }
+    __shared__ double wsum[kDsumParts / 64];
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int q = t; q < nrows; q += kDsumParts) s += rows[q];
+    for (int h = 32; h > 0; h >>= 1) s += __shfl_down(s, h, 64);
+    if ((t & 63) == 0) wsum[t >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double total = 0.0;
-        for (int i = 0; i < ni; i++) total += planes[i];
+    if (t == 0) {
+        const double total = (wsum[0] + wsum[2]) + (wsum[1] + wsum[3]);
         *out = total;
         if (ctl.flags) {
             int* f = ctl.flags;
@@ -375,7 +465,13 @@ template <typename P, typename B, typename R>
 hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, int colour,
                        hipStream_t s, const int* done) {
     const int half = (sh.km - 1 + 1) / 2;
-    hipLaunchKernelGGL((k_line<P, B, R>), dim3((half + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, p, b, colour, done);
+    const size_t lds = (size_t)t.ni * 64 * sizeof(R);
+    if (lds <= 48 * 1024)
+        hipLaunchKernelGGL((k_line_par<P, B, R>), dim3((half + 63) / 64, sh.jm - 2), dim3(64, kLineWaves), lds, s, sh,
+                           g, t, p, b, colour, done);
+    else   // a column too deep for the block's LDS: one thread per column
+        hipLaunchKernelGGL((k_line<P, B, R>), dim3((half + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, p, b, colour,
+                           done);
     return hipGetLastError();
 }
 
@@ -397,12 +493,20 @@ hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p
                        double* As, hipStream_t s, const int* done) {
     if (t.J > kBaroMaxJ) return hipErrorInvalidValue;   // the theta solves stage a mode in LDS
     const int N = sh.km - 1;
-    hipLaunchKernelGGL((k_col_resid<P, B>), dim3((N + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, (const P*)p, b,
-                       R, done);
-    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, t.J), dim3(64), 0, s, t, (const double*)R, Ac, As, done);
+    const size_t lds = (size_t)(sh.im - 2) * 64 * sizeof(double);
+    if (lds <= 48 * 1024)
+        hipLaunchKernelGGL((k_col_resid_par<P, B>), dim3((N + 63) / 64, sh.jm - 2), dim3(64, kLineWaves), lds, s, sh, g,
+                           t, (const P*)p, b, R, done);
+    else
+        hipLaunchKernelGGL((k_col_resid<P, B>), dim3((N + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, (const P*)p,
+                           b, R, done);
+    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, (t.J + kDftRows - 1) / kDftRows), dim3(64, kDftRows), 0,
+                       s, t, (const double*)R, Ac, As, done);
     hipLaunchKernelGGL(k_baro_tri, dim3(2 * (t.NM + 1)), dim3(64), 0, s, t, 1.0 / 3.0, Ac, As, done);
-    hipLaunchKernelGGL((k_dft_inv_add<P>), dim3((N + 63) / 64, t.J), dim3(64), 0, s, sh, t, (const double*)Ac,
-                       (const double*)As, p, done);
+    hipLaunchKernelGGL(k_dft_inv, dim3((N + 63) / 64, (t.J + kDftRows - 1) / kDftRows), dim3(64, kDftRows), 0, s, t,
+                       (const double*)Ac, (const double*)As, R, done);
+    hipLaunchKernelGGL((k_add_q<P>), dim3((N + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0, s, sh,
+                       (const double*)R, p, done);
     return hipGetLastError();
 }
 
@@ -427,8 +531,8 @@ hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, c
             hipLaunchKernelGGL((k_dsum_rows<DSUM_RESID2, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, r_out, done);
             break;
     }
-    if (sh.im - 2 > 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(1024), 0, s, sh.im - 2, sh.jm - 2, (const double*)rows, out, ctl);
+    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(kDsumParts), 0, s, (sh.im - 2) * (sh.jm - 2), (const double*)rows,
+                       out, ctl);
     return hipGetLastError();
 }
 

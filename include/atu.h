@@ -282,6 +282,9 @@ int atu_window_rewind(atu_ctx* ctx);
  * summed duration and the launch count of one kernel class: "rk_stage", "rb_pass", "hydro", "bc",
  * "proj_src", "residual", "line", "baro". */
 int atu_profile_begin(atu_ctx* ctx);
+/* The same for the listed classes only (comma-separated names), leaving the other launches
+ * unbracketed: an event pair between two kernels delays the second by a few microseconds. */
+int atu_profile_begin_classes(atu_ctx* ctx, const char* classes);
 int atu_profile_query(atu_ctx* ctx, const char* kernel_class, double* total_ms, int* launches);
 int atu_profile_end(atu_ctx* ctx);
 

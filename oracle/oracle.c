@@ -745,14 +745,22 @@ static double endw(int x, int lo, int hi) { return (x == lo || x == hi) ? 1.5 : 
 
 /* The canonical order of every interior sum of the converging solve, shared with the GPU: per
  * (i, j) row, chunks of 64 consecutive k reduced by a halving tree (v[l] += v[l + h], h = 32..1,
- * missing k as 0), the chunks of a row added in sequence, the rows of a level in sequence (j), the
- * levels in sequence (i). `v` is a dense field; only interior cells are read. */
+ * missing k as 0) and the chunks of a row added in sequence; the row sums, numbered q = (i-1) nj +
+ * (j-1), are dealt to 256 partial sums (partial t adds rows t, t + 256, ... in sequence), each 64
+ * consecutive partials are reduced by the halving tree, and the 4 results by the halving tree
+ * (h = 2, 1). `v` is a dense field; only interior cells are read. */
+#define DSUM_PARTS 256
+static void halving_tree(double* t, int w) {
+    for (int h = w / 2; h > 0; h >>= 1)
+        for (int l = 0; l < h; l++) t[l] += t[l + h];
+}
+
 static double dsum_interior(int im, int jm, int km, const double* v) {
     const int n = km - 1;
     const int nch = (n + 63) / 64;
-    double total = 0.0;
-    for (int i = 1; i < im - 1; i++) {
-        double plane = 0.0;
+    const int ni = im - 2, nj = jm - 2;
+    double* rows = (double*)malloc(sizeof(double) * (size_t)ni * nj);
+    for (int i = 1; i < im - 1; i++)
         for (int j = 1; j < jm - 1; j++) {
             double row = 0.0;
             for (int c = 0; c < nch; c++) {
@@ -761,15 +769,24 @@ static double dsum_interior(int im, int jm, int km, const double* v) {
                     const int k = 64 * c + l;
                     t[l] = (k < n) ? v[IDX(i, j, k)] : 0.0;
                 }
-                for (int h = 32; h > 0; h >>= 1)
-                    for (int l = 0; l < h; l++) t[l] += t[l + h];
+                halving_tree(t, 64);
                 row += t[0];
             }
-            plane += row;
+            rows[(size_t)(i - 1) * nj + (j - 1)] = row;
         }
-        total += plane;
+    double part[DSUM_PARTS], wsum[DSUM_PARTS / 64];
+    for (int t = 0; t < DSUM_PARTS; t++) {
+        double s = 0.0;
+        for (size_t q = t; q < (size_t)ni * nj; q += DSUM_PARTS) s += rows[q];
+        part[t] = s;
     }
-    return total;
+    free(rows);
+    for (int w = 0; w < DSUM_PARTS / 64; w++) {
+        halving_tree(part + 64 * w, 64);
+        wsum[w] = part[64 * w];
+    }
+    halving_tree(wsum, DSUM_PARTS / 64);
+    return wsum[0];
 }
 
 /* Thomas factors of one column (depend on i and j only): cp[i], dn[i] for i = 1..im-2. */
@@ -938,7 +955,7 @@ static void baro_correct(const orc_baro* B, const orc_geo* G, const orc_pgeo* q,
             if (jj == J - 1) a = e - c13 * e;
             double rhs = d[jj];
             if (m == 0 && jj == 0) rhs = 0.0;
-            const double dp = (rhs - a * prev) / dn[jj];
+            const double dp = (rhs - a * prev) * (1.0 / dn[jj]);
             d[jj] = dp;
             prev = dp;
         }
