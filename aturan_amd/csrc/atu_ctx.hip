@@ -69,6 +69,9 @@ struct atu_ctx {
     double* BAC = nullptr;    // (NM+1)*J
     double* BAS = nullptr;
     int* CTL = nullptr;       // device-side convergence control (CTL_N ints)
+    SolveMailbox* MBOX = nullptr;     // its host copy (pinned, mapped) and the device address of it
+    SolveMailbox* MBOX_DEV = nullptr;
+    hipEvent_t ev_ctl = nullptr;      // recorded after a batch of iterations; polled by the host
     // PressureSolverUran: its weight tables, the last stage's rhs_u, rhs_v, rhs_w
     UranTables huran;
     UranTabs duran{};
@@ -148,7 +151,7 @@ struct atu_ctx {
     }
     // side stream for the hydrostatic prep of the next step, which overlaps the Poisson solve
     hipStream_t side = nullptr;
-    hipEvent_t ev_bc = nullptr, ev_hydro = nullptr;
+    hipEvent_t ev_tbc = nullptr, ev_bc = nullptr, ev_hydro = nullptr;
     double* prog(double* buf, int f) const { return buf + (size_t)f * sh.fs; }
 };
 
@@ -158,6 +161,8 @@ atu_ctx::~atu_ctx() {
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
     if (CTL) hipFree(CTL);
+    if (MBOX) hipHostFree(MBOX);
+    if (ev_ctl) hipEventDestroy(ev_ctl);
     if (uran_blob) hipFree(uran_blob);
     if (RHS3) hipFree(RHS3);
     for (double* p : {win.ckpt, win.tape, win.tape_level, win.LY, win.LYN, win.LS, win.LK, win.LK2, win.LAUX, win.LP,
@@ -166,6 +171,7 @@ atu_ctx::~atu_ctx() {
         if (p) hipFree(p);
     for (float* p : {Y32, SA32, SB32, FROZEN32, AUX32, E32, R32})
         if (p) hipFree(p);
+    if (ev_tbc) hipEventDestroy(ev_tbc);
     if (ev_bc) hipEventDestroy(ev_bc);
     if (ev_hydro) hipEventDestroy(ev_hydro);
     if (side) hipStreamDestroy(side);
@@ -255,7 +261,11 @@ int upload_solve_tables(atu_ctx* c) {
         c->hip(hipMalloc((void**)&c->BR, (size_t)t.J * t.N * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->BAC, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->BAS, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc") ||
-        c->hip(hipMalloc((void**)&c->CTL, CTL_N * sizeof(int)), "hipMalloc"))
+        c->hip(hipMalloc((void**)&c->CTL, CTL_N * sizeof(int)), "hipMalloc") ||
+        c->hip(hipHostMalloc((void**)&c->MBOX, sizeof(SolveMailbox), hipHostMallocMapped | hipHostMallocCoherent),
+               "hipHostMalloc(mailbox)") ||
+        c->hip(hipHostGetDevicePointer((void**)&c->MBOX_DEV, c->MBOX, 0), "hipHostGetDevicePointer") ||
+        c->hip(hipEventCreateWithFlags(&c->ev_ctl, hipEventDisableTiming), "hipEventCreate"))
         return ATU_ENOMEM;
     return ATU_OK;
 }
@@ -422,9 +432,52 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
                   "hydro");
 }
 
+// The next step's hydrostatic prep on the side stream, in two parts so that it overlaps this step's
+// BC and Poisson solve: the level means right after the BC of t (which the step then runs first),
+// the column scan after the rest of the BC.
+BcTargets bc_targets(atu_ctx* c) {
+    return c->mixed() ? BcTargets{c->Y, nullptr, nullptr, c->Y32, c->SA32, c->SB32, c->FROZEN, c->FROZEN32}
+                      : BcTargets{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
+}
+int bc_t_then_level(atu_ctx* c) {
+    int e;
+    const BcTargets t = bc_targets(c);
+    if ((e = c->hip(c->timed(PC_BC, [&] { return launch_bc_part(c->sh, c->params, t, ATU_T, ATU_T + 1, false,
+                                                                 c->stream); }), "bc(t)")) ||
+        (e = c->hip(hipEventRecord(c->ev_tbc, c->stream), "hipEventRecord")) ||
+        (e = c->hip(hipStreamWaitEvent(c->side, c->ev_tbc, 0), "hipStreamWaitEvent")))
+        return e;
+    return c->hip(c->timed_on(PC_HYDRO, c->side,
+                              [&] {
+                                  return launch_level(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T), c->PSTAT,
+                                                      c->LEVEL, c->side);
+                              }),
+                  "level");
+}
+// the BC of every field but t (after bc_t_then_level)
+int do_bc_rest(atu_ctx* c) {
+    const BcTargets t = bc_targets(c);
+    return c->hip(c->timed(PC_BC, [&] { return launch_bc_part(c->sh, c->params, t, ATU_T + 1, ATU_NPROG, true,
+                                                               c->stream); }), "bc");
+}
+int side_scan_after_bc(atu_ctx* c) {
+    int e;
+    if ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
+        (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")))
+        return e;
+    if ((e = c->hip(c->timed_on(PC_HYDRO, c->side,
+                                [&] {
+                                    return launch_hydro_scan(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T),
+                                                             c->prog(c->Y, ATU_V), c->prog(c->Y, ATU_W), c->PSTAT,
+                                                             c->LEVEL, c->PHYDRO, c->side);
+                                }),
+                    "hydro_scan")))
+        return e;
+    return c->hip(hipEventRecord(c->ev_hydro, c->side), "hipEventRecord");
+}
+
 int do_bc(atu_ctx* c) {
-    const BcTargets t = c->mixed() ? BcTargets{c->Y, nullptr, nullptr, c->Y32, c->SA32, c->SB32, c->FROZEN, c->FROZEN32}
-                                   : BcTargets{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
+    const BcTargets t = bc_targets(c);
     return c->hip(c->timed(PC_BC, [&] { return launch_bc(c->sh, c->params, t, c->stream); }), "bc");
 }
 
@@ -450,8 +503,9 @@ int do_residual(atu_ctx* c, double* rel, double* srcn) {
 // consistent source b = src - (y.src / y.1) in RES, then iterations until ||b - A p|| <= rel_tol ||b|| (or the iteration limit), then the corners,
 // the seam and the held interior mean.
 //
-// FP64 context: each iteration is two line passes, the walls, the barotropic correction, the walls
-// and the residual norm, all on p in FP64.
+// FP64 context: each iteration is two line passes, the barotropic correction and the residual norm,
+// all on p in FP64 (the line passes hold the walls folded, and the residuals form them from the
+// interior, so the wall values are written once, after the last iteration).
 // Mixed context (iterative refinement): each outer iteration forms the residual r = b - A p in
 // FP64 (stored in FP32), solves A e = r approximately with `inner` FP32 line passes plus barotropic
 // corrections on an FP32 correction e, and accumulates p += e in FP64.
@@ -477,21 +531,25 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     int flags[CTL_N] = {0, 0, 0, 0};
     int it = 0, outer = 0;
     double rel = INFINITY, bnorm = 0.0;
+    // The last residual norm of a batch has written the flags and norms into the host mailbox; the
+    // host polls an event recorded after the batch (no copy, no blocking wait).
     auto read_ctl = [&]() -> int {
-        double h[2];
-        if (H(hipMemcpyAsync(h, c->SCAL + 2, sizeof(h), hipMemcpyDeviceToHost, s), "hipMemcpy(norms)") ||
-            H(hipMemcpyAsync(flags, F, sizeof(flags), hipMemcpyDeviceToHost, s), "hipMemcpy(ctl)") ||
-            H(hipStreamSynchronize(s), "hipStreamSynchronize"))
-            return ATU_EHIP;
-        bnorm = std::sqrt(h[0]);
-        rel = bnorm > 0.0 ? std::sqrt(h[1]) / bnorm : std::sqrt(h[1]);
+        if (H(hipEventRecord(c->ev_ctl, s), "hipEventRecord")) return ATU_EHIP;
+        hipError_t q;
+        while ((q = hipEventQuery(c->ev_ctl)) == hipErrorNotReady) {
+        }
+        if (H(q, "hipEventQuery")) return ATU_EHIP;
+        const volatile SolveMailbox* mb = c->MBOX;
+        for (int k = 0; k < CTL_N; k++) flags[k] = mb->flags[k];
+        bnorm = std::sqrt(mb->bb);
+        rel = bnorm > 0.0 ? std::sqrt(mb->rr) / bnorm : std::sqrt(mb->rr);
         if (!std::isfinite(rel)) return c->fail(ATU_ENONFINITE, "converging solve: non-finite residual");
         return ATU_OK;
     };
     if ((e = H(hipMemsetAsync(F, 0, CTL_N * sizeof(int), s), "hipMemsetAsync(ctl)"))) return e;
     const bool fixed = !(o->rel_tol > 0.0);   // no tolerance: run max_it iterations in one go
     if (!c->mixed()) {
-        const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, 1, 0};
+        const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, 1, 0, c->MBOX_DEV};
         int launched = 0, batch = fixed ? max_it : std::max(1, c->pred_iters);
         while (launched < max_it) {
             const int nb = std::min(batch, max_it - launched);
@@ -501,12 +559,10 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
                                                            sh, c->geo, c->dsol, P, c->RES, colour, s, done); }),
                                "line")))
                         return e;
-                if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, done); }), "walls")) ||
-                    (e = H(c->timed(PC_BARO, [&] { return launch_baro<double, double>(sh, c->geo, c->dsol, P, c->RES,
+                if ((e = H(c->timed(PC_BARO, [&] { return launch_baro<double, double>(sh, c->geo, c->dsol, P, c->RES,
                                                                                        c->BR, c->BAC, c->BAS, s,
                                                                                        done); }),
                            "baro")) ||
-                    (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, done); }), "walls")) ||
                     (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES,
                                                                               c->SROWS, c->SCAL + 3, s, nullptr,
                                                                               chk); }), "dsum")))
@@ -526,8 +582,8 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         // and the next residual with its test.
         const int inner = o->sweeps > 0 ? o->sweeps : 2;
         const int max_ref = max_it / inner;
-        const SolveCtl first{F, c->SCAL + 2, o->rel_tol, 0, 0};
-        const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, inner, 1};
+        const SolveCtl first{F, c->SCAL + 2, o->rel_tol, 0, 0, c->MBOX_DEV};
+        const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, inner, 1, c->MBOX_DEV};
         auto residual = [&](const SolveCtl& k) {
             return H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_RESID2, sh, c->geo, P, c->RES, c->SROWS,
                                                                         c->SCAL + 3, s, c->R32, k); }), "dsum");
@@ -544,15 +600,12 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
                                                                sh, c->geo, c->dsol, c->E32, c->R32, colour, s, done); }),
                                    "line")))
                             return e;
-                    if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s, done); }), "walls")) ||
-                        (e = H(c->timed(PC_BARO, [&] { return launch_baro<float, float>(sh, c->geo, c->dsol, c->E32,
+                    if ((e = H(c->timed(PC_BARO, [&] { return launch_baro<float, float>(sh, c->geo, c->dsol, c->E32,
                                                                                          c->R32, c->BR, c->BAC, c->BAS,
-                                                                                         s, done); }), "baro")) ||
-                        (e = H(c->timed(PC_LINE, [&] { return launch_walls<float>(sh, c->E32, s, done); }), "walls")))
+                                                                                         s, done); }), "baro")))
                         return e;
                 }
                 if ((e = H(c->timed(PC_LINE, [&] { return launch_add_correction(sh, c->E32, P, s, done); }), "add")) ||
-                    (e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, done); }), "walls")) ||
                     (e = residual(chk)))
                     return e;
             }
@@ -566,7 +619,9 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         outer = flags[CTL_OUTER];
         c->pred_outer = std::max(1, outer);
     }
-    if ((e = H(launch_p_corners<double>(sh, c->consts, P, s), "corners")) ||
+    // the walls of the final interior (the iterations' residuals form them as they read them)
+    if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")) ||
+        (e = H(launch_p_corners<double>(sh, c->consts, P, s), "corners")) ||
         (e = H(launch_seam<double>(sh, P, s), "seam")) ||
         (e = H(launch_dsum<double>(DSUM_P, sh, c->geo, P, nullptr, c->SROWS, c->SCAL + 4, s), "dsum")) ||
         (e = H(launch_shift_all<double>(sh, c->SCAL, 1, 4, c->dsol.ncell, P, s), "shift")))
@@ -764,13 +819,9 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
                 return e;
         }
     }
-    if ((e = do_bc(c))) return e;
+    if ((e = last ? do_bc(c) : (bc_t_then_level(c) ? ATU_EHIP : do_bc_rest(c)))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))) return e;
-    if (!last && ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
-                  (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")) ||
-                  (e = do_hydro(c, c->side)) ||
-                  (e = c->hip(hipEventRecord(c->ev_hydro, c->side), "hipEventRecord"))))
-        return e;
+    if (!last && (e = side_scan_after_bc(c))) return e;
     if ((e = do_pressure(c, &o, nullptr))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_SRC), c->SRC, 1))) return e;
     return ATU_OK;
@@ -912,6 +963,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     }
     c->own_stream = true;
     if ((e = c->hip(hipStreamCreate(&c->side), "hipStreamCreate(side)")) ||
+        (e = c->hip(hipEventCreateWithFlags(&c->ev_tbc, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_bc, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_hydro, hipEventDisableTiming), "hipEventCreate"))) {
         g_create_error = c->err;
@@ -1088,8 +1140,9 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
     }
     if (opts && !(opts->omega > 0.0 && opts->omega < 2.0))
         return ctx->fail(ATU_EINVAL, "atu_step: omega must lie in (0, 2)");
-    // The hydrostatic prep of step n+1 reads only t, v, w after the BC of step n, and the Poisson
-    // solve of step n reads only aux and p_dyn: they run concurrently, the prep on the side stream.
+    // The hydrostatic prep of step n+1 reads only t, v, w of step n's result, and the BC and the
+    // Poisson solve of step n do not read its output: the prep runs on the side stream, its level
+    // means once t's BC is done (the step does t's BC first), its column scan after the rest.
     for (int n = 0; n < nsteps; n++) {
         int e;
         if (n == 0) {
@@ -1097,14 +1150,10 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
         } else if ((e = ctx->hip(hipStreamWaitEvent(ctx->stream, ctx->ev_hydro, 0), "hipStreamWaitEvent"))) {
             return e;
         }
-        if ((e = do_rk4(ctx)) || (e = do_bc(ctx))) return e;
-        if (n + 1 < nsteps) {
-            if ((e = ctx->hip(hipEventRecord(ctx->ev_bc, ctx->stream), "hipEventRecord")) ||
-                (e = ctx->hip(hipStreamWaitEvent(ctx->side, ctx->ev_bc, 0), "hipStreamWaitEvent")) ||
-                (e = do_hydro(ctx, ctx->side)) ||
-                (e = ctx->hip(hipEventRecord(ctx->ev_hydro, ctx->side), "hipEventRecord")))
-                return e;
-        }
+        const bool more = n + 1 < nsteps;
+        if ((e = do_rk4(ctx)) || (e = more ? (bc_t_then_level(ctx) ? ATU_EHIP : do_bc_rest(ctx)) : do_bc(ctx)) ||
+            (more && (e = side_scan_after_bc(ctx))))
+            return e;
         if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;
     }
     return ATU_OK;

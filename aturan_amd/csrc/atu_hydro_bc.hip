@@ -143,6 +143,8 @@ struct BcCtx {
     BcTargets t;
     double c43, c13;
     int lid_vw, rigid_lid, radius_positive;
+    int f0, f1;     // the prognostic fields [f0, f1) of this pass
+    int rest;       // this pass also does the frozen fields and the field-specific forms
 };
 
 __device__ __forceinline__ void bc_put(const BcCtx& b, size_t q, double val) {
@@ -175,13 +177,14 @@ __global__ void __launch_bounds__(256) k_bc_radius(BcCtx b) {
     const size_t col = (size_t)j * sh.ldk + k;
     const size_t P = sh.plane;
     const int L = sh.im - 1;
-    for (int f = 0; f < kNumBcFields; f++) {
+    for (int f = b.f0; f < b.f1; f++) {
         const double* F = b.t.y + (size_t)f * sh.fs + col;
         const double lo = b.c43 * F[1 * P] - b.c13 * F[2 * P];
         const double hi = b.c43 * F[(size_t)(L - 1) * P] - b.c13 * F[(size_t)(L - 2) * P];
         bc_put(b, (size_t)f * sh.fs + col, lo);
         bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, hi);
     }
+    if (!b.rest) return;
     for (int q = 0; q < 4; q++) {
         const size_t o = (size_t)frozen_bc(q) * sh.fs + col;
         const double* F = b.t.frozen + o;
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(256) k_bc_theta(BcCtx b) {
     const int ldk = sh.ldk;
     const int Jl = sh.jm - 1;
     const size_t base = (size_t)i * sh.plane + k;
-    for (int f = 0; f < kNumBcFields; f++) {
+    for (int f = b.f0; f < b.f1; f++) {
         const size_t q0 = (size_t)f * sh.fs + base;
         if (f == ATU_V || f == ATU_W) {
             bc_put(b, q0, 0.0);
@@ -241,6 +244,7 @@ __global__ void __launch_bounds__(256) k_bc_theta(BcCtx b) {
         bc_put(b, q0, lo);
         bc_put(b, q0 + (size_t)Jl * ldk, hi);
     }
+    if (!b.rest) return;
     for (int q = 0; q < 4; q++) {
         const size_t o = (size_t)frozen_bc(q) * sh.fs + base;
         fz_put(b, o, 0.0);
@@ -255,10 +259,11 @@ __global__ void __launch_bounds__(256) k_bc_phi(BcCtx b) {
     if (j >= b.sh.jm) return;
     const Shape& sh = b.sh;
     const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
-    for (int f = 0; f < kNumBcFields; f++) {
+    for (int f = b.f0; f < b.f1; f++) {
         const size_t q = (size_t)f * sh.fs + row;
         bc_put(b, q + sh.km - 1, b.t.y[q]);
     }
+    if (!b.rest) return;
     for (int q = 0; q < 4; q++) {
         const size_t o = (size_t)frozen_bc(q) * sh.fs + row;
         fz_put(b, o + sh.km - 1, b.t.frozen[o]);
@@ -279,13 +284,26 @@ __global__ void __launch_bounds__(256) k_species_clamp(BcCtx b) {
 
 }  // namespace
 
+hipError_t launch_level(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* pstat,
+                        double* level_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(128), 0, s, sh, g, c, t, pstat, level_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_hydro_scan(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
+                             const double* w, const double* pstat, const double* level, double* phydro,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_hydro_scan, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, sh, g, c, t, v, w, pstat,
+                       level, phydro);
+    return hipGetLastError();
+}
+
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
                         const double* w, const double* pstat, double* level_out, double* phydro,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(128), 0, s, sh, g, c, t, pstat, level_out);
-    hipLaunchKernelGGL(k_hydro_scan, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, sh, g, c, t, v, w, pstat,
-                       level_out, phydro);
-    return hipGetLastError();
+    const hipError_t e = launch_level(sh, g, c, t, pstat, level_out, s);
+    if (e != hipSuccess) return e;
+    return launch_hydro_scan(sh, g, c, t, v, w, pstat, level_out, phydro, s);
 }
 
 hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s) {
@@ -303,7 +321,8 @@ hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s, const int* 
     return hipGetLastError();
 }
 
-hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s) {
+hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets& t, int f0, int f1, bool rest,
+                          hipStream_t s) {
     BcCtx b;
     b.sh = sh;
     b.t = t;
@@ -312,12 +331,19 @@ hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, h
     b.lid_vw = p.lid_vw;
     b.rigid_lid = p.rigid_lid;
     b.radius_positive = p.radius_positive;
+    b.f0 = f0 < 0 ? 0 : f0;
+    b.f1 = f1 > kNumBcFields ? kNumBcFields : f1;
+    b.rest = rest ? 1 : 0;
     hipLaunchKernelGGL(k_bc_radius, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_bc_theta, dim3((sh.km + 255) / 256, sh.im), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_bc_phi, dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, b);
-    if (p.species_clamp)
+    if (rest && p.species_clamp)
         hipLaunchKernelGGL(k_species_clamp, dim3((sh.km + 255) / 256, sh.jm, sh.im), dim3(256), 0, s, b);
     return hipGetLastError();
+}
+
+hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s) {
+    return launch_bc_part(sh, p, t, 0, kNumBcFields, true, s);
 }
 
 }  // namespace atu

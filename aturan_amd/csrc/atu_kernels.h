@@ -61,6 +61,12 @@ hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t,
                         const double* v, const double* w, const double* pstat, double* level_out,
                         double* phydro, hipStream_t s);
+// The two halves: the level means (which read t only) and the column scan (t, v, w).
+hipError_t launch_level(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* pstat,
+                        double* level_out, hipStream_t s);
+hipError_t launch_hydro_scan(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
+                             const double* w, const double* pstat, const double* level, double* phydro,
+                             hipStream_t s);
 
 // Element-wise precision conversion of n values (FP32 mirrors of FP64 fields and back).
 hipError_t launch_convert(const double* src, float* dst, size_t n, hipStream_t s);
@@ -82,6 +88,11 @@ struct BcTargets {
     float* frozen32;
 };
 hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s);
+// The same passes on the prognostic fields [f0, f1) only, and with `rest` the frozen fields and the
+// field-specific forms (rigid lid, species floor, lid form of v and w, species clamp). The fields do
+// not interact in the BC, so a pass over [0, 1) followed by one over [1, 17) with `rest` is the BC.
+hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets& t, int f0, int f1, bool rest,
+                          hipStream_t s);
 
 // Projection prep on aux (prepareProjectionBoundaries) and the clamped divergence source.
 template <typename S>
@@ -109,11 +120,18 @@ enum { DSUM_P = 0, DSUM_YSRC = 1, DSUM_SQ = 2, DSUM_RESID2 = 3 };
 // an iteration given that flag returns at once, so the host can queue several iterations and read
 // the outcome once. The residual norm's final sum updates the flags (see k_dsum_final).
 enum { CTL_DONE = 0, CTL_ITERS = 1, CTL_OUTER = 2, CTL_N = 4 };
+// What the host reads after a batch: written by the residual norm's final sum into host memory
+// (pinned, mapped), so the host needs no copy, only to know that the batch has run.
+struct SolveMailbox {
+    int flags[CTL_N];
+    double rr, bb;              // |r|^2, |b|^2
+};
 struct SolveCtl {
     int* flags = nullptr;       // CTL_N ints, or null: no control
     const double* bb = nullptr; // |b|^2
     double tol = 0.0;
     int iters = 0, outer = 0;   // added to the counters before the test (the work just done)
+    SolveMailbox* mbox = nullptr;   // device address of the host mailbox, or null
 };
 // P: storage of the unknown, B: of the right-hand side, R: arithmetic of the line pass
 template <typename P, typename B, typename R>

@@ -209,18 +209,28 @@ __global__ void __launch_bounds__(256) k_seam(Shape sh, P* p) {
     p[row + sh.km - 1] = p[row];
 }
 
+// b - A p at an interior cell. The radial and polar wall values it reads are formed from the
+// interior as the wall pass forms them (4/3 p_1 - 1/3 p_2, rounded to P), so no wall pass is
+// needed between an update of the interior and a residual: the iterations form the walls only once,
+// after the last one.
 template <typename P, typename B>
 __device__ __forceinline__ double resid_at(const Shape& sh, const DevGeo& g, const P* p, const B* b, int i, int j,
                                            int k) {
     const int N = sh.km - 1, ldk = sh.ldk;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
     const size_t row = ((size_t)i * sh.jm + j) * ldk;
     const size_t x = row + k;
     const size_t xw = row + (k == 0 ? N - 1 : k - 1), xe = row + (k == N - 1 ? 0 : k + 1);
     const size_t ij = (size_t)i * sh.jm + j;
-    const double acc = ((double)p[x + sh.plane] + (double)p[x - sh.plane]) * g.num1
-                       + ((double)p[x + ldk] + (double)p[x - ldk]) * g.num2[i]
-                       + ((double)p[xe] + (double)p[xw]) * g.num3[ij];
-    return (double)b[x] - (acc - g.den[ij] * (double)p[x]);
+    const double pc = (double)p[x];
+    const double rp = (double)p[x + sh.plane], rn = (double)p[x - sh.plane];
+    const double tp = (double)p[x + ldk], tn = (double)p[x - ldk];
+    const double r_up = (i + 1 == sh.im - 1) ? (double)(P)(c43 * pc - c13 * rn) : rp;
+    const double r_dn = (i - 1 == 0) ? (double)(P)(c43 * pc - c13 * rp) : rn;
+    const double t_up = (j + 1 == sh.jm - 1) ? (double)(P)(c43 * pc - c13 * tn) : tp;
+    const double t_dn = (j - 1 == 0) ? (double)(P)(c43 * pc - c13 * tp) : tn;
+    const double acc = (r_up + r_dn) * g.num1 + (t_up + t_dn) * g.num2[i] + ((double)p[xe] + (double)p[xw]) * g.num3[ij];
+    return (double)b[x] - (acc - g.den[ij] * pc);
 }
 
 // R(jj, k) = sum_i a_i r_i / S2 over each interior column
@@ -419,6 +429,12 @@ __global__ void __launch_bounds__(kDsumParts) k_dsum_final(int nrows, const doub
             const double b2 = *ctl.bb;
             const double rel = b2 > 0.0 ? sqrt(total) / sqrt(b2) : sqrt(total);
             if (rel <= ctl.tol) f[CTL_DONE] = 1;
+            if (ctl.mbox) {
+                for (int q = 0; q < CTL_N; q++) ctl.mbox->flags[q] = f[q];
+                ctl.mbox->rr = total;
+                ctl.mbox->bb = b2;
+                __threadfence_system();
+            }
         }
     }
 }
