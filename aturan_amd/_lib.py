@@ -65,7 +65,7 @@ class Params(C.Structure):
 
 
 class SolverOpts(C.Structure):
-    _fields_ = [("mode", C.c_int32), ("sweeps", C.c_int32), ("max_sweeps", C.c_int32), ("pad_", C.c_int32),
+    _fields_ = [("mode", C.c_int32), ("sweeps", C.c_int32), ("max_sweeps", C.c_int32), ("warm_start", C.c_int32),
                 ("omega", C.c_double), ("rel_tol", C.c_double)]
 
 

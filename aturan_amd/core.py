@@ -65,11 +65,12 @@ class SolveResult:
 
 
 def solver_opts(mode: str = "reference", sweeps: int = 1, omega: float = 1.0, rel_tol: float = 1e-8,
-                max_sweeps: int = 200) -> L.SolverOpts:
+                max_sweeps: int = 200, warm_start: bool = False) -> L.SolverOpts:
     o = L.SolverOpts()
     o.mode = {"reference": L.SOLVE_REFERENCE, "converge": L.SOLVE_CONVERGE, "uran": L.SOLVE_URAN}[mode]
     o.sweeps = sweeps
     o.max_sweeps = max_sweeps
+    o.warm_start = int(warm_start)
     o.omega = omega
     o.rel_tol = rel_tol
     return o
@@ -154,9 +155,10 @@ class UranusCore:
         return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)
 
     def step(self, nsteps: int = 1, mode: str = "reference", sweeps: int = 1, omega: float = 1.0,
-             rel_tol: float = 1e-8, max_sweeps: int = 200) -> SolveResult:
-        """The benchmark's outer step: hydro prep, RK4, BC, Poisson solve; nsteps times."""
-        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps)
+             rel_tol: float = 1e-8, max_sweeps: int = 200, warm_start: bool = False) -> SolveResult:
+        """The benchmark's outer step: hydro prep, RK4, BC, Poisson solve; nsteps times (converge with
+        warm_start: each solve after the first starts from 2 p_n - p_{n-1})."""
+        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps, warm_start)
         st = L.SolveStats()
         _check(self._lib.atu_step(self._ctx, nsteps, C.byref(o), C.byref(st)), self._ctx, "atu_step")
         return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)

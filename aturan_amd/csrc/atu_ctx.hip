@@ -26,8 +26,9 @@ struct ProfEvent {
     hipEvent_t a, b;
 };
 
-const char* kClassNames[] = {"rk_stage", "rb_pass", "hydro", "bc", "proj_src", "residual", "line", "baro", "adjoint"};
-enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_ADJ, PC_N };
+const char* kClassNames[] = {"rk_stage", "rb_pass", "hydro", "bc", "proj_src", "residual", "line", "baro", "adjoint",
+                             "step"};
+enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_ADJ, PC_STEP, PC_N };
 
 }  // namespace
 
@@ -64,6 +65,9 @@ struct atu_ctx {
     DevSolve dsol{};
     void* solve_blob = nullptr;
     double* SROWS = nullptr;  // (im-2)*(jm-2)
+    double* SROWS2 = nullptr; // the same, for a second sum of one pass
+    double* PPREV = nullptr;  // p_dyn as the last warm-started solve found it (interior)
+    unsigned long state_epoch = 0, warm_epoch = ~0ul;   // warm start valid iff equal
     double* SCAL = nullptr;   // 8 scalars: 0 y.src, 1 mean before, 2 |b|^2, 3 |r|^2, 4 mean after
     double* BR = nullptr;     // J*N
     double* BAC = nullptr;    // (NM+1)*J
@@ -151,12 +155,12 @@ struct atu_ctx {
     }
     // side stream for the hydrostatic prep of the next step, which overlaps the Poisson solve
     hipStream_t side = nullptr;
-    hipEvent_t ev_tbc = nullptr, ev_bc = nullptr, ev_hydro = nullptr;
+    hipEvent_t ev_t3 = nullptr, ev_tbc = nullptr, ev_bc = nullptr, ev_hydro = nullptr;
     double* prog(double* buf, int f) const { return buf + (size_t)f * sh.fs; }
 };
 
 atu_ctx::~atu_ctx() {
-    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SCAL,
+    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SROWS2, PPREV, SCAL,
                       BR, BAC, BAS})
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
@@ -171,6 +175,7 @@ atu_ctx::~atu_ctx() {
         if (p) hipFree(p);
     for (float* p : {Y32, SA32, SB32, FROZEN32, AUX32, E32, R32})
         if (p) hipFree(p);
+    if (ev_t3) hipEventDestroy(ev_t3);
     if (ev_tbc) hipEventDestroy(ev_tbc);
     if (ev_bc) hipEventDestroy(ev_bc);
     if (ev_hydro) hipEventDestroy(ev_hydro);
@@ -258,6 +263,7 @@ int upload_solve_tables(atu_ctx* c) {
     const size_t nrow = (size_t)(c->sh.im - 2) * (c->sh.jm - 2);
     if (c->hip(hipMalloc((void**)&c->SROWS, nrow * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->SCAL, 8 * sizeof(double)), "hipMalloc") ||
+        c->hip(hipMalloc((void**)&c->SROWS2, nrow * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->BR, (size_t)t.J * t.N * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->BAC, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc") ||
         c->hip(hipMalloc((void**)&c->BAS, (size_t)(t.NM + 1) * t.J * sizeof(double)), "hipMalloc") ||
@@ -406,18 +412,48 @@ StageArgs<float> stage_args32(atu_ctx* c, int stage) {
     return a;
 }
 
-int do_rk4(atu_ctx* c) {
+// the BC targets: y and the stage buffers that carry its boundary planes (FP64 or FP32 mirrors)
+BcTargets bc_targets(atu_ctx* c) {
+    return c->mixed() ? BcTargets{c->Y, nullptr, nullptr, c->Y32, c->SA32, c->SB32, c->FROZEN, c->FROZEN32}
+                      : BcTargets{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
+}
+
+hipError_t launch_stage_part(atu_ctx* c, int s, int part) {
+    return c->mixed() ? launch_rk_stage_mixed(s, stage_args32(c, s), c->stream, part)
+                      : launch_rk_stage_f64(s, stage_args(c, s), c->stream, part);
+}
+
+// The last stage with the next step's hydrostatic prep started early: t's update first, then on
+// the side stream t's BC (its own fields only) and the level means, while the main stream updates
+// the other fields. ev_tbc marks t's BC on the side stream.
+int rk_last_stage_split(atu_ctx* c) {
+    const BcTargets t = bc_targets(c);
+    const hipError_t e = c->timed(PC_RK, [&]() -> hipError_t {
+        hipError_t r;
+        if ((r = launch_stage_part(c, 3, 1)) != hipSuccess || (r = hipEventRecord(c->ev_t3, c->stream)) != hipSuccess ||
+            (r = hipStreamWaitEvent(c->side, c->ev_t3, 0)) != hipSuccess ||
+            (r = c->timed_on(PC_BC, c->side,
+                             [&] { return launch_bc_part(c->sh, c->params, t, ATU_T, ATU_T + 1, false, c->side); })) !=
+                hipSuccess ||
+            (r = hipEventRecord(c->ev_tbc, c->side)) != hipSuccess ||
+            (r = c->timed_on(PC_HYDRO, c->side,
+                             [&] {
+                                 return launch_level(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T), c->PSTAT,
+                                                     c->LEVEL, c->side);
+                             })) != hipSuccess)
+            return r;
+        return launch_stage_part(c, 3, 2);
+    });
+    return c->hip(e, "rk_stage");
+}
+
+// The four stages; with `overlap`, the last one as rk_last_stage_split (the caller then runs
+// do_bc_rest and side_scan_after_bc instead of do_bc).
+int do_rk4(atu_ctx* c, bool overlap = false) {
     c->rhs_valid = c->keep_rhs && !c->mixed();
     for (int s = 0; s < 4; s++) {
-        hipError_t e;
-        if (c->mixed()) {
-            const StageArgs<float> a = stage_args32(c, s);
-            e = c->timed(PC_RK, [&] { return launch_rk_stage_mixed(s, a, c->stream); });
-        } else {
-            const StageArgs<double> a = stage_args(c, s);
-            e = c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); });
-        }
-        if (c->hip(e, "rk_stage")) return ATU_EHIP;
+        if (s == 3 && overlap) return rk_last_stage_split(c);
+        if (c->hip(c->timed(PC_RK, [&] { return launch_stage_part(c, s, 0); }), "rk_stage")) return ATU_EHIP;
     }
     return ATU_OK;
 }
@@ -432,29 +468,10 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
                   "hydro");
 }
 
-// The next step's hydrostatic prep on the side stream, in two parts so that it overlaps this step's
-// BC and Poisson solve: the level means right after the BC of t (which the step then runs first),
-// the column scan after the rest of the BC.
-BcTargets bc_targets(atu_ctx* c) {
-    return c->mixed() ? BcTargets{c->Y, nullptr, nullptr, c->Y32, c->SA32, c->SB32, c->FROZEN, c->FROZEN32}
-                      : BcTargets{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
-}
-int bc_t_then_level(atu_ctx* c) {
-    int e;
-    const BcTargets t = bc_targets(c);
-    if ((e = c->hip(c->timed(PC_BC, [&] { return launch_bc_part(c->sh, c->params, t, ATU_T, ATU_T + 1, false,
-                                                                 c->stream); }), "bc(t)")) ||
-        (e = c->hip(hipEventRecord(c->ev_tbc, c->stream), "hipEventRecord")) ||
-        (e = c->hip(hipStreamWaitEvent(c->side, c->ev_tbc, 0), "hipStreamWaitEvent")))
-        return e;
-    return c->hip(c->timed_on(PC_HYDRO, c->side,
-                              [&] {
-                                  return launch_level(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T), c->PSTAT,
-                                                      c->LEVEL, c->side);
-                              }),
-                  "level");
-}
-// the BC of every field but t (after bc_t_then_level)
+// The next step's hydrostatic prep on the side stream, in two parts so that it overlaps the end of
+// this step: the level means from the last RK stage on (rk_last_stage_split), the column scan after
+// the BC.
+// the BC of every field but t (after rk_last_stage_split)
 int do_bc_rest(atu_ctx* c) {
     const BcTargets t = bc_targets(c);
     return c->hip(c->timed(PC_BC, [&] { return launch_bc_part(c->sh, c->params, t, ATU_T + 1, ATU_NPROG, true,
@@ -514,14 +531,17 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     hipStream_t s = c->stream;
     auto H = [&](hipError_t e, const char* w) { return c->hip(e, w); };
     int e;
-    if ((e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_YSRC, sh, c->geo, P, SRCP, c->SROWS,
-                                                                  c->SCAL + 0, s); }), "dsum")) ||
-        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_P, sh, c->geo, P, nullptr, c->SROWS,
-                                                                  c->SCAL + 1, s); }), "dsum")) ||
-        (e = H(launch_make_b(sh, SRCP, c->SCAL, c->dsol.yw, c->RES, s), "make_b")) ||
-        (e = H(c->timed(PC_RES, [&] { return launch_dsum<double>(DSUM_SQ, sh, c->geo, P, c->RES, c->SROWS,
-                                                                  c->SCAL + 2, s); }), "dsum")))
+    // y.src, the mean of p to hold, the control flags cleared, b and |b|^2; with a warm start the
+    // same pass keeps p_n and starts the iterations from 2 p_n - p_{n-1}
+    const bool record = o->warm_start != 0 && P == c->PDYN;
+    const bool extrap = record && c->warm_epoch == c->state_epoch;
+    if (record && !c->PPREV && H(hipMalloc((void**)&c->PPREV, sh.fs * sizeof(double)), "hipMalloc(p_prev)"))
+        return ATU_ENOMEM;
+    if ((e = H(c->timed(PC_RES, [&] { return launch_solve_setup(sh, P, SRCP, c->SROWS, c->SROWS2, c->SCAL,
+                                                                 c->dsol.yw, c->RES, c->CTL, record ? c->PPREV : nullptr,
+                                                                 extrap, s); }), "solve setup")))
         return e;
+    c->warm_epoch = ~0ul;
     const int max_it = o->max_sweeps > 0 ? o->max_sweeps : 1;
     // Convergence is decided on the device (SolveCtl): the iterations' kernels return at once after
     // the residual norm has met the tolerance, so the host queues as many iterations as the last
@@ -546,7 +566,6 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         if (!std::isfinite(rel)) return c->fail(ATU_ENONFINITE, "converging solve: non-finite residual");
         return ATU_OK;
     };
-    if ((e = H(hipMemsetAsync(F, 0, CTL_N * sizeof(int), s), "hipMemsetAsync(ctl)"))) return e;
     const bool fixed = !(o->rel_tol > 0.0);   // no tolerance: run max_it iterations in one go
     if (!c->mixed()) {
         const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, 1, 0, c->MBOX_DEV};
@@ -634,6 +653,7 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     }
     if (rel > o->rel_tol && o->rel_tol > 0.0)
         return c->fail(ATU_ENOCONV, "converging solve: tolerance not reached within max_sweeps");
+    if (record) c->warm_epoch = c->state_epoch;
     return ATU_OK;
 }
 
@@ -804,6 +824,10 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
                                  "hipMemcpyAsync(level)")))
         return e;
     for (int s = 0; s < 4; s++) {
+        if (s == 3 && !last) {
+            if ((e = rk_last_stage_split(c))) return e;
+            break;
+        }
         if (!c->mixed()) {
             const StageArgs<double> a = stage_args(c, s);
             if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")))
@@ -819,8 +843,10 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
                 return e;
         }
     }
-    if ((e = last ? do_bc(c) : (bc_t_then_level(c) ? ATU_EHIP : do_bc_rest(c)))) return e;
-    if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))) return e;
+    if ((e = last ? do_bc(c) : do_bc_rest(c))) return e;
+    if (slot >= 0 && ((!last && (e = c->hip(hipStreamWaitEvent(c->stream, c->ev_tbc, 0), "hipStreamWaitEvent"))) ||
+                      (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))))
+        return e;
     if (!last && (e = side_scan_after_bc(c))) return e;
     if ((e = do_pressure(c, &o, nullptr))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_SRC), c->SRC, 1))) return e;
@@ -963,6 +989,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     }
     c->own_stream = true;
     if ((e = c->hip(hipStreamCreate(&c->side), "hipStreamCreate(side)")) ||
+        (e = c->hip(hipEventCreateWithFlags(&c->ev_t3, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_tbc, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_bc, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_hydro, hipEventDisableTiming), "hipEventCreate"))) {
@@ -1047,6 +1074,7 @@ int atu_synchronize(atu_ctx* ctx) {
 
 int atu_set_field(atu_ctx* ctx, int f, const double* host) {
     if (!ctx || !host) return ATU_EINVAL;
+    ctx->state_epoch++;
     const bool aux = (f >= ATU_AUX_U && f <= ATU_AUX_W);
     double* d = field_ptr(ctx, f);
     if (!d) return ctx->fail(ATU_EINVAL, "atu_set_field: unknown field id");
@@ -1092,6 +1120,7 @@ int atu_get_field(atu_ctx* ctx, int f, double* host) {
 
 int atu_hydro_prep(atu_ctx* ctx, double* level_means) {
     if (!ctx) return ATU_EINVAL;
+    ctx->state_epoch++;
     int e = do_hydro(ctx, ctx->stream);
     if (e) return e;
     if (level_means) {
@@ -1104,11 +1133,12 @@ int atu_hydro_prep(atu_ctx* ctx, double* level_means) {
     return ATU_OK;
 }
 
-int atu_rk4(atu_ctx* ctx) { return ctx ? do_rk4(ctx) : ATU_EINVAL; }
-int atu_apply_bc(atu_ctx* ctx) { return ctx ? do_bc(ctx) : ATU_EINVAL; }
+int atu_rk4(atu_ctx* ctx) { return ctx ? (ctx->state_epoch++, do_rk4(ctx)) : ATU_EINVAL; }
+int atu_apply_bc(atu_ctx* ctx) { return ctx ? (ctx->state_epoch++, do_bc(ctx)) : ATU_EINVAL; }
 
 int atu_pressure_solve(atu_ctx* ctx, const atu_solver_opts* opts, atu_solve_stats* stats) {
     if (!ctx) return ATU_EINVAL;
+    ctx->state_epoch++;
     if (opts && !(opts->omega > 0.0 && opts->omega < 2.0))
         return ctx->fail(ATU_EINVAL, "atu_pressure_solve: omega must lie in (0, 2)");
     return do_pressure(ctx, opts, stats);
@@ -1145,22 +1175,34 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
     // means once t's BC is done (the step does t's BC first), its column scan after the rest.
     for (int n = 0; n < nsteps; n++) {
         int e;
+        const bool step_ev = ctx->prof && (ctx->prof_mask & (1u << PC_STEP));
+        ProfEvent pe{PC_STEP, nullptr, nullptr};
+        if (step_ev) {
+            pe.a = ctx->take_event();
+            pe.b = ctx->take_event();
+            hipEventRecord(pe.a, ctx->stream);
+        }
         if (n == 0) {
             if ((e = do_hydro(ctx, ctx->stream))) return e;
         } else if ((e = ctx->hip(hipStreamWaitEvent(ctx->stream, ctx->ev_hydro, 0), "hipStreamWaitEvent"))) {
             return e;
         }
         const bool more = n + 1 < nsteps;
-        if ((e = do_rk4(ctx)) || (e = more ? (bc_t_then_level(ctx) ? ATU_EHIP : do_bc_rest(ctx)) : do_bc(ctx)) ||
+        if ((e = do_rk4(ctx, more)) || (e = more ? do_bc_rest(ctx) : do_bc(ctx)) ||
             (more && (e = side_scan_after_bc(ctx))))
             return e;
         if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;
+        if (step_ev) {
+            hipEventRecord(pe.b, ctx->stream);
+            ctx->events.push_back(pe);
+        }
     }
     return ATU_OK;
 }
 
 int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) {
     if (!ctx || !opts || opts->nsteps < 1) return ATU_EINVAL;
+    ctx->state_epoch++;
     if (!ctx->params.rigid_lid) return ctx->fail(ATU_EUNSUPPORTED, "atu_window_forward: needs rigid_lid = 1");
     if (opts->solve_iters <= 0 && !(opts->rel_tol > 0.0))
         return ctx->fail(ATU_EINVAL, "atu_window_forward: give solve_iters > 0 or rel_tol > 0");
@@ -1192,6 +1234,7 @@ int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) 
 
 int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* grad_nue) {
     if (!ctx) return ATU_EINVAL;
+    ctx->state_epoch++;
     auto& W = ctx->win;
     if (!W.ready) return ctx->fail(ATU_EINVAL, "atu_window_backward: no atu_window_forward to differentiate");
     const Shape& sh = ctx->sh;
@@ -1224,6 +1267,7 @@ int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* gra
 
 int atu_window_rewind(atu_ctx* ctx) {
     if (!ctx) return ATU_EINVAL;
+    ctx->state_epoch++;
     if (!ctx->win.ready) return ctx->fail(ATU_EINVAL, "atu_window_rewind: no atu_window_forward to rewind to");
     int e = ck_restore(ctx, 0);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

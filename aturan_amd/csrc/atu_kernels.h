@@ -53,8 +53,9 @@ struct StageArgs {
 };
 
 // Launchers (stream-ordered, asynchronous).
-hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s);
-hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream_t s);
+// part (stage 3 only): 0 every field, 1 t only, 2 every field but t (1 then 2 is the stage).
+hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s, int part = 0);
+hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream_t s, int part = 0);
 
 // Hydrostatic prep: level means (in the reference's serial summation order) into `level_out` [im],
 // then the column scan into `phydro`.
@@ -151,6 +152,13 @@ template <typename P>
 hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, const double* v, double* rows,
                        double* out, hipStream_t s, float* r_out = nullptr, const SolveCtl& ctl = SolveCtl());
 hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, double yw, double* b, hipStream_t s);
+// The start of a converging solve in four launches: scal[0] = y . src and scal[1] = the interior sum
+// of p (same orders as launch_dsum), the control flags cleared, b = src - scal[0] / yw and
+// scal[2] = |b|^2. `rows`, `rows2`: (im-2)*(jm-2) doubles each. With `pprev` the first pass also
+// stores p's interior there and, with `extrap`, replaces it by 2 p - pprev (the warm start).
+hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, double* rows, double* rows2,
+                              double* scal, double yw, double* b, int* flags, double* pprev, bool extrap,
+                              hipStream_t s);
 hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done = nullptr);
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,

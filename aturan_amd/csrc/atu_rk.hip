@@ -95,7 +95,9 @@ __device__ __forceinline__ double cp_lapse_dev(const StageArgs<S>& a, double T) 
 }
 
 // The tendencies of one cell in arithmetic precision R from stencil storage S, folded in FP64.
-template <typename R, typename S, int STAGE>
+// PART 0: every field; PART 1: t only; PART 2: every field but t (t's tendency reads no other
+// tendency, so the last stage can update t first and the rest after).
+template <typename R, typename S, int STAGE, int PART>
 __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int k) {
     const int nk = a.sh.km - 1;
     const int ldk = a.sh.ldk;
@@ -130,7 +132,16 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     auto FZ = [&](int f) -> R { return (R)a.frozen[(size_t)(f - ATU_NUE) * fs + c]; };
 
     // velocities and the pressure gradient
-    const D7<R> U = ST(ATU_U), V = ST(ATU_V), W = ST(ATU_W);
+    D7<R> U{}, V{}, W{};
+    if constexpr (PART != 1) {
+        U = ST(ATU_U);
+        V = ST(ATU_V);
+        W = ST(ATU_W);
+    } else {   // t's tendency reads the velocities at the cell only
+        U.c = (R)a.xs[(size_t)ATU_U * fs + c];
+        V.c = (R)a.xs[(size_t)ATU_V * fs + c];
+        W.c = (R)a.xs[(size_t)ATU_W * fs + c];
+    }
     const R u0 = U.c, v0 = V.c, w0 = W.c;
     // p_dyn (|p| ~ 20) varies slowly: its differences are formed in FP64 and rounded once to R,
     // since an FP32 difference of two such values keeps few significant bits
@@ -153,7 +164,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     const R re = (R)C.re;
 
     // ---- temperature (:280-296, :298, :333-334, :540-557) ----
-    {
+    if constexpr (PART != 2) {
         const D7<R> T = ST(ATU_T);
         R pressure_t = (R)C.coeff_energy_p * scale_p * (u0 * dpdr + v_invrm * dpdthe + w_invrs * dpdphi);
         if ((C.cp_mode != 0 || C.cp_gas != 0) && T.c > R(0.0))
@@ -167,6 +178,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         const R rhs_t = +pressure_t - transport_t + diffusion_t / (R)C.re_pr + diffusion_t * nue_t_s + precip_t;
         fold<STAGE, CL_T>(a, ATU_T, c, Y0(ATU_T, T.c), (double)rhs_t);
     }
+    if constexpr (PART == 1) return;
 
     // ---- radial velocity u (:303-304, :336-338, :723-732) ----
     {
@@ -275,15 +287,15 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
 // FP64 storage: FP64 arithmetic. FP32 storage (the mixed path): FP32 arithmetic, except on the
 // rows next to the poles where the sin(theta) floor of the metric is active, which run the same
 // stencil in FP64 arithmetic (j is uniform across a wavefront, so the branch does not diverge).
-template <typename S, int STAGE>
+template <typename S, int STAGE, int PART>
 __device__ __forceinline__ void cell(const StageArgs<S>& a, int i, int j, int k) {
     if constexpr (sizeof(S) == sizeof(double)) {
-        rk_cell<double, S, STAGE>(a, i, j, k);
+        rk_cell<double, S, STAGE, PART>(a, i, j, k);
     } else {
         if (a.geo.pole_row[j] != 0.0)
-            rk_cell<double, S, STAGE>(a, i, j, k);
+            rk_cell<double, S, STAGE, PART>(a, i, j, k);
         else
-            rk_cell<float, S, STAGE>(a, i, j, k);
+            rk_cell<float, S, STAGE, PART>(a, i, j, k);
     }
 }
 
@@ -291,7 +303,7 @@ __device__ __forceinline__ void cell(const StageArgs<S>& a, int i, int j, int k)
 // blocks, 4x4, 2x4, 4x2 and 8x2, cut the fetched bytes by up to 11% but ran 1-6% slower).
 // BAND 0: every interior row, the arithmetic picked per row; BAND 1: rows [jb0, jb1) in FP32;
 // BAND 2: the rows outside [jb0, jb1) in FP64 (blockIdx.y counts the rows of both bands).
-template <typename S, int STAGE, int TJ, int TI, int BAND>
+template <typename S, int STAGE, int TJ, int TI, int BAND, int PART>
 __global__ void __launch_bounds__(64 * TJ * TI) k_rk_stage(const StageArgs<S> a) {
     const int nk = a.sh.km - 1;
     const int k = blockIdx.x * 64 + threadIdx.x;
@@ -301,30 +313,37 @@ __global__ void __launch_bounds__(64 * TJ * TI) k_rk_stage(const StageArgs<S> a)
     if constexpr (BAND == 0) {
         const int j = 1 + r;
         if (j > a.sh.jm - 2) return;
-        cell<S, STAGE>(a, i, j, k);
+        cell<S, STAGE, PART>(a, i, j, k);
     } else if constexpr (BAND == 1) {
         const int j = a.jb0 + r;
         if (j >= a.jb1) return;
-        rk_cell<float, S, STAGE>(a, i, j, k);
+        rk_cell<float, S, STAGE, PART>(a, i, j, k);
     } else {
         const int nlo = a.jb0 - 1;
         const int j = r < nlo ? 1 + r : a.jb1 + (r - nlo);
         if (j > a.sh.jm - 2) return;
-        rk_cell<double, S, STAGE>(a, i, j, k);
+        rk_cell<double, S, STAGE, PART>(a, i, j, k);
     }
 }
 
 template <typename S, int TJ, int TI, int BAND>
-hipError_t launch_rows(int stage, int rows, const StageArgs<S>& a, hipStream_t s) {
+hipError_t launch_rows(int stage, int part, int rows, const StageArgs<S>& a, hipStream_t s) {
     if (rows <= 0) return hipSuccess;
     const int nk = a.sh.km - 1;
     const dim3 block(64, TJ, TI);
     const dim3 grid((nk + 63) / 64, (rows + TJ - 1) / TJ, (a.sh.im - 2 + TI - 1) / TI);
     switch (stage) {
-        case 0: hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI, BAND>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_rk_stage<S, 1, TJ, TI, BAND>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_rk_stage<S, 2, TJ, TI, BAND>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_rk_stage<S, 3, TJ, TI, BAND>), grid, block, 0, s, a); break;
+        case 0: hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI, BAND, 0>), grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_rk_stage<S, 1, TJ, TI, BAND, 0>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_rk_stage<S, 2, TJ, TI, BAND, 0>), grid, block, 0, s, a); break;
+        default:
+            if (part == 1)
+                hipLaunchKernelGGL((k_rk_stage<S, 3, TJ, TI, BAND, 1>), grid, block, 0, s, a);
+            else if (part == 2)
+                hipLaunchKernelGGL((k_rk_stage<S, 3, TJ, TI, BAND, 2>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_rk_stage<S, 3, TJ, TI, BAND, 0>), grid, block, 0, s, a);
+            break;
     }
     return hipGetLastError();
 }
@@ -332,25 +351,26 @@ hipError_t launch_rows(int stage, int rows, const StageArgs<S>& a, hipStream_t s
 // FP64 storage: one launch. FP32 storage with banded pole rows: the FP32 rows and the FP64 bands as
 // two launches (each kernel holds one arithmetic path, so neither carries the other's registers).
 template <typename S>
-hipError_t launch_stage(int stage, const StageArgs<S>& a, hipStream_t s) {
+hipError_t launch_stage(int stage, int part, const StageArgs<S>& a, hipStream_t s) {
     const int interior = a.sh.jm - 2;
     if constexpr (sizeof(S) == sizeof(double)) {
-        return launch_rows<S, 4, 1, 0>(stage, interior, a, s);
+        return launch_rows<S, 4, 1, 0>(stage, part, interior, a, s);
     } else {
-        if (a.jb0 < 1 || a.jb1 < a.jb0 || a.jb1 > a.sh.jm - 1) return launch_rows<S, 4, 1, 0>(stage, interior, a, s);
-        hipError_t e = launch_rows<S, 4, 1, 1>(stage, a.jb1 - a.jb0, a, s);
+        if (a.jb0 < 1 || a.jb1 < a.jb0 || a.jb1 > a.sh.jm - 1)
+            return launch_rows<S, 4, 1, 0>(stage, part, interior, a, s);
+        hipError_t e = launch_rows<S, 4, 1, 1>(stage, part, a.jb1 - a.jb0, a, s);
         if (e != hipSuccess) return e;
-        return launch_rows<S, 4, 1, 2>(stage, interior - (a.jb1 - a.jb0), a, s);
+        return launch_rows<S, 4, 1, 2>(stage, part, interior - (a.jb1 - a.jb0), a, s);
     }
 }
 
 }  // namespace
 
-hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s) {
-    return launch_stage<double>(stage, a, s);
+hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s, int part) {
+    return launch_stage<double>(stage, part, a, s);
 }
-hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream_t s) {
-    return launch_stage<float>(stage, a, s);
+hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream_t s, int part) {
+    return launch_stage<float>(stage, part, a, s);
 }
 
 }  // namespace atu

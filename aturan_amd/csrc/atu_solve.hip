@@ -404,6 +404,59 @@ __global__ void __launch_bounds__(1024) k_dsum_rows(Shape sh, DevGeo g, const P*
     }
 }
 
+// The row reduction of k_dsum_rows for a block of one row (64-wide trees, chunks in sequence).
+__device__ __forceinline__ void row_sum_store(double term, double* chunk, double* out) {
+    for (int h = 32; h > 0; h >>= 1) term += __shfl_down(term, h, 64);
+    if ((threadIdx.x & 63) == 0) chunk[threadIdx.x >> 6] = term;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double row = 0.0;
+        for (int c = 0; c < (int)(blockDim.x >> 6); c++) row += chunk[c];
+        *out = row;
+    }
+}
+
+// The two sums the solve starts with, in one pass: y . src and the interior sum of p. With pprev,
+// the pass also keeps p in pprev and (extrap) starts the iterations from 2 p - pprev.
+__global__ void __launch_bounds__(1024) k_dsum_rows_ysrc_p(Shape sh, double* p, const double* src, double* rows_y,
+                                                           double* rows_p, double* pprev, int extrap) {
+    const int N = sh.km - 1;
+    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
+    const int k = threadIdx.x;
+    double ty = 0.0, tp = 0.0;
+    if (k < N) {
+        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+        ty = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * src[x];
+        tp = p[x];
+        if (pprev) {
+            if (extrap) p[x] = 2.0 * tp - pprev[x];
+            pprev[x] = tp;
+        }
+    }
+    __shared__ double chunk_y[16], chunk_p[16];
+    const size_t r = (size_t)(i - 1) * (sh.jm - 2) + (j - 1);
+    row_sum_store(ty, chunk_y, rows_y + r);
+    row_sum_store(tp, chunk_p, rows_p + r);
+}
+
+// The consistent source b = src - (y . src) / (y . 1) and the rows of |b|^2, in one pass.
+__global__ void __launch_bounds__(1024) k_make_b_sq_rows(Shape sh, const double* src, const double* sc, double yw,
+                                                         double* b, double* rows) {
+    const int N = sh.km - 1;
+    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
+    const int k = threadIdx.x;
+    double term = 0.0;
+    if (k < N) {
+        const double shift = sc[0] / yw;
+        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+        const double bv = src[x] - shift;
+        b[x] = bv;
+        term = bv * bv;
+    }
+    __shared__ double chunk[16];
+    row_sum_store(term, chunk, rows + (size_t)(i - 1) * (sh.jm - 2) + (j - 1));
+}
+
 // The total of the row sums: partial t (one thread of 256) adds rows t, t + 256, ... in sequence,
 // each wave reduces its 64 partials by the halving tree, and the 4 wave results are reduced by the
 // halving tree. With a control block (the residual norm of the converging solve) it also decides
@@ -436,6 +489,30 @@ __global__ void __launch_bounds__(kDsumParts) k_dsum_final(int nrows, const doub
                 __threadfence_system();
             }
         }
+    }
+}
+
+// Two totals in the order of k_dsum_final (each array on its own), and, when given, the solve's
+// control flags cleared for the iterations that follow.
+__device__ __forceinline__ double total_256(int nrows, const double* rows, double* wsum) {
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int q = t; q < nrows; q += kDsumParts) s += rows[q];
+    for (int h = 32; h > 0; h >>= 1) s += __shfl_down(s, h, 64);
+    if ((t & 63) == 0) wsum[t >> 6] = s;
+    __syncthreads();
+    return (wsum[0] + wsum[2]) + (wsum[1] + wsum[3]);
+}
+__global__ void __launch_bounds__(kDsumParts) k_dsum_final2(int nrows, const double* r1, const double* r2, double* out1,
+                                                            double* out2, int* reset_flags) {
+    __shared__ double w1[kDsumParts / 64], w2[kDsumParts / 64];
+    const double a = total_256(nrows, r1, w1);
+    const double b = total_256(nrows, r2, w2);
+    if (threadIdx.x == 0) {
+        *out1 = a;
+        *out2 = b;
+        if (reset_flags)
+            for (int q = 0; q < CTL_N; q++) reset_flags[q] = 0;
     }
 }
 
@@ -549,6 +626,22 @@ hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, c
     }
     hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(kDsumParts), 0, s, (sh.im - 2) * (sh.jm - 2), (const double*)rows,
                        out, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, double* rows, double* rows2,
+                              double* scal, double yw, double* b, int* flags, double* pprev, bool extrap,
+                              hipStream_t s) {
+    const int N = sh.km - 1;
+    const int threads = ((N + 63) / 64) * 64;
+    if (threads > 1024) return hipErrorInvalidValue;
+    const dim3 grid(sh.jm - 2, sh.im - 2);
+    const int nrows = (sh.im - 2) * (sh.jm - 2);
+    hipLaunchKernelGGL(k_dsum_rows_ysrc_p, grid, dim3(threads), 0, s, sh, p, src, rows, rows2, pprev, extrap ? 1 : 0);
+    hipLaunchKernelGGL(k_dsum_final2, dim3(1), dim3(kDsumParts), 0, s, nrows, (const double*)rows,
+                       (const double*)rows2, scal + 0, scal + 1, flags);
+    hipLaunchKernelGGL(k_make_b_sq_rows, grid, dim3(threads), 0, s, sh, src, (const double*)scal, yw, b, rows);
+    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(kDsumParts), 0, s, nrows, (const double*)rows, scal + 2, SolveCtl());
     return hipGetLastError();
 }
 

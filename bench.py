@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true", help="skip the 1/10/100-step mixed-vs-FP64 comparison")
+    ap.add_argument("--cold-start", action="store_true",
+                    help="start every solve from p_n (default: from the extrapolation 2 p_n - p_{n-1})")
+    ap.add_argument("--no-events", action="store_true",
+                    help="no HIP events in the timed region and no per-class pass (for runs under a profiler)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="window of the adjoint timing (0: skip)")
     ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--refined", action="store_true",
@@ -99,21 +103,26 @@ def initial_state(refine=1):
 PROFILE_CLASSES = ("rk_stage", "line", "baro", "residual", "hydro", "bc", "proj_src")
 
 
-def timed_run(core, steps, warmup, opts, torch):
-    """The timed region: only the stage kernels are bracketed by HIP events (for the roofline)."""
+def timed_run(core, steps, warmup, opts, torch, events=True):
+    """The timed region: only the stage kernels and the whole steps are bracketed by HIP events (for
+    the roofline and the stream time per step)."""
     for _ in range(warmup):
         core.step(1, **opts)
     core.synchronize()
     torch.cuda.synchronize()
-    core.profile_begin("rk_stage")
+    if events:
+        core.profile_begin("rk_stage,step")
     t0 = time.perf_counter()
     st = core.step(steps, **opts)
     core.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    if not events:
+        return el, st, (0.0, 0), 0.0
     rk = core.profile_query("rk_stage")
+    gpu_step_ms = core.profile_query("step")[0] / steps
     core.profile_end()
-    return el, st, rk
+    return el, st, rk, gpu_step_ms
 
 
 def kernel_breakdown(core, steps, opts):
@@ -196,8 +205,8 @@ def refined_bench(args):
     import aturan_amd as A
     g, p, F = initial_state(2)
     n_int = (g.im - 2) * (g.jm - 2) * (g.km - 1)
-    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60)
-    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner)
+    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, warm_start=not args.cold_start)
+    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner, warm_start=not args.cold_start)
     out, fields = {}, {}
 
     def progress(msg):
@@ -299,19 +308,20 @@ def main():
         refined_bench(args)
         return
     g, p, F = initial_state()
-    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60)
-    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner)
+    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, warm_start=not args.cold_start)
+    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner, warm_start=not args.cold_start)
     line = {"metric": "RK4 steps/s at 41x181x361 (plain FP64 GPU step, Poisson to 1e-8)", "value": None,
             "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic"}
     cfg = {"workload": f"41x181x361 shell, seeded synthetic state: hydro prep + RK4 + BC + red-black line/barotropic "
-                       f"Poisson solve to relative residual {args.tol:g}, {args.steps} steps"}
+                       f"Poisson solve to relative residual {args.tol:g}, {args.steps} steps",
+           "solve_start": "p_n" if args.cold_start else "2 p_n - p_(n-1) (linear extrapolation of the last two solutions)"}
     if args.mode in ("both", "fp64"):
         with A.UranusCore(g, p, "fp64") as core:
             core.set_state(F)
-            el, st, (rk_ms, rk_n) = timed_run(core, args.steps, args.warmup, opts64, torch)
-            breakdown = kernel_breakdown(core, min(args.steps, 20), opts64)
+            el, st, (rk_ms, rk_n), gpu_ms = timed_run(core, args.steps, args.warmup, opts64, torch, not args.no_events)
+            breakdown = None if args.no_events else kernel_breakdown(core, min(args.steps, 20), opts64)
         sps = args.steps / el
         achieved = sum(RK_PASSES_FP64) * 8 * N_INT * (rk_n // 4) / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
         line.update(value=round(sps, 3), ms_per_step=round(1e3 * el / args.steps, 4))
@@ -323,11 +333,13 @@ def main():
                             "avg_launch_ms": round(rk_ms / max(rk_n, 1), 5),
                             "bytes_per_launch": sum(RK_PASSES_FP64) * 8 * N_INT / 4}
         line["kernels_ms_per_step"] = breakdown
+        line["gpu_ms_per_step"] = round(gpu_ms, 4)   # stream time of each step (events), gaps included
     if args.mode in ("both", "mixed"):
         with A.UranusCore(g, p, "mixed") as core:
             core.set_state(F)
-            el, st, (rk_ms, rk_n) = timed_run(core, args.steps, args.warmup, optsmx, torch)
-            breakdown = kernel_breakdown(core, min(args.steps, 20), optsmx)
+            el, st, (rk_ms, rk_n), gpu_ms_mx = timed_run(core, args.steps, args.warmup, optsmx, torch,
+                                                        not args.no_events)
+            breakdown = None if args.no_events else kernel_breakdown(core, min(args.steps, 20), optsmx)
         sps = args.steps / el
         achieved = sum(RK_BYTES_MIXED) * N_INT * (rk_n // 4) / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
         mixed = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 * el / args.steps, 4),
@@ -335,7 +347,7 @@ def main():
                  "rk_stage_GBps": round(achieved, 1), "rk_stage_frac": round(achieved / PEAK_HBM_GBS, 4),
                  "solve_iterations_last_step": st.sweeps_done, "solve_refinements_last_step": st.outer_iters,
                  "solve_rel_resid_last_step": st.rel_resid,
-                 "kernels_ms_per_step": breakdown}
+                 "gpu_ms_per_step": round(gpu_ms_mx, 4), "kernels_ms_per_step": breakdown}
         if args.mode == "mixed":
             line.update(value=round(sps, 3), ms_per_step=mixed["ms_per_step"], dtype="f32/f64",
                         metric="RK4 steps/s at 41x181x361 (mixed FP32/FP64 GPU step, Poisson to 1e-8)")

@@ -142,7 +142,12 @@ typedef struct atu_solver_opts {
     int32_t sweeps;          /* REFERENCE: sweeps to run. CONVERGE on a mixed context: FP32
                                 smoothing iterations per FP64 refinement (<= 0: 2)            */
     int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing iterations                 */
-    int32_t pad_;
+    int32_t warm_start;      /* CONVERGE in atu_step: start each solve from the extrapolation
+                                2 p_n - p_{n-1} of the last two solutions (the held mean stays
+                                p_n's). The context forgets p_{n-1} on atu_set_field,
+                                atu_hydro_prep, atu_rk4, atu_apply_bc, atu_pressure_solve and the
+                                window calls; a caller that writes p_dyn through
+                                atu_field_device_ptr passes 0 on its next atu_step. 0: from p_n  */
     double omega;            /* relaxation factor (1 = Gauss-Seidel)                          */
     double rel_tol;          /* CONVERGE: relative residual target (FP64)                     */
 } atu_solver_opts;
@@ -280,7 +285,8 @@ int atu_window_rewind(atu_ctx* ctx);
  * Between atu_profile_begin and atu_profile_query every kernel launch of the context is bracketed
  * by HIP events recorded on the context's stream. atu_profile_query synchronises and returns the
  * summed duration and the launch count of one kernel class: "rk_stage", "rb_pass", "hydro", "bc",
- * "proj_src", "residual", "line", "baro". */
+ * "proj_src", "residual", "line", "baro", "adjoint", or "step" (each whole step of atu_step, from
+ * its first launch to its last on the context's stream, gaps included). */
 int atu_profile_begin(atu_ctx* ctx);
 /* The same for the listed classes only (comma-separated names), leaving the other launches
  * unbracketed: an event pair between two kernels delays the second by a few microseconds. */

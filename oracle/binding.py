@@ -46,6 +46,8 @@ def load() -> C.CDLL:
         "orc_pressure_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_double, C.POINTER(C.c_int),
                                             C.POINTER(C.c_double), _DP]),
         "orc_step": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double]),
+        "orc_step_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_double)]),
     }
     for n, (res, args) in sig.items():
         f = getattr(lib, n)
@@ -137,3 +139,13 @@ def residual(grid, params, fields: np.ndarray, src: np.ndarray) -> tuple[float, 
 
 def step(grid, params, fields: np.ndarray, nsteps: int, sweeps: int, omega: float) -> None:
     _ok(load().orc_step(_p(grid), _p(params), _d(fields), nsteps, sweeps, omega), "orc_step")
+
+
+def step_converge(grid, params, fields: np.ndarray, nsteps: int, max_iters: int, rel_tol: float,
+                  warm: bool) -> tuple[int, float]:
+    """nsteps outer steps with the converging solve (warm: start each solve after the first from
+    2 p_n - p_{n-1}); returns the last solve's iteration count and relative residual."""
+    it, rel = C.c_int(), C.c_double()
+    _ok(load().orc_step_converge(_p(grid), _p(params), _d(fields), nsteps, max_iters, rel_tol, int(warm),
+                                 C.byref(it), C.byref(rel)), "orc_step_converge")
+    return it.value, rel.value

@@ -980,8 +980,11 @@ static void baro_correct(const orc_baro* B, const orc_geo* G, const orc_pgeo* q,
     free(R); free(Ac); free(As);
 }
 
-int orc_pressure_converge(const atu_grid* g, const atu_params* p, double* fields, int max_iters, double rel_tol,
-                          int* iters_out, double* rel_out, double* src_out) {
+/* `pprev` (optional, interior cells): the previous solution. When given, it receives p as the
+ * solve finds it; with `extrap` the iterations start from 2 p - pprev instead of p (the held mean
+ * is p's before the extrapolation). */
+static int converge_impl(const atu_grid* g, const atu_params* p, double* fields, int max_iters, double rel_tol,
+                         int* iters_out, double* rel_out, double* src_out, double* pprev, int extrap) {
     orc_geo G;
     geo_init(&G, g, p);
     orc_pgeo q;
@@ -1008,6 +1011,14 @@ int orc_pressure_converge(const atu_grid* g, const atu_params* p, double* fields
     const double shift = ys / yw;
     const double ncell = (double)ni * (jm - 2) * n;
     const double mean0 = dsum_interior(im, jm, km, pd) / ncell;
+    if (pprev)
+        for (int i = 1; i < im - 1; i++)
+            for (int j = 1; j < jm - 1; j++)
+                for (int k = 0; k < n; k++) {
+                    const double a = pd[IDX(i, j, k)];
+                    if (extrap) pd[IDX(i, j, k)] = 2.0 * a - pprev[IDX(i, j, k)];
+                    pprev[IDX(i, j, k)] = a;
+                }
     memset(b, 0, sizeof(double) * n3);
     for (int i = 1; i < im - 1; i++)
         for (int j = 1; j < jm - 1; j++)
@@ -1047,6 +1058,11 @@ int orc_pressure_converge(const atu_grid* g, const atu_params* p, double* fields
     pgeo_free(&q);
     geo_free(&G);
     return 0;
+}
+
+int orc_pressure_converge(const atu_grid* g, const atu_params* p, double* fields, int max_iters, double rel_tol,
+                          int* iters_out, double* rel_out, double* src_out) {
+    return converge_impl(g, p, fields, max_iters, rel_tol, iters_out, rel_out, src_out, NULL, 0);
 }
 
 /* ------------------------------------------------------------------------------------------- */
@@ -1196,4 +1212,20 @@ int orc_step(const atu_grid* g, const atu_params* p, double* fields, int nsteps,
         if ((e = orc_pressure(g, p, fields, sweeps, omega, NULL))) return e;
     }
     return 0;
+}
+
+/* The same outer step with the converging solve; with `warm`, every solve after the first starts
+ * from the linear extrapolation 2 p_n - p_{n-1} of the last two solutions. iters_out / rel_out:
+ * the last solve's. */
+int orc_step_converge(const atu_grid* g, const atu_params* p, double* fields, int nsteps, int max_iters,
+                      double rel_tol, int warm, int* iters_out, double* rel_out) {
+    const size_t n3 = (size_t)g->im * g->jm * g->km;
+    double* pprev = warm ? calloc(n3, sizeof(double)) : NULL;
+    int e = 0;
+    for (int s = 0; s < nsteps && !e; s++) {
+        if ((e = orc_hydro(g, p, fields, NULL)) || (e = orc_rk4(g, p, fields)) || (e = orc_bc(g, p, fields))) break;
+        e = converge_impl(g, p, fields, max_iters, rel_tol, iters_out, rel_out, NULL, pprev, warm && s > 0);
+    }
+    free(pprev);
+    return e;
 }

@@ -189,3 +189,23 @@ def test_full_step_matches_oracle(setup, oracle):
     oracle.step(g, p, F, 2, 20, 1.5)
     errs = {n: rel_err(G[A.FIELD[n]], F[A.FIELD[n]]) for n in PROG + ["p_dyn", "p_hydro"]}
     assert report("2 steps", errs) <= TOL
+
+
+@pytest.mark.parametrize("warm", [False, True])
+def test_converge_steps_match_oracle(setup, oracle, warm):
+    """The benchmark's outer step (hydro, RK4, BC, converging solve to 1e-8), three steps in one call
+    and one more in a second call; with warm, every solve after the first starts from 2 p_n - p_{n-1}."""
+    print(f"[case] four converging steps, warm start {warm}", flush=True)
+    g, p, F0 = setup
+    F = F0.copy()
+    with A.UranusCore(g, p) as core:
+        core.set_state(F)
+        st3 = core.step(3, "converge", rel_tol=1e-8, max_sweeps=50, warm_start=warm)
+        st4 = core.step(1, "converge", rel_tol=1e-8, max_sweeps=50, warm_start=warm)
+        G = core.get_state()
+    it, rel = oracle.step_converge(g, p, F, 4, 50, 1e-8, warm)
+    errs = {n: rel_err(G[A.FIELD[n]], F[A.FIELD[n]]) for n in PROG + ["p_dyn", "p_hydro"]}
+    print(f"[parity] iterations: gpu step 3 {st3.sweeps_done}, step 4 {st4.sweeps_done}; oracle step 4 {it}; "
+          f"rel {st4.rel_resid:.3e} / {rel:.3e}", flush=True)
+    assert report(f"4 converging steps warm={warm}", errs) == 0.0
+    assert st4.sweeps_done == it and st4.rel_resid == rel and rel <= 1e-8
