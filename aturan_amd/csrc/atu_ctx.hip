@@ -599,7 +599,7 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         // The FP64 residual of p, kept in FP32 as the right-hand side of the correction, and its
         // test; then refinements, each `inner` FP32 iterations on the correction e, p += e in FP64,
         // and the next residual with its test.
-        const int inner = o->sweeps > 0 ? o->sweeps : 2;
+        const int inner = o->sweeps > 0 ? o->sweeps : 1;
         const int max_ref = max_it / inner;
         const SolveCtl first{F, c->SCAL + 2, o->rel_tol, 0, 0, c->MBOX_DEV};
         const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, inner, 1, c->MBOX_DEV};

@@ -167,6 +167,10 @@ __device__ __forceinline__ void fz_put(const BcCtx& b, size_t q, double val) {
     if (b.t.frozen32) b.t.frozen32[q] = (float)val;
 }
 
+// The BC kernels take one field per blockIdx.z: the prognostic fields f0 + z, then (with `rest`)
+// the four frozen fields. The fields do not interact in the BC, and each field's own forms follow
+// its NEUMANN form in the same thread, as in the reference's order.
+
 // bcRadius with margin 0 (every j, every k), NEUMANN form, then the rigid lid on u, the species
 // floor (floorRadialSpecies) and the lid form of v and w (BC_LID_VW).
 __global__ void __launch_bounds__(256) k_bc_radius(BcCtx b) {
@@ -177,48 +181,42 @@ __global__ void __launch_bounds__(256) k_bc_radius(BcCtx b) {
     const size_t col = (size_t)j * sh.ldk + k;
     const size_t P = sh.plane;
     const int L = sh.im - 1;
-    for (int f = b.f0; f < b.f1; f++) {
-        const double* F = b.t.y + (size_t)f * sh.fs + col;
-        const double lo = b.c43 * F[1 * P] - b.c13 * F[2 * P];
-        const double hi = b.c43 * F[(size_t)(L - 1) * P] - b.c13 * F[(size_t)(L - 2) * P];
-        bc_put(b, (size_t)f * sh.fs + col, lo);
-        bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, hi);
-    }
-    if (!b.rest) return;
-    for (int q = 0; q < 4; q++) {
-        const size_t o = (size_t)frozen_bc(q) * sh.fs + col;
+    const int z = blockIdx.z, nf = b.f1 - b.f0;
+    if (z >= nf) {   // a frozen field
+        const size_t o = (size_t)frozen_bc(z - nf) * sh.fs + col;
         const double* F = b.t.frozen + o;
         const double lo = b.c43 * F[1 * P] - b.c13 * F[2 * P];
         const double hi = b.c43 * F[(size_t)(L - 1) * P] - b.c13 * F[(size_t)(L - 2) * P];
         fz_put(b, o, lo);
         fz_put(b, o + (size_t)L * P, hi);
+        return;
     }
-    if (b.rigid_lid) {
+    const int f = b.f0 + z;
+    const double* F = b.t.y + (size_t)f * sh.fs + col;
+    const double lo = b.c43 * F[1 * P] - b.c13 * F[2 * P];
+    const double hi = b.c43 * F[(size_t)(L - 1) * P] - b.c13 * F[(size_t)(L - 2) * P];
+    bc_put(b, (size_t)f * sh.fs + col, lo);
+    bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, hi);
+    if (!b.rest) return;
+    if (f == ATU_U && b.rigid_lid) {
         bc_put(b, (size_t)ATU_U * sh.fs + col, 0.0);
         bc_put(b, (size_t)ATU_U * sh.fs + col + (size_t)L * P, 0.0);
     }
-    if (b.radius_positive) {
-        for (int f = ATU_CH4; f <= ATU_NH4SH; f++) {
-            const double* F = b.t.y + (size_t)f * sh.fs + col;
-            const double f0 = F[0], fl = F[(size_t)L * P];
-            if (f0 < 0.0) {
-                const double f1 = F[P];
-                bc_put(b, (size_t)f * sh.fs + col, b.radius_positive == 2 ? (0.0 < f1 ? f1 : 0.0) : 0.0);
-            }
-            if (fl < 0.0) {
-                const double f1 = F[(size_t)(L - 1) * P];
-                bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P,
-                       b.radius_positive == 2 ? (0.0 < f1 ? f1 : 0.0) : 0.0);
-            }
+    if (b.radius_positive && f >= ATU_CH4 && f <= ATU_NH4SH) {
+        const double f0 = F[0], fl = F[(size_t)L * P];
+        if (f0 < 0.0) {
+            const double f1 = F[P];
+            bc_put(b, (size_t)f * sh.fs + col, b.radius_positive == 2 ? (0.0 < f1 ? f1 : 0.0) : 0.0);
+        }
+        if (fl < 0.0) {
+            const double f1 = F[(size_t)(L - 1) * P];
+            bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, b.radius_positive == 2 ? (0.0 < f1 ? f1 : 0.0) : 0.0);
         }
     }
-    if (b.lid_vw == 2 || b.lid_vw == 3) {
-        for (int f = ATU_V; f <= ATU_W; f++) {
-            const double* F = b.t.y + (size_t)f * sh.fs + col;
-            const double fa = F[(size_t)(L - 1) * P], fb = F[(size_t)(L - 2) * P], fc = F[(size_t)(L - 3) * P];
-            const double val = (b.lid_vw == 2) ? 2.0 * fa - fb : fc - 3.0 * fb + 3.0 * fa;
-            bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, val);
-        }
+    if ((b.lid_vw == 2 || b.lid_vw == 3) && (f == ATU_V || f == ATU_W)) {
+        const double fa = F[(size_t)(L - 1) * P], fb = F[(size_t)(L - 2) * P], fc = F[(size_t)(L - 3) * P];
+        const double val = (b.lid_vw == 2) ? 2.0 * fa - fb : fc - 3.0 * fb + 3.0 * fa;
+        bc_put(b, (size_t)f * sh.fs + col + (size_t)L * P, val);
     }
 }
 
@@ -231,25 +229,25 @@ __global__ void __launch_bounds__(256) k_bc_theta(BcCtx b) {
     const int ldk = sh.ldk;
     const int Jl = sh.jm - 1;
     const size_t base = (size_t)i * sh.plane + k;
-    for (int f = b.f0; f < b.f1; f++) {
-        const size_t q0 = (size_t)f * sh.fs + base;
-        if (f == ATU_V || f == ATU_W) {
-            bc_put(b, q0, 0.0);
-            bc_put(b, q0 + (size_t)Jl * ldk, 0.0);
-            continue;
-        }
-        const double* F = b.t.y + q0;
-        const double lo = b.c43 * F[(size_t)1 * ldk] - b.c13 * F[(size_t)2 * ldk];
-        const double hi = b.c43 * F[(size_t)(Jl - 1) * ldk] - b.c13 * F[(size_t)(Jl - 2) * ldk];
-        bc_put(b, q0, lo);
-        bc_put(b, q0 + (size_t)Jl * ldk, hi);
-    }
-    if (!b.rest) return;
-    for (int q = 0; q < 4; q++) {
-        const size_t o = (size_t)frozen_bc(q) * sh.fs + base;
+    const int z = blockIdx.z, nf = b.f1 - b.f0;
+    if (z >= nf) {
+        const size_t o = (size_t)frozen_bc(z - nf) * sh.fs + base;
         fz_put(b, o, 0.0);
         fz_put(b, o + (size_t)Jl * ldk, 0.0);
+        return;
     }
+    const int f = b.f0 + z;
+    const size_t q0 = (size_t)f * sh.fs + base;
+    if (f == ATU_V || f == ATU_W) {
+        bc_put(b, q0, 0.0);
+        bc_put(b, q0 + (size_t)Jl * ldk, 0.0);
+        return;
+    }
+    const double* F = b.t.y + q0;
+    const double lo = b.c43 * F[(size_t)1 * ldk] - b.c13 * F[(size_t)2 * ldk];
+    const double hi = b.c43 * F[(size_t)(Jl - 1) * ldk] - b.c13 * F[(size_t)(Jl - 2) * ldk];
+    bc_put(b, q0, lo);
+    bc_put(b, q0 + (size_t)Jl * ldk, hi);
 }
 
 // bcPhi with the periodic seam: k = km-1 is the copy of k = 0.
@@ -259,15 +257,14 @@ __global__ void __launch_bounds__(256) k_bc_phi(BcCtx b) {
     if (j >= b.sh.jm) return;
     const Shape& sh = b.sh;
     const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
-    for (int f = b.f0; f < b.f1; f++) {
-        const size_t q = (size_t)f * sh.fs + row;
-        bc_put(b, q + sh.km - 1, b.t.y[q]);
-    }
-    if (!b.rest) return;
-    for (int q = 0; q < 4; q++) {
-        const size_t o = (size_t)frozen_bc(q) * sh.fs + row;
+    const int z = blockIdx.z, nf = b.f1 - b.f0;
+    if (z >= nf) {
+        const size_t o = (size_t)frozen_bc(z - nf) * sh.fs + row;
         fz_put(b, o + sh.km - 1, b.t.frozen[o]);
+        return;
     }
+    const size_t q = (size_t)(b.f0 + z) * sh.fs + row;
+    bc_put(b, q + sh.km - 1, b.t.y[q]);
 }
 
 // clampNegativeSpecies (planet/BC_Uran.h:110-128): every species value that is not >= 0 -> 0.
@@ -334,9 +331,11 @@ hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets&
     b.f0 = f0 < 0 ? 0 : f0;
     b.f1 = f1 > kNumBcFields ? kNumBcFields : f1;
     b.rest = rest ? 1 : 0;
-    hipLaunchKernelGGL(k_bc_radius, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, b);
-    hipLaunchKernelGGL(k_bc_theta, dim3((sh.km + 255) / 256, sh.im), dim3(256), 0, s, b);
-    hipLaunchKernelGGL(k_bc_phi, dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, b);
+    const int nz = (b.f1 - b.f0) + (rest ? 4 : 0);
+    if (nz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bc_radius, dim3((sh.km + 255) / 256, sh.jm, nz), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_bc_theta, dim3((sh.km + 255) / 256, sh.im, nz), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(k_bc_phi, dim3((sh.jm + 255) / 256, sh.im, nz), dim3(256), 0, s, b);
     if (rest && p.species_clamp)
         hipLaunchKernelGGL(k_species_clamp, dim3((sh.km + 255) / 256, sh.jm, sh.im), dim3(256), 0, s, b);
     return hipGetLastError();

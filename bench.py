@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tol", type=float, default=1e-8, help="relative residual of the Poisson solve")
-    ap.add_argument("--inner", type=int, default=2, help="FP32 smoothing iterations per refinement (mixed)")
+    ap.add_argument("--inner", type=int, default=1, help="FP32 smoothing iterations per refinement (mixed)")
     ap.add_argument("--mode", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -109,13 +109,15 @@ def timed_run(core, steps, warmup, opts, torch, events=True):
     for _ in range(warmup):
         core.step(1, **opts)
     core.synchronize()
-    torch.cuda.synchronize()
+    if torch is not None:
+        torch.cuda.synchronize()
     if events:
         core.profile_begin("rk_stage,step")
     t0 = time.perf_counter()
     st = core.step(steps, **opts)
     core.synchronize()
-    torch.cuda.synchronize()
+    if torch is not None:
+        torch.cuda.synchronize()
     el = time.perf_counter() - t0
     if not events:
         return el, st, (0.0, 0), 0.0
@@ -164,7 +166,7 @@ def mixed_errors(g, p, F, opts64, optsmx):
     return res
 
 
-def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s):
+def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s, events=True):
     """Forward window (with checkpoints) and hand-written adjoint (recompute + reverse) timings."""
     import aturan_amd as A
     with A.UranusCore(g, p, "fp64") as core:
@@ -181,12 +183,13 @@ def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s):
         core.synchronize()
         t2 = time.perf_counter()
         prof_ms = None
-        core.window_rewind()
-        core.window_forward(nsteps, checkpoint_every=K, rel_tol=tol)
-        core.profile_begin()
-        core.window_backward(1.0)
-        prof_ms = core.profile_query("adjoint")[0]
-        core.profile_end()
+        if events:
+            core.window_rewind()
+            core.window_forward(nsteps, checkpoint_every=K, rel_tol=tol)
+            core.profile_begin()
+            core.window_backward(1.0)
+            prof_ms = core.profile_query("adjoint")[0]
+            core.profile_end()
     fwd, bwd = t1 - t0, t2 - t1
     return {"window_steps": nsteps, "checkpoint_every": K, "loss": loss, "forward_s": round(fwd, 4),
             "backward_s": round(bwd, 4), "backward_over_forward": round(bwd / fwd, 3),
@@ -301,7 +304,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    import torch
+    # under a profiler (--no-events) the run leaves torch out: rocprofv3's kernel tracing with torch's
+    # runtime initialised in the same process has given wrong results from this library's kernels
+    torch = None
+    if not args.no_events:
+        import torch
     import aturan_amd as A
 
     if args.refined:
@@ -356,7 +363,7 @@ def main():
         line["mixed"] = mixed
     if args.adjoint_steps > 0 and args.mode in ("both", "fp64"):
         line["adjoint"] = adjoint_bench(g, p, F, args.adjoint_steps, args.checkpoint_every, args.tol,
-                                        line["ms_per_step"] / 1e3 if line["ms_per_step"] else None)
+                                        line["ms_per_step"] / 1e3 if line["ms_per_step"] else None, not args.no_events)
     line["config"] = cfg
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds)

@@ -140,7 +140,7 @@ typedef enum atu_solve_mode {
 typedef struct atu_solver_opts {
     int32_t mode;            /* atu_solve_mode                                                */
     int32_t sweeps;          /* REFERENCE: sweeps to run. CONVERGE on a mixed context: FP32
-                                smoothing iterations per FP64 refinement (<= 0: 2)            */
+                                smoothing iterations per FP64 refinement (<= 0: 1)            */
     int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing iterations                 */
     int32_t warm_start;      /* CONVERGE in atu_step: start each solve from the extrapolation
                                 2 p_n - p_{n-1} of the last two solutions (the held mean stays
