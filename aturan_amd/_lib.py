@@ -106,6 +106,7 @@ _SIGS = {
     "atu_window_backward": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
     "atu_window_rewind": (C.c_int, [C.c_void_p]),
     "atu_build_info": (C.c_char_p, []),
+    "atu_damp_wiggles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
     "atu_profile_begin": (C.c_int, [C.c_void_p]),
     "atu_profile_begin_classes": (C.c_int, [C.c_void_p, C.c_char_p]),
     "atu_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.POINTER(C.c_int)]),

@@ -163,6 +163,13 @@ class UranusCore:
         _check(self._lib.atu_step(self._ctx, nsteps, C.byref(o), C.byref(st)), self._ctx, "atu_step")
         return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)
 
+    def damp_wiggles(self, field: str | int, along_i: bool, along_j: bool, along_k: bool, strength: float = 1.0,
+                     passes: int = 1) -> None:
+        """AtomUtils::damp_wiggles on one prognostic field (atu_damp_wiggles)."""
+        f = L.FIELD[field] if isinstance(field, str) else int(field)
+        _check(self._lib.atu_damp_wiggles(self._ctx, f, int(along_i), int(along_j), int(along_k), strength, passes),
+               self._ctx, "atu_damp_wiggles")
+
     def pressure_residual(self) -> tuple[float, float]:
         r, s = C.c_double(), C.c_double()
         _check(self._lib.atu_pressure_residual(self._ctx, C.byref(r), C.byref(s)), self._ctx,

@@ -67,6 +67,7 @@ struct atu_ctx {
     double* SROWS = nullptr;  // (im-2)*(jm-2)
     double* SROWS2 = nullptr; // the same, for a second sum of one pass
     double* PPREV = nullptr;  // p_dyn as the last warm-started solve found it (interior)
+    double* DAMP = nullptr;   // scratch field of the wiggle filter
     unsigned long state_epoch = 0, warm_epoch = ~0ul;   // warm start valid iff equal
     double* SCAL = nullptr;   // 8 scalars: 0 y.src, 1 mean before, 2 |b|^2, 3 |r|^2, 4 mean after
     double* BR = nullptr;     // J*N
@@ -160,7 +161,7 @@ struct atu_ctx {
 };
 
 atu_ctx::~atu_ctx() {
-    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SROWS2, PPREV, SCAL,
+    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SROWS2, PPREV, DAMP, SCAL,
                       BR, BAC, BAS})
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
@@ -1142,6 +1143,25 @@ int atu_pressure_solve(atu_ctx* ctx, const atu_solver_opts* opts, atu_solve_stat
     if (opts && !(opts->omega > 0.0 && opts->omega < 2.0))
         return ctx->fail(ATU_EINVAL, "atu_pressure_solve: omega must lie in (0, 2)");
     return do_pressure(ctx, opts, stats);
+}
+
+int atu_damp_wiggles(atu_ctx* ctx, int field, int along_i, int along_j, int along_k, double strength, int passes) {
+    if (!ctx) return ATU_EINVAL;
+    if (field < 0 || field >= ATU_NPROG) return ctx->fail(ATU_EINVAL, "atu_damp_wiggles: field must be prognostic");
+    if (passes < 0) return ctx->fail(ATU_EINVAL, "atu_damp_wiggles: passes must be >= 0");
+    ctx->state_epoch++;
+    if (!ctx->DAMP && ctx->hip(hipMalloc((void**)&ctx->DAMP, ctx->sh.fs * sizeof(double)), "hipMalloc(damp)"))
+        return ATU_ENOMEM;
+    double* f = ctx->prog(ctx->Y, field);
+    if (ctx->hip(launch_damp_wiggles(ctx->sh, f, ctx->DAMP, along_i != 0, along_j != 0, along_k != 0, strength,
+                                     passes, ctx->stream),
+                 "damp_wiggles"))
+        return ATU_EHIP;
+    // the FP32 mirror the mixed path's first stage reads
+    if (ctx->mixed() && ctx->hip(launch_convert(f, ctx->Y32 + (size_t)field * ctx->sh.fs, ctx->sh.fs, ctx->stream),
+                                 "convert"))
+        return ATU_EHIP;
+    return ATU_OK;
 }
 
 int atu_pressure_residual(atu_ctx* ctx, double* rel, double* srcn) {

@@ -164,6 +164,11 @@ template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
                             hipStream_t s);
 
+// ---- the Shapiro wiggle filter (atu_damp.hip): AtomUtils::damp_wiggles on one field; `scratch`
+// holds a field (its contents are not kept)
+hipError_t launch_damp_wiggles(const Shape& sh, double* field, double* scratch, bool along_i, bool along_j,
+                               bool along_k, double strength, int passes, hipStream_t s);
+
 // ---- PressureSolverUran (atu_uran.hip), the reference's default solver ----
 struct UranTabs {
     const double *a_inv_rm, *a_num2, *m_num2;                                  // [im]

@@ -273,27 +273,63 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_col_resid_par(Shape sh, Dev
     }
 }
 
+// The DFT tables depend on (m k mod N) only: ct[m][k] = ct[1][(m k) mod N], the same bits (the host
+// builds every entry from the reduced angle). The DFT kernels stage that one row of N cosines and
+// sines in LDS with the rows of data they need, and index it with the running (m k) mod N.
+constexpr int kDftRows = 4, kDftMaxN = 1024;
+
+// Copy n values src(q) -> dst[q] with a block of nt threads, kStageBatch loads in flight per thread
+// (the loads of a batch are issued before any of its stores).
+constexpr int kStageBatch = 8;
+template <typename Src>
+__device__ __forceinline__ void stage_lds(double* dst, int n, int tid, int nt, Src src) {
+    for (int q0 = tid; q0 < n; q0 += kStageBatch * nt) {
+        double v[kStageBatch];
+#pragma unroll
+        for (int u = 0; u < kStageBatch; u++) {
+            const int q = q0 + u * nt;
+            v[u] = q < n ? src(q) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kStageBatch; u++) {
+            const int q = q0 + u * nt;
+            if (q < n) dst[q] = v[u];
+        }
+    }
+}
+
 // forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin, each sum
-// over k in order. A block is 64 modes x kDftRows rows, one wave per row: the waves read the same
-// table values, which the CU's cache serves after the first.
-constexpr int kDftRows = 4;
+// over k in order. A block is 64 modes x kDftRows rows (one wave per row).
 __global__ void __launch_bounds__(64 * kDftRows) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As,
                                                            const int* done) {
     if (done && *done) return;
+    __shared__ double c1[kDftMaxN], s1[kDftMaxN], rr[kDftRows][kDftMaxN];
+    const int N = t.N, M1 = t.NM + 1, J = t.J;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int jj0 = blockIdx.y * kDftRows;
+    const int nt = 64 * kDftRows;
+    stage_lds(c1, N, tid, nt, [&](int q) { return t.ct[(size_t)N + q]; });
+    stage_lds(s1, N, tid, nt, [&](int q) { return t.st[(size_t)N + q]; });
+    stage_lds(&rr[0][0], kDftRows * kDftMaxN, tid, nt, [&](int q) {
+        const int r = q / kDftMaxN, k = q - r * kDftMaxN;
+        return (jj0 + r < J && k < N) ? R[(size_t)(jj0 + r) * N + k] : 0.0;
+    });
+    __syncthreads();
     const int m = blockIdx.x * 64 + threadIdx.x;
-    const int jj = blockIdx.y * kDftRows + __builtin_amdgcn_readfirstlane(threadIdx.y);
-    if (m > t.NM || jj >= t.J) return;
-    const int N = t.N, M1 = t.NM + 1;
-    const double* r = R + (size_t)jj * N;
+    const int r = threadIdx.y;
+    if (m >= M1 || jj0 + r >= J) return;
     double sc = 0.0, ss = 0.0;
+    int q = 0;   // (m k) mod N
 #pragma unroll 8
     for (int k = 0; k < N; k++) {
-        const double rv = r[k];
-        sc += rv * t.ctT[(size_t)k * M1 + m];
-        ss += rv * t.stT[(size_t)k * M1 + m];
+        const double rv = rr[r][k];
+        sc += rv * c1[q];
+        ss += rv * s1[q];
+        q += m;
+        if (q >= N) q -= N;
     }
-    Ac[(size_t)m * t.J + jj] = sc;
-    As[(size_t)m * t.J + jj] = ss;
+    Ac[(size_t)m * J + jj0 + r] = sc;
+    As[(size_t)m * J + jj0 + r] = ss;
 }
 
 // Per mode and part: the folded theta system, mode 0 pinned at j = 1. One 64-lane block per
@@ -323,6 +359,7 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
     if (threadIdx.x == 0) {
         const double e = t.e;
         double prev = 0.0;
+#pragma unroll 8
         for (int jj = 0; jj < J; jj++) {
             double a = (jj == 0) ? 0.0 : e;
             if (jj == J - 1) a = e - c13 * e;
@@ -333,6 +370,7 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
             prev = dp;
         }
         double x = sd[J - 1];
+#pragma unroll 8
         for (int jj = J - 2; jj >= 0; jj--) {
             x = sd[jj] - scp[jj] * x;
             sd[jj] = x;
@@ -342,22 +380,43 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
     for (int jj = threadIdx.x; jj < J; jj += blockDim.x) d[jj] = sd[jj];
 }
 
-// inverse DFT of the correction: Q[jj][k] = (sum over modes in order) / N; a block is 64 k x
-// kDftRows rows, one wave per row, the waves sharing the table values through the CU's cache. Q may
-// overwrite the column residual.
+// inverse DFT of the correction: Q[jj][k] = (Ac[0] c_0 + sum_{m=1}^{NM-1} 2 (Ac c_m + As s_m) +
+// Ac[NM] c_NM) / N, the modes in order; a block is 64 k x kDftRows rows (one wave per row), with the
+// coefficients of its rows and the one table row in LDS. Q may overwrite the column residual.
+constexpr int kDftMaxM = 1024;
 __global__ void __launch_bounds__(64 * kDftRows) k_dft_inv(DevSolve t, const double* Ac, const double* As, double* Q,
                                                            const int* done) {
     if (done && *done) return;
+    __shared__ double c1[kDftMaxN], s1[kDftMaxN], ta[kDftRows][kDftMaxM / 2 + 1], tb[kDftRows][kDftMaxM / 2 + 1];
     const int N = t.N, NM = t.NM, J = t.J;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int jj0 = blockIdx.y * kDftRows;
+    const int nt = 64 * kDftRows;
+    constexpr int MW = kDftMaxM / 2 + 1;
+    stage_lds(c1, N, tid, nt, [&](int q) { return t.ct[(size_t)N + q]; });
+    stage_lds(s1, N, tid, nt, [&](int q) { return t.st[(size_t)N + q]; });
+    stage_lds(&ta[0][0], kDftRows * MW, tid, nt, [&](int q) {
+        const int r = q / MW, m = q - r * MW;
+        return (jj0 + r < J && m <= NM) ? Ac[(size_t)m * J + jj0 + r] : 0.0;
+    });
+    stage_lds(&tb[0][0], kDftRows * MW, tid, nt, [&](int q) {
+        const int r = q / MW, m = q - r * MW;
+        return (jj0 + r < J && m <= NM) ? As[(size_t)m * J + jj0 + r] : 0.0;
+    });
+    __syncthreads();
     const int k = blockIdx.x * 64 + threadIdx.x;
-    const int jj = blockIdx.y * kDftRows + __builtin_amdgcn_readfirstlane(threadIdx.y);
-    if (k >= N || jj >= J) return;
-    double s = Ac[jj] * t.ct[k];
-#pragma unroll 8
-    for (int m = 1; m < NM; m++)
-        s += 2.0 * (Ac[(size_t)m * J + jj] * t.ct[(size_t)m * N + k] + As[(size_t)m * J + jj] * t.st[(size_t)m * N + k]);
-    s += Ac[(size_t)NM * J + jj] * t.ct[(size_t)NM * N + k];
-    Q[(size_t)jj * N + k] = s / N;
+    const int r = threadIdx.y;
+    if (k >= N || jj0 + r >= J) return;
+    double s = ta[r][0] * c1[0];
+    int q = k;   // (m k) mod N
+#pragma unroll 4
+    for (int m = 1; m < NM; m++) {
+        s += 2.0 * (ta[r][m] * c1[q] + tb[r][m] * s1[q]);
+        q += k;
+        if (q >= N) q -= N;
+    }
+    s += ta[r][NM] * c1[q];
+    Q[(size_t)(jj0 + r) * N + k] = s / N;
 }
 
 // the correction added to every interior level: p += Q on the interior
@@ -585,6 +644,7 @@ template <typename P, typename B>
 hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
                        double* As, hipStream_t s, const int* done) {
     if (t.J > kBaroMaxJ) return hipErrorInvalidValue;   // the theta solves stage a mode in LDS
+    if (t.N > kDftMaxN || t.NM + 1 > kDftMaxM / 2 + 1 || t.NM < 1) return hipErrorInvalidValue;   // the DFTs' LDS
     const int N = sh.km - 1;
     const size_t lds = (size_t)(sh.im - 2) * 64 * sizeof(double);
     if (lds <= 48 * 1024)

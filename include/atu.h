@@ -241,6 +241,14 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
  * (FP64; the source must have been prepared by a solve). */
 int atu_pressure_residual(atu_ctx* ctx, double* rel_resid, double* src_norm);
 
+/* Shapiro (1-2-1) wiggle damping of one prognostic field: AtomUtils::damp_wiggles
+ * (lib/Utils.cpp:224-306) with i_surface = nullptr, as the reference's loop calls it after the RK
+ * step on t (planet/cUranusModel.cpp:1218-1220) and on u, v, w (:1232-1236): `passes` times along
+ * k (periodic over km-1 meridians, k = km-1 the copy of k = 0), along j (clamped), along i
+ * (clamped), f + strength/4 (f[n-1] - 2 f + f[n+1]). The reference's loop applies the domain BC
+ * after it (atu_apply_bc). */
+int atu_damp_wiggles(atu_ctx* ctx, int field, int along_i, int along_j, int along_k, double strength, int passes);
+
 /* Device pointer of a field's storage (for zero-copy interop); `ld_k` receives the k stride in
  * elements, `elem_bytes` 8 or 4. The layout is [im][jm][ld_k]. */
 int atu_field_device_ptr(atu_ctx* ctx, int field, void** dptr, int* ld_k, int* elem_bytes);

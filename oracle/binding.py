@@ -46,6 +46,7 @@ def load() -> C.CDLL:
         "orc_pressure_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_double, C.POINTER(C.c_int),
                                             C.POINTER(C.c_double), _DP]),
         "orc_step": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double]),
+        "orc_damp_wiggles": (C.c_int, [V, _DP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
         "orc_step_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
                                         C.POINTER(C.c_double)]),
     }
@@ -149,3 +150,10 @@ def step_converge(grid, params, fields: np.ndarray, nsteps: int, max_iters: int,
     _ok(load().orc_step_converge(_p(grid), _p(params), _d(fields), nsteps, max_iters, rel_tol, int(warm),
                                  C.byref(it), C.byref(rel)), "orc_step_converge")
     return it.value, rel.value
+
+
+def damp_wiggles(grid, fields: np.ndarray, field: int, along_i: bool, along_j: bool, along_k: bool,
+                 strength: float = 1.0, passes: int = 1) -> None:
+    """AtomUtils::damp_wiggles on one field (no fluid floor, periodic phi seam)."""
+    _ok(load().orc_damp_wiggles(_p(grid), _d(fields), field, int(along_i), int(along_j), int(along_k), strength,
+                                passes), "orc_damp_wiggles")

@@ -1214,6 +1214,62 @@ int orc_step(const atu_grid* g, const atu_params* p, double* fields, int nsteps,
     return 0;
 }
 
+/* ------------------------------------------------------------------------------------------- */
+/* Shapiro (1-2-1) wiggle damping, AtomUtils::damp_wiggles (lib/Utils.cpp:224-306), without a fluid */
+/* floor (i_surface = nullptr, as every call in the reference's loop) and with the periodic seam   */
+/* the model sets (planet/cUranusModel.cpp:459): per pass along k (k = 0..km-2 over the ring of    */
+/* km-1 meridians, then k = km-1 the copy of k = 0), along j (clamped at the poles), along i       */
+/* (clamped), each from a copy of the field: f + strength/4 (f[n-1] - 2 f + f[n+1]).              */
+/* ------------------------------------------------------------------------------------------- */
+int orc_damp_wiggles(const atu_grid* g, double* fields, int field, int along_i, int along_j, int along_k,
+                     double strength, int passes) {
+    const int im = g->im, jm = g->jm, km = g->km;
+    const size_t n3 = (size_t)im * jm * km;
+    if (field < 0 || field >= ATU_NFIELDS || km < 3) return 1;
+    double* f = fields + (size_t)field * n3;
+    double* tmp = malloc(sizeof(double) * n3);
+    const double coeff = strength * 0.25;
+    for (int pass = 0; pass < passes; pass++) {
+        if (along_k) {
+            memcpy(tmp, f, sizeof(double) * n3);
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int i = 0; i < im; i++)
+                for (int j = 0; j < jm; j++) {
+                    for (int k = 0; k < km - 1; k++) {
+                        const int km1 = k == 0 ? km - 2 : k - 1, kp1 = k == km - 2 ? 0 : k + 1;
+                        f[IDX(i, j, k)] = tmp[IDX(i, j, k)] + coeff * (tmp[IDX(i, j, km1)] - 2.0 * tmp[IDX(i, j, k)]
+                                                                      + tmp[IDX(i, j, kp1)]);
+                    }
+                    f[IDX(i, j, km - 1)] = f[IDX(i, j, 0)];
+                }
+        }
+        if (along_j) {
+            memcpy(tmp, f, sizeof(double) * n3);
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int i = 0; i < im; i++)
+                for (int j = 0; j < jm; j++) {
+                    const int jm1 = j > 0 ? j - 1 : 0, jp1 = j < jm - 1 ? j + 1 : jm - 1;
+                    for (int k = 0; k < km; k++)
+                        f[IDX(i, j, k)] = tmp[IDX(i, j, k)] + coeff * (tmp[IDX(i, jm1, k)] - 2.0 * tmp[IDX(i, j, k)]
+                                                                      + tmp[IDX(i, jp1, k)]);
+                }
+        }
+        if (along_i) {
+            memcpy(tmp, f, sizeof(double) * n3);
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int j = 0; j < jm; j++)
+                for (int k = 0; k < km; k++)
+                    for (int i = 0; i < im; i++) {
+                        const int im1 = i > 0 ? i - 1 : 0, ip1 = i < im - 1 ? i + 1 : im - 1;
+                        f[IDX(i, j, k)] = tmp[IDX(i, j, k)] + coeff * (tmp[IDX(im1, j, k)] - 2.0 * tmp[IDX(i, j, k)]
+                                                                      + tmp[IDX(ip1, j, k)]);
+                    }
+        }
+    }
+    free(tmp);
+    return 0;
+}
+
 /* The same outer step with the converging solve; with `warm`, every solve after the first starts
  * from the linear extrapolation 2 p_n - p_{n-1} of the last two solutions. iters_out / rel_out:
  * the last solve's. */

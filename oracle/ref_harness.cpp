@@ -10,6 +10,8 @@
 //   mode step     `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolver)
 //   mode rk4_uran restoreVar(1) + RungeKuttaUran + PressureSolverUran::run (the default solver)
 //   mode step_uran `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolverUran)
+//   mode damp_t   AtomUtils::damp_wiggles on t: along j and k (strength 1), then along i (0.5)
+//   mode damp_uvw AtomUtils::damp_wiggles on u, v, w along i, j and k, strength 1, `sweeps` passes
 // in/out: int32 im, jm, km, nfields, then nfields dense [im][jm][km] FP64 arrays in the atu_field
 // order of include/atu.h; out appends im doubles of buoy_ref_level.
 
@@ -126,6 +128,8 @@ int main(int argc, char** argv) {
         BC_Uran(m).bcTheta();
         BC_Uran(m).bcPhi();
     };
+    // the model sets the wiggle filter's seam convention in Run() (planet/cUranusModel.cpp:459)
+    AtomUtils::set_phi_seam_periodic(ATPhys::phi_periodic<cUranusModel>());
     if (mode == "bc") {
         bc();
     } else if (mode == "rk4") {
@@ -151,6 +155,11 @@ int main(int argc, char** argv) {
             bc();
             PressureSolverUran(m).run();
         }
+    } else if (mode == "damp_t") {
+        AtomUtils::damp_wiggles(m.t, nullptr, false, true, true, 1.0, 1);
+        AtomUtils::damp_wiggles(m.t, nullptr, true, false, false, 0.5, 1);
+    } else if (mode == "damp_uvw") {
+        for (Array* f : {&m.u, &m.v, &m.w}) AtomUtils::damp_wiggles(*f, nullptr, true, true, true, 1.0, sweeps);
     } else {
         std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
         return 2;

@@ -42,6 +42,8 @@ CASES = {
     "step_2x3": ("step", 2, 3, True),
     "rk4_uran": ("rk4_uran", 1, 1, True),
     "step_uran_2": ("step_uran", 2, 1, True),
+    "damp_t": ("damp_t", 1, 1, True),
+    "damp_uvw_2": ("damp_uvw", 1, 2, True),
 }
 
 

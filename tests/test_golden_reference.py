@@ -35,6 +35,12 @@ def run_oracle(oracle, name, g, p):
         level = oracle.hydro(g, p, F)
         rhs3 = oracle.rk4_rhs(g, p, F)
         oracle.pressure_uran(g, p, F, rhs3, 1)
+    elif mode == "damp_t":
+        oracle.damp_wiggles(g, F, A.FIELD["t"], False, True, True, 1.0, 1)
+        oracle.damp_wiggles(g, F, A.FIELD["t"], True, False, False, 0.5, 1)
+    elif mode == "damp_uvw":
+        for f in ("u", "v", "w"):
+            oracle.damp_wiggles(g, F, A.FIELD[f], True, True, True, 1.0, sweeps)
     elif mode == "step_uran":
         for _ in range(steps):
             level = oracle.hydro(g, p, F)

@@ -42,6 +42,12 @@ def test_gpu_matches_compiled_reference(oracle, name):
             level = core.computeHydrostaticPressure()
             core.RungeKuttaUran()
             core.PressureSolver("uran", sweeps=1)
+        elif mode == "damp_t":
+            core.damp_wiggles("t", False, True, True, 1.0, 1)
+            core.damp_wiggles("t", True, False, False, 0.5, 1)
+        elif mode == "damp_uvw":
+            for f in ("u", "v", "w"):
+                core.damp_wiggles(f, True, True, True, 1.0, sweeps)
         elif mode == "step_uran":
             core.step(steps, "uran", sweeps=1)
         else:

@@ -209,3 +209,20 @@ def test_converge_steps_match_oracle(setup, oracle, warm):
           f"rel {st4.rel_resid:.3e} / {rel:.3e}", flush=True)
     assert report(f"4 converging steps warm={warm}", errs) == 0.0
     assert st4.sweeps_done == it and st4.rel_resid == rel and rel <= 1e-8
+
+
+@pytest.mark.parametrize("dirs,strength,passes", [((False, True, True), 1.0, 1), ((True, False, False), 0.5, 1),
+                                                  ((True, True, True), 1.0, 3)])
+def test_damp_wiggles_matches_oracle(setup, oracle, dirs, strength, passes):
+    print(f"[case] Shapiro filter along (i, j, k) = {dirs}, strength {strength}, {passes} passes", flush=True)
+    g, p, F0 = setup
+    F = F0.copy()
+    with A.UranusCore(g, p) as core:
+        core.set_state(F)
+        for f in ("t", "u", "w"):
+            core.damp_wiggles(f, *dirs, strength, passes)
+        G = core.get_state()
+    for f in ("t", "u", "w"):
+        oracle.damp_wiggles(g, F, A.FIELD[f], *dirs, strength, passes)
+    errs = {n: rel_err(G[A.FIELD[n]], F[A.FIELD[n]]) for n in PROG}
+    assert report(f"damp {dirs} x{passes}", errs) == 0.0
