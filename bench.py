@@ -13,6 +13,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -50,6 +51,7 @@ def parse():
     ap.add_argument("--no-errors", action="store_true", help="skip the 1/10/100-step mixed-vs-FP64 comparison")
     ap.add_argument("--cold-start", action="store_true",
                     help="start every solve from p_n (default: from the extrapolation 2 p_n - p_{n-1})")
+    ap.add_argument("--cpu-one-thread", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP events in the timed region and no per-class pass (for runs under a profiler)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="window of the adjoint timing (0: skip)")
@@ -267,36 +269,67 @@ def refined_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(tol, budget_s):
+    """The oracle on the same workload (warm-started solves, as the GPU runs it): one warm-up step,
+    then one call of n steps sized to about budget_s; and, in a child process pinned to one OpenMP
+    thread, one step."""
     import aturan_amd as A
     from oracle import binding as O
     g = A.default_grid(1)
     p = A.default_params(g)
     F = A.synthetic_state(g, p, 20261203)
     O.bc(g, p, F)
-
-    def one():
-        O.hydro(g, p, F)
-        O.rk4(g, p, F)
-        O.bc(g, p, F)
-        O.pressure_converge(g, p, F, 200, tol)
-
-    one()                                  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        one()
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 50:
-            break
+    t0 = time.perf_counter()
+    O.step_converge(g, p, F, 1, 200, tol, True)          # warm-up
+    per = time.perf_counter() - t0
+    n = max(2, min(50, int(budget_s / max(per, 1e-3))))
+    t0 = time.perf_counter()
+    O.step_converge(g, p, F, n, 200, tol, True)
+    el = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    one = None
+    try:
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--cpu-one-thread", "--tol", str(tol)],
+                           env=env, capture_output=True, text=True, timeout=240)
+        one = json.loads(r.stdout.strip().splitlines()[-1])["steps_per_s"] if r.returncode == 0 else None
+    except (subprocess.TimeoutExpired, ValueError, IndexError, KeyError):
+        one = None
     return {"value": round(n / el, 4), "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"{n} steps (hydro + RK4 + BC + Poisson to {tol:g}) of the 41x181x361 synthetic state, "
-                      f"FP64 OpenMP oracle (C, -O2), after 1 warm-up step"}
+            "cpu": cpu_model(), "one_thread_steps_per_s": one,
+            "sample": f"{n} steps (hydro + RK4 + BC + warm-started Poisson to {tol:g}) of the 41x181x361 synthetic "
+                      f"state in one call, FP64 OpenMP oracle (C, -O2), after 1 warm-up step; one_thread: one "
+                      f"step after one warm-up step with OMP_NUM_THREADS=1"}
+
+
+def cpu_one_thread(tol):
+    import aturan_amd as A
+    from oracle import binding as O
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F = A.synthetic_state(g, p, 20261203)
+    O.bc(g, p, F)
+    O.step_converge(g, p, F, 1, 200, tol, True)
+    t0 = time.perf_counter()
+    O.step_converge(g, p, F, 1, 200, tol, True)
+    print(json.dumps({"steps_per_s": round(1.0 / (time.perf_counter() - t0), 4)}), flush=True)
 
 
 def main():
     args = parse()
+    if args.cpu_one_thread:
+        cpu_one_thread(args.tol)
+        return
     if args.copy_calibration:
         copy_calibration()
         return
