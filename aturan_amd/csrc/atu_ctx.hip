@@ -553,9 +553,10 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     int it = 0, outer = 0;
     double rel = INFINITY, bnorm = 0.0;
     // The last residual norm of a batch has written the flags and norms into the host mailbox; the
-    // host polls an event recorded after the batch (no copy, no blocking wait).
+    // host polls an event recorded after the batch (no copy, no blocking wait). mark_ctl records it;
+    // read_ctl waits for it and reads the mailbox.
+    auto mark_ctl = [&]() -> int { return H(hipEventRecord(c->ev_ctl, s), "hipEventRecord"); };
     auto read_ctl = [&]() -> int {
-        if (H(hipEventRecord(c->ev_ctl, s), "hipEventRecord")) return ATU_EHIP;
         hipError_t q;
         while ((q = hipEventQuery(c->ev_ctl)) == hipErrorNotReady) {
         }
@@ -568,6 +569,33 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         return ATU_OK;
     };
     const bool fixed = !(o->rel_tol > 0.0);   // no tolerance: run max_it iterations in one go
+    // The end of the solve: the walls of the final interior (the iterations' residuals form them as
+    // they read them), the corners, the seam and the held interior mean. After a batch it is queued
+    // before the host reads the outcome, each kernel returning at once unless the batch converged
+    // (skip = the not-done flag), so the GPU runs it while the host waits; a solve that did not
+    // converge that way runs it unconditionally at the end.
+    bool posted = false;
+    auto post = [&](const int* skip) -> int {
+        int r;
+        if ((r = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, skip); }), "walls")) ||
+            (r = H(launch_p_corners<double>(sh, c->consts, P, s, skip), "corners")) ||
+            (r = H(launch_seam<double>(sh, P, s, skip), "seam")))
+            return r;
+        SolveCtl k;
+        k.skip = skip;
+        if ((r = H(launch_dsum<double>(DSUM_P, sh, c->geo, P, nullptr, c->SROWS, c->SCAL + 4, s, nullptr, k), "dsum")) ||
+            (r = H(launch_shift_all<double>(sh, c->SCAL, 1, 4, c->dsol.ncell, P, s, skip), "shift")))
+            return r;
+        return ATU_OK;
+    };
+    auto batch_end = [&]() -> int {
+        int r;
+        if ((r = mark_ctl())) return r;
+        if (!fixed && (r = post(F + CTL_NOTDONE))) return r;
+        if ((r = read_ctl())) return r;
+        posted = !fixed && flags[CTL_DONE];
+        return ATU_OK;
+    };
     if (!c->mixed()) {
         const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, 1, 0, c->MBOX_DEV};
         int launched = 0, batch = fixed ? max_it : std::max(1, c->pred_iters);
@@ -589,7 +617,7 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
                     return e;
             }
             launched += nb;
-            if ((e = read_ctl())) return e;
+            if ((e = batch_end())) return e;
             if (flags[CTL_DONE]) break;
             batch = 1;
         }
@@ -630,22 +658,16 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
                     return e;
             }
             launched += nb;
-            if ((e = read_ctl())) return e;
+            if ((e = batch_end())) return e;
             if (flags[CTL_DONE]) break;
             batch = 1;
         }
-        if (max_ref == 0 && (e = read_ctl())) return e;
+        if (max_ref == 0 && ((e = mark_ctl()) || (e = read_ctl()))) return e;
         it = flags[CTL_ITERS];
         outer = flags[CTL_OUTER];
         c->pred_outer = std::max(1, outer);
     }
-    // the walls of the final interior (the iterations' residuals form them as they read them)
-    if ((e = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s); }), "walls")) ||
-        (e = H(launch_p_corners<double>(sh, c->consts, P, s), "corners")) ||
-        (e = H(launch_seam<double>(sh, P, s), "seam")) ||
-        (e = H(launch_dsum<double>(DSUM_P, sh, c->geo, P, nullptr, c->SROWS, c->SCAL + 4, s), "dsum")) ||
-        (e = H(launch_shift_all<double>(sh, c->SCAL, 1, 4, c->dsol.ncell, P, s), "shift")))
-        return e;
+    if (!posted && (e = post(nullptr))) return e;
     if (st) {
         st->sweeps_done = it;
         st->outer_iters = outer;

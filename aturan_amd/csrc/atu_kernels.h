@@ -108,7 +108,7 @@ hipError_t launch_rb_pass(const Shape& sh, const DevGeo& g, const StepConsts& c,
                           int colour, double omega, hipStream_t s);
 // The four corner rows (i = 0, im-1 at j = 0, jm-1) and their seam copies, after the last sweep.
 template <typename P>
-hipError_t launch_p_corners(const Shape& sh, const StepConsts& c, P* p, hipStream_t s);
+hipError_t launch_p_corners(const Shape& sh, const StepConsts& c, P* p, hipStream_t s, const int* skip = nullptr);
 
 // ---- the converging solve (atu_solve.hip) ----
 struct DevSolve {
@@ -120,7 +120,7 @@ enum { DSUM_P = 0, DSUM_YSRC = 1, DSUM_SQ = 2, DSUM_RESID2 = 3 };
 // Device-side control of the converging solve. flags[CTL_DONE] set means converged: every kernel of
 // an iteration given that flag returns at once, so the host can queue several iterations and read
 // the outcome once. The residual norm's final sum updates the flags (see k_dsum_final).
-enum { CTL_DONE = 0, CTL_ITERS = 1, CTL_OUTER = 2, CTL_N = 4 };
+enum { CTL_DONE = 0, CTL_ITERS = 1, CTL_OUTER = 2, CTL_NOTDONE = 3, CTL_N = 4 };
 // What the host reads after a batch: written by the residual norm's final sum into host memory
 // (pinned, mapped), so the host needs no copy, only to know that the batch has run.
 struct SolveMailbox {
@@ -133,6 +133,7 @@ struct SolveCtl {
     double tol = 0.0;
     int iters = 0, outer = 0;   // added to the counters before the test (the work just done)
     SolveMailbox* mbox = nullptr;   // device address of the host mailbox, or null
+    const int* skip = nullptr;      // when non-null and set, the sum's kernels return at once
 };
 // P: storage of the unknown, B: of the right-hand side, R: arithmetic of the line pass
 template <typename P, typename B, typename R>
@@ -141,7 +142,7 @@ hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p
 template <typename P>
 hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s, const int* done = nullptr);
 template <typename P>
-hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s);
+hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s, const int* skip = nullptr);
 template <typename P, typename B>
 hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
                        double* As, hipStream_t s, const int* done = nullptr);
@@ -162,7 +163,7 @@ hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, dou
 hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done = nullptr);
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
-                            hipStream_t s);
+                            hipStream_t s, const int* skip = nullptr);
 
 // ---- the Shapiro wiggle filter (atu_damp.hip): AtomUtils::damp_wiggles on one field; `scratch`
 // holds a field (its contents are not kept)

@@ -149,7 +149,8 @@ __global__ void __launch_bounds__(256) k_rb_pass(Shape sh, DevGeo g, double c43,
 
 // The corner rows: the polar pass at i = 0 and im-1 reads the radial walls of the same sweep.
 template <typename P>
-__global__ void k_p_corners(Shape sh, double c43, double c13, P* p) {
+__global__ void k_p_corners(Shape sh, double c43, double c13, P* p, const int* skip) {
+    if (skip && *skip) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int nk = sh.km - 1;
     if (k >= nk) return;
@@ -267,8 +268,8 @@ hipError_t launch_rb_pass(const Shape& sh, const DevGeo& g, const StepConsts& c,
 }
 
 template <typename P>
-hipError_t launch_p_corners(const Shape& sh, const StepConsts& c, P* p, hipStream_t s) {
-    hipLaunchKernelGGL((k_p_corners<P>), dim3((sh.km - 1 + 255) / 256), dim3(256), 0, s, sh, c.c43, c.c13, p);
+hipError_t launch_p_corners(const Shape& sh, const StepConsts& c, P* p, hipStream_t s, const int* skip) {
+    hipLaunchKernelGGL((k_p_corners<P>), dim3((sh.km - 1 + 255) / 256), dim3(256), 0, s, sh, c.c43, c.c13, p, skip);
     return hipGetLastError();
 }
 
@@ -290,8 +291,8 @@ template hipError_t launch_rb_pass<double>(const Shape&, const DevGeo&, const St
                                            double, hipStream_t);
 template hipError_t launch_rb_pass<float>(const Shape&, const DevGeo&, const StepConsts&, float*, const double*, int,
                                           double, hipStream_t);
-template hipError_t launch_p_corners<double>(const Shape&, const StepConsts&, double*, hipStream_t);
-template hipError_t launch_p_corners<float>(const Shape&, const StepConsts&, float*, hipStream_t);
+template hipError_t launch_p_corners<double>(const Shape&, const StepConsts&, double*, hipStream_t, const int*);
+template hipError_t launch_p_corners<float>(const Shape&, const StepConsts&, float*, hipStream_t, const int*);
 template hipError_t launch_residual<double>(const Shape&, const DevGeo&, const double*, const double*, double*, double*,
                                             double*, hipStream_t);
 template hipError_t launch_residual<float>(const Shape&, const DevGeo&, const float*, const double*, double*, double*,

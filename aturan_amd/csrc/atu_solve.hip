@@ -201,7 +201,8 @@ __global__ void __launch_bounds__(256) k_walls(Shape sh, double c43, double c13,
 
 // k = km-1 is the copy of k = 0 for every (i, j)
 template <typename P>
-__global__ void __launch_bounds__(256) k_seam(Shape sh, P* p) {
+__global__ void __launch_bounds__(256) k_seam(Shape sh, P* p, const int* skip) {
+    if (skip && *skip) return;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     if (j >= sh.jm) return;
@@ -524,6 +525,7 @@ __global__ void __launch_bounds__(1024) k_make_b_sq_rows(Shape sh, const double*
 constexpr int kDsumParts = 256;
 __global__ void __launch_bounds__(kDsumParts) k_dsum_final(int nrows, const double* rows, double* out, SolveCtl ctl) {
     if (ctl.flags && ctl.flags[CTL_DONE]) return;
+    if (ctl.skip && *ctl.skip) return;
     __shared__ double wsum[kDsumParts / 64];
     const int t = threadIdx.x;
     double s = 0.0;
@@ -541,6 +543,7 @@ __global__ void __launch_bounds__(kDsumParts) k_dsum_final(int nrows, const doub
             const double b2 = *ctl.bb;
             const double rel = b2 > 0.0 ? sqrt(total) / sqrt(b2) : sqrt(total);
             if (rel <= ctl.tol) f[CTL_DONE] = 1;
+            f[CTL_NOTDONE] = f[CTL_DONE] ? 0 : 1;
             if (ctl.mbox) {
                 for (int q = 0; q < CTL_N; q++) ctl.mbox->flags[q] = f[q];
                 ctl.mbox->rr = total;
@@ -570,8 +573,10 @@ __global__ void __launch_bounds__(kDsumParts) k_dsum_final2(int nrows, const dou
     if (threadIdx.x == 0) {
         *out1 = a;
         *out2 = b;
-        if (reset_flags)
+        if (reset_flags) {
             for (int q = 0; q < CTL_N; q++) reset_flags[q] = 0;
+            reset_flags[CTL_NOTDONE] = 1;
+        }
     }
 }
 
@@ -589,7 +594,8 @@ __global__ void __launch_bounds__(256) k_make_b(Shape sh, const double* src, con
 // hold the interior mean: p -= (mean_now - mean_before) on every cell
 template <typename P>
 __global__ void __launch_bounds__(256) k_shift_all(Shape sh, const double* sc, int slot_before, int slot_now,
-                                                   double ncell, P* p) {
+                                                   double ncell, P* p, const int* skip) {
+    if (skip && *skip) return;
     const size_t n = sh.fs;
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -635,8 +641,8 @@ hipError_t launch_walls(const Shape& sh, P* p, hipStream_t s, const int* done) {
 }
 
 template <typename P>
-hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s) {
-    hipLaunchKernelGGL((k_seam<P>), dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, p);
+hipError_t launch_seam(const Shape& sh, P* p, hipStream_t s, const int* skip) {
+    hipLaunchKernelGGL((k_seam<P>), dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, p, skip);
     return hipGetLastError();
 }
 
@@ -666,7 +672,7 @@ hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p
 template <typename P>
 hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, const double* v, double* rows,
                        double* out, hipStream_t s, float* r_out, const SolveCtl& ctl) {
-    const int* done = ctl.flags ? ctl.flags + CTL_DONE : nullptr;
+    const int* done = ctl.skip ? ctl.skip : (ctl.flags ? ctl.flags + CTL_DONE : nullptr);
     const int N = sh.km - 1;
     const int threads = ((N + 63) / 64) * 64;
     const dim3 grid(sh.jm - 2, sh.im - 2);
@@ -719,9 +725,9 @@ hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hip
 
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
-                            hipStream_t s) {
+                            hipStream_t s, const int* skip) {
     hipLaunchKernelGGL((k_shift_all<P>), dim3((unsigned)((sh.fs + 255) / 256)), dim3(256), 0, s, sh, sc, slot_before,
-                       slot_now, ncell, p);
+                       slot_now, ncell, p, skip);
     return hipGetLastError();
 }
 
@@ -735,10 +741,11 @@ template hipError_t launch_baro<float, float>(const Shape&, const DevGeo&, const
                                               double*, double*, double*, hipStream_t, const int*);
 template hipError_t launch_walls<double>(const Shape&, double*, hipStream_t, const int*);
 template hipError_t launch_walls<float>(const Shape&, float*, hipStream_t, const int*);
-template hipError_t launch_seam<double>(const Shape&, double*, hipStream_t);
-template hipError_t launch_seam<float>(const Shape&, float*, hipStream_t);
+template hipError_t launch_seam<double>(const Shape&, double*, hipStream_t, const int*);
+template hipError_t launch_seam<float>(const Shape&, float*, hipStream_t, const int*);
 template hipError_t launch_dsum<double>(int, const Shape&, const DevGeo&, const double*, const double*, double*,
                                         double*, hipStream_t, float*, const SolveCtl&);
-template hipError_t launch_shift_all<double>(const Shape&, const double*, int, int, double, double*, hipStream_t);
+template hipError_t launch_shift_all<double>(const Shape&, const double*, int, int, double, double*, hipStream_t,
+                                              const int*);
 
 }  // namespace atu
