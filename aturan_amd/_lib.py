@@ -79,6 +79,15 @@ class WindowOpts(C.Structure):
                 ("pad_", C.c_int32), ("rel_tol", C.c_double), ("target_amp", C.c_double)]
 
 
+class TurbOpts(C.Structure):
+    _fields_ = [("model", C.c_int32), ("free_flow", C.c_int32), ("abl_taper", C.c_int32), ("curvature", C.c_int32),
+                ("k_bg", C.c_double), ("l_bg", C.c_double), ("nue_max", C.c_double), ("vel_star_max", C.c_double),
+                ("abl_height", C.c_double)]
+
+
+TURB_MODELS = {"k_omega_SST": 0, "k_omega": 1, "k_epsilon": 2}
+TURB_ARRAYS = {"prod": 0, "tken": 1, "disn": 2, "vel_star": 3}
+
 _DP = C.POINTER(C.c_double)
 _SIGS = {
     "atu_grid_default": (C.c_int, [C.c_int, C.POINTER(Grid)]),
@@ -107,6 +116,11 @@ _SIGS = {
     "atu_window_rewind": (C.c_int, [C.c_void_p]),
     "atu_build_info": (C.c_char_p, []),
     "atu_damp_wiggles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
+    "atu_turb_opts_default": (C.c_int, [C.POINTER(TurbOpts)]),
+    "atu_turbulence_init": (C.c_int, [C.c_void_p, C.POINTER(TurbOpts)]),
+    "atu_turbulence_run": (C.c_int, [C.c_void_p, C.POINTER(TurbOpts)]),
+    "atu_turb_set": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "atu_turb_get": (C.c_int, [C.c_void_p, C.c_int, _DP]),
     "atu_profile_begin": (C.c_int, [C.c_void_p]),
     "atu_profile_begin_classes": (C.c_int, [C.c_void_p, C.c_char_p]),
     "atu_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.POINTER(C.c_int)]),

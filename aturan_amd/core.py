@@ -56,6 +56,22 @@ def cp_table(mode: int = 2) -> np.ndarray:
     return out
 
 
+def turb_opts(model: str = "k_omega_SST", free_flow: bool = True, abl_taper: bool = False, curvature: bool = True,
+              k_bg: float = 0.0, l_bg: float = 0.0, nue_max: float = 0.0, vel_star_max: float = 0.0,
+              abl_height: float = 0.0) -> L.TurbOpts:
+    """The closure's knobs (the reference's ATURAN_TURB_MODEL, _TURB_FREE, _ABL_TAPER, _TURB_CURV,
+    _TURB_K_BG, _TURB_L_BG, _NUE_MAX, _VEL_STAR_MAX); 0 selects the reference's default."""
+    o = L.TurbOpts()
+    _check(L.load().atu_turb_opts_default(C.byref(o)), None, "atu_turb_opts_default")
+    o.model = L.TURB_MODELS[model]
+    o.free_flow, o.abl_taper, o.curvature = int(free_flow), int(abl_taper), int(curvature)
+    for n, v in (("k_bg", k_bg), ("l_bg", l_bg), ("nue_max", nue_max), ("vel_star_max", vel_star_max),
+                 ("abl_height", abl_height)):
+        if v > 0.0:
+            setattr(o, n, float(v))
+    return o
+
+
 @dataclass
 class SolveResult:
     sweeps_done: int
@@ -169,6 +185,37 @@ class UranusCore:
         f = L.FIELD[field] if isinstance(field, str) else int(field)
         _check(self._lib.atu_damp_wiggles(self._ctx, f, int(along_i), int(along_j), int(along_k), strength, passes),
                self._ctx, "atu_damp_wiggles")
+
+    # ---- the turbulence closure, Turbulence<cUranusModel> (planet/Turbulence.h) ----
+    def Turbulence_init(self, **knobs) -> None:
+        """Turbulence::init (planet/Turbulence.h:282-303): seed tke and dis, then fill nue, prod and the
+        sources. Knobs as turb_opts()."""
+        o = turb_opts(**knobs)
+        _check(self._lib.atu_turbulence_init(self._ctx, C.byref(o)), self._ctx, "atu_turbulence_init")
+
+    def Turbulence_run(self, **knobs) -> None:
+        """Turbulence::run (planet/Turbulence.h:306-332), the loop's call after restoreVar
+        (planet/cUranusModel.cpp:1270). Knobs as turb_opts()."""
+        o = turb_opts(**knobs)
+        _check(self._lib.atu_turbulence_run(self._ctx, C.byref(o)), self._ctx, "atu_turbulence_run")
+
+    def turb_array(self, name: str) -> np.ndarray:
+        """The closure's own arrays: prod, tken, disn ([im][jm][km]) or vel_star ([jm][km])."""
+        g = self.grid
+        shape = (g.jm, g.km) if name == "vel_star" else (g.im, g.jm, g.km)
+        out = np.empty(shape, dtype=np.float64)
+        _check(self._lib.atu_turb_get(self._ctx, L.TURB_ARRAYS[name], out.ctypes.data_as(L._DP)), self._ctx,
+               "atu_turb_get")
+        return out
+
+    def set_turb_array(self, name: str, values: np.ndarray) -> None:
+        g = self.grid
+        shape = (g.jm, g.km) if name == "vel_star" else (g.im, g.jm, g.km)
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
+        _check(self._lib.atu_turb_set(self._ctx, L.TURB_ARRAYS[name], a.ctypes.data_as(L._DP)), self._ctx,
+               "atu_turb_set")
 
     def pressure_residual(self) -> tuple[float, float]:
         r, s = C.c_double(), C.c_double()

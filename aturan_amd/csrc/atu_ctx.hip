@@ -27,8 +27,8 @@ struct ProfEvent {
 };
 
 const char* kClassNames[] = {"rk_stage", "rb_pass", "hydro", "bc", "proj_src", "residual", "line", "baro", "adjoint",
-                             "step"};
-enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_ADJ, PC_STEP, PC_N };
+                             "step", "turbulence"};
+enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_ADJ, PC_STEP, PC_TURB, PC_N };
 
 }  // namespace
 
@@ -68,6 +68,14 @@ struct atu_ctx {
     double* SROWS2 = nullptr; // the same, for a second sum of one pass
     double* PPREV = nullptr;  // p_dyn as the last warm-started solve found it (interior)
     double* DAMP = nullptr;   // scratch field of the wiggle filter
+    // the turbulence closure: prod, tken, disn, the friction velocity and the tables of the last
+    // knobs (allocated by the first closure call or atu_turb_set)
+    double* TURB3 = nullptr;      // 3 fields
+    double* VSTAR = nullptr;      // [jm][ldk]
+    double* TURBTAB = nullptr;    // per-level and per-row tables
+    atu_turb_opts turb_opts{};
+    bool turb_tab_ok = false;
+    TurbArgs turb{};
     unsigned long state_epoch = 0, warm_epoch = ~0ul;   // warm start valid iff equal
     double* SCAL = nullptr;   // 8 scalars: 0 y.src, 1 mean before, 2 |b|^2, 3 |r|^2, 4 mean after
     double* BR = nullptr;     // J*N
@@ -162,7 +170,7 @@ struct atu_ctx {
 
 atu_ctx::~atu_ctx() {
     for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SROWS2, PPREV, DAMP, SCAL,
-                      BR, BAC, BAS})
+                      BR, BAC, BAS, TURB3, VSTAR, TURBTAB})
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
     if (CTL) hipFree(CTL);
@@ -974,6 +982,124 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
     return ATU_OK;
 }
 
+// After a field's master changed on the device: restoreVar's role for a prognostic field of an FP64
+// context (every stage buffer starts from the same state, boundary planes included), the FP32
+// mirrors of a mixed one.
+int field_written(atu_ctx* c, int f) {
+    const size_t fs = c->sh.fs;
+    double* d = field_ptr(c, f);
+    int e = ATU_OK;
+    if (!c->mixed() && f < ATU_NPROG) {
+        e = c->hip(hipMemcpyAsync(c->prog(c->SA, f), d, fs * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                   "hipMemcpyAsync(D2D)");
+        if (!e)
+            e = c->hip(hipMemcpyAsync(c->prog(c->SB, f), d, fs * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                       "hipMemcpyAsync(D2D)");
+    }
+    if (!e && c->mixed()) {
+        if (f < ATU_NPROG) {
+            e = mirror(c, d, c->Y32 + (size_t)f * fs);
+            if (!e) e = mirror(c, d, c->SA32 + (size_t)f * fs);
+            if (!e) e = mirror(c, d, c->SB32 + (size_t)f * fs);
+        } else if (f >= ATU_NUE && f < ATU_NUE + ATU_NFROZEN) {
+            e = mirror(c, d, c->FROZEN32 + (size_t)(f - ATU_NUE) * fs);
+        } else if (f >= ATU_AUX_U && f <= ATU_AUX_W) {
+            e = mirror(c, d, c->AUX32 + (size_t)(f - ATU_AUX_U) * fs);   // staged through RES
+        }
+    }
+    return e;
+}
+
+int turb_alloc(atu_ctx* c) {
+    const Shape& sh = c->sh;
+    if (!c->TURB3) {
+        if (c->hip(hipMalloc((void**)&c->TURB3, 3 * sh.fs * sizeof(double)), "hipMalloc(turbulence)") ||
+            c->hip(hipMemsetAsync(c->TURB3, 0, 3 * sh.fs * sizeof(double), c->stream), "hipMemsetAsync"))
+            return ATU_ENOMEM;
+    }
+    if (!c->VSTAR) {
+        if (c->hip(hipMalloc((void**)&c->VSTAR, (size_t)sh.jm * sh.ldk * sizeof(double)), "hipMalloc(turbulence)") ||
+            c->hip(hipMemsetAsync(c->VSTAR, 0, (size_t)sh.jm * sh.ldk * sizeof(double), c->stream), "hipMemsetAsync"))
+            return ATU_ENOMEM;
+    }
+    return ATU_OK;
+}
+
+// Turbulence<cUranusModel>::init (init) or ::run (planet/Turbulence.h:282-332) on the FP64 masters,
+// then the stage buffers or FP32 mirrors of the five fields it writes.
+int do_turbulence(atu_ctx* c, const atu_turb_opts* in, bool init) {
+    if (!c) return ATU_EINVAL;
+    atu_turb_opts def;
+    atu_turb_opts_default(&def);
+    atu_turb_opts o;
+    if (!resolve_turb_opts(c->grid, c->params, in ? *in : def, o))
+        return c->fail(ATU_EINVAL, "turbulence: knob out of range");
+    const Shape& sh = c->sh;
+    if (sh.im < 3 || sh.jm < 3 || sh.km < 4) return c->fail(ATU_EINVAL, "turbulence: grid too small");
+    int e = turb_alloc(c);
+    if (e) return e;
+    if (!c->turb_tab_ok || std::memcmp(&o, &c->turb_opts, sizeof o) != 0) {
+        TurbTables t;
+        build_turb_tables(c->grid, c->params, c->hgeo, o, t);
+        const std::vector<const std::vector<double>*> parts = {&t.rm, &t.inv_rm, &t.ystar_sst, &t.yphys_ke,
+                                                               &t.ystar_ke, &t.taper, &t.karman_z, &t.pz,
+                                                               &t.sin_pair, &t.cos_pair, &t.sin_alone};
+        std::vector<double> blob;
+        std::vector<size_t> off;
+        for (auto* v : parts) {
+            off.push_back(blob.size());
+            blob.insert(blob.end(), v->begin(), v->end());
+        }
+        if (!c->TURBTAB && c->hip(hipMalloc((void**)&c->TURBTAB, blob.size() * sizeof(double)), "hipMalloc(turbulence)"))
+            return ATU_ENOMEM;
+        // the previous call's launches may still read the old tables
+        if (c->hip(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ||
+            c->hip(hipMemcpy(c->TURBTAB, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice),
+                   "hipMemcpy(turbulence tables)"))
+            return ATU_EHIP;
+        TurbArgs& a = c->turb;
+        const double* d = c->TURBTAB;
+        const double** dst[] = {&a.rm, &a.inv_rm, &a.ystar_sst, &a.yphys_ke, &a.ystar_ke, &a.taper, &a.karman_z, &a.pz,
+                                &a.sin_pair, &a.cos_pair, &a.sin_alone};
+        for (size_t q = 0; q < parts.size(); q++) *dst[q] = d + off[q];
+        a.model = o.model;
+        a.free_flow = o.free_flow;
+        a.curvature = o.curvature;
+        a.inv_2dr = c->hgeo.inv_2dr;
+        a.inv_2dthe = c->hgeo.inv_2dthe;
+        a.inv_2dphi = c->hgeo.inv_2dphi;
+        a.dr = c->grid.dr;
+        a.dthe = c->grid.dthe;
+        a.dphi = c->grid.dphi;
+        a.nue_max_nd = t.nue_max_nd; a.k_bg_free_nd = t.k_bg_free_nd; a.kbg_den = t.kbg_den;
+        a.nue_air_nd = t.nue_air_nd; a.nue_air = t.nue_air; a.u_0 = t.u_0; a.L_m = t.L_m; a.re_turb = t.re_turb;
+        a.sqrt_bet_star = t.sqrt_bet_star; a.sqrt_cd = t.sqrt_cd; a.vel_star_max = t.vel_star_max;
+        a.init_tke = t.init_tke; a.init_dis = t.init_dis; a.inv_u02 = t.inv_u02; a.nd_omega = t.nd_omega;
+        a.nd_eps = t.nd_eps; a.c_m025 = t.c_m025; a.c_075 = t.c_075; a.sqrt_cnue = t.sqrt_cnue;
+        a.karman_l = t.karman_l;
+        c->turb_opts = o;
+        c->turb_tab_ok = true;
+    }
+    TurbArgs a = c->turb;
+    a.u = field_ptr(c, ATU_U);
+    a.v = field_ptr(c, ATU_V);
+    a.w = field_ptr(c, ATU_W);
+    a.tke = field_ptr(c, ATU_TKE);
+    a.dis = field_ptr(c, ATU_DIS);
+    a.nue = field_ptr(c, ATU_NUE);
+    a.tke_source = field_ptr(c, ATU_TKE_SOURCE);
+    a.dis_source = field_ptr(c, ATU_DIS_SOURCE);
+    a.prod = c->TURB3;
+    a.tken = c->TURB3 + sh.fs;
+    a.disn = c->TURB3 + 2 * sh.fs;
+    a.vel_star = c->VSTAR;
+    if ((e = c->hip(c->timed(PC_TURB, [&] { return launch_turbulence(sh, a, init, c->stream); }), "turbulence")))
+        return e;
+    for (int f : {ATU_TKE, ATU_DIS, ATU_NUE, ATU_TKE_SOURCE, ATU_DIS_SOURCE})
+        if ((e = field_written(c, f))) return e;
+    return ATU_OK;
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -1098,31 +1224,10 @@ int atu_synchronize(atu_ctx* ctx) {
 int atu_set_field(atu_ctx* ctx, int f, const double* host) {
     if (!ctx || !host) return ATU_EINVAL;
     ctx->state_epoch++;
-    const bool aux = (f >= ATU_AUX_U && f <= ATU_AUX_W);
     double* d = field_ptr(ctx, f);
     if (!d) return ctx->fail(ATU_EINVAL, "atu_set_field: unknown field id");
-    const size_t fs = ctx->sh.fs;
     int e = put_field(ctx, d, host);
-    if (!e && !ctx->mixed() && f < ATU_NPROG) {
-        // restoreVar's role: every stage buffer starts from the same state, boundary planes included
-        // (copied on the device from the uploaded field)
-        e = ctx->hip(hipMemcpyAsync(ctx->prog(ctx->SA, f), d, fs * sizeof(double), hipMemcpyDeviceToDevice,
-                                    ctx->stream), "hipMemcpyAsync(D2D)");
-        if (!e)
-            e = ctx->hip(hipMemcpyAsync(ctx->prog(ctx->SB, f), d, fs * sizeof(double), hipMemcpyDeviceToDevice,
-                                        ctx->stream), "hipMemcpyAsync(D2D)");
-    }
-    if (!e && ctx->mixed()) {
-        if (f < ATU_NPROG) {
-            e = mirror(ctx, d, ctx->Y32 + (size_t)f * fs);
-            if (!e) e = mirror(ctx, d, ctx->SA32 + (size_t)f * fs);
-            if (!e) e = mirror(ctx, d, ctx->SB32 + (size_t)f * fs);
-        } else if (f >= ATU_NUE && f < ATU_NUE + ATU_NFROZEN) {
-            e = mirror(ctx, d, ctx->FROZEN32 + (size_t)(f - ATU_NUE) * fs);
-        } else if (aux) {
-            e = mirror(ctx, d, ctx->AUX32 + (size_t)(f - ATU_AUX_U) * fs);   // staged through RES
-        }
-    }
+    if (!e) e = field_written(ctx, f);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
 }
@@ -1137,6 +1242,41 @@ int atu_get_field(atu_ctx* ctx, int f, double* host) {
         e = ctx->hip(launch_convert(ctx->AUX32 + (size_t)(f - ATU_AUX_U) * ctx->sh.fs, ctx->RES, ctx->sh.fs,
                                     ctx->stream), "convert");
     if (!e) e = get_field(ctx, d, host);
+    if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return e;
+}
+
+int atu_turbulence_init(atu_ctx* ctx, const atu_turb_opts* opts) { return do_turbulence(ctx, opts, true); }
+int atu_turbulence_run(atu_ctx* ctx, const atu_turb_opts* opts) { return do_turbulence(ctx, opts, false); }
+
+int atu_turb_set(atu_ctx* ctx, int which, const double* host) {
+    if (!ctx || !host) return ATU_EINVAL;
+    if (which < ATU_TURB_PROD || which > ATU_TURB_VEL_STAR) return ctx->fail(ATU_EINVAL, "atu_turb_set: unknown array");
+    int e = turb_alloc(ctx);
+    if (e) return e;
+    const Shape& sh = ctx->sh;
+    if (which == ATU_TURB_VEL_STAR)
+        e = ctx->hip(hipMemcpy2DAsync(ctx->VSTAR, (size_t)sh.ldk * sizeof(double), host, (size_t)sh.km * sizeof(double),
+                                      (size_t)sh.km * sizeof(double), (size_t)sh.jm, hipMemcpyHostToDevice, ctx->stream),
+                     "hipMemcpy2DAsync(H2D)");
+    else
+        e = put_field(ctx, ctx->TURB3 + (size_t)which * sh.fs, host);
+    if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return e;
+}
+
+int atu_turb_get(atu_ctx* ctx, int which, double* host) {
+    if (!ctx || !host) return ATU_EINVAL;
+    if (which < ATU_TURB_PROD || which > ATU_TURB_VEL_STAR) return ctx->fail(ATU_EINVAL, "atu_turb_get: unknown array");
+    int e = turb_alloc(ctx);
+    if (e) return e;
+    const Shape& sh = ctx->sh;
+    if (which == ATU_TURB_VEL_STAR)
+        e = ctx->hip(hipMemcpy2DAsync(host, (size_t)sh.km * sizeof(double), ctx->VSTAR, (size_t)sh.ldk * sizeof(double),
+                                      (size_t)sh.km * sizeof(double), (size_t)sh.jm, hipMemcpyDeviceToHost, ctx->stream),
+                     "hipMemcpy2DAsync(D2H)");
+    else
+        e = get_field(ctx, ctx->TURB3 + (size_t)which * sh.fs, host);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
 }

@@ -377,6 +377,132 @@ void build_step_consts(const atu_grid& g, const atu_params& p, StepConsts& c) {
 // =============================================================================================
 using namespace atu;
 
+bool atu::resolve_turb_opts(const atu_grid& g, const atu_params& p, const atu_turb_opts& in, atu_turb_opts& o) {
+    o = in;
+    if (o.model < ATU_TURB_K_OMEGA_SST || o.model > ATU_TURB_K_EPSILON) return false;
+    if (o.free_flow < 0 || o.free_flow > 1 || o.abl_taper < 0 || o.abl_taper > 1 || o.curvature < 0 ||
+        o.curvature > 1)
+        return false;
+    // planet/Turbulence.h:150-151 (k_bg), 152-156 (L_bg = L_atm_m / (im-1)), 125-128 (nue_max),
+    // 349-352 (u_tau ceiling); planet/cUranusModel.h:364 (abl_height)
+    if (!(o.k_bg > 0.0)) o.k_bg = 1.0;
+    if (!(o.l_bg > 0.0)) o.l_bg = (p.L_atm * 1.0e3) / double(g.im - 1);
+    if (!(o.nue_max > 0.0)) o.nue_max = 1.0e5;
+    if (!(o.vel_star_max > 0.0)) o.vel_star_max = std::sqrt(0.002) * 200.0;
+    if (!(o.abl_height > 0.0)) o.abl_height = 20000.0;
+    return true;
+}
+
+// planet/Turbulence.h, with the model's layer heights (planet/cUranusModel.h:1199-1209: FP32
+// multiples of L_atm/(im-1), in km) and its accessors height_m (:262-265) and layer_thickness_m
+// (planet/cUranusModel.h:372-375). The closure's constants: C_nue 0.028, Karman 0.42, dis_min 1e-10
+// (:191-195); vel_star_ref 0.4 m/s, z_0 0.1 m, nue_gas 1.8e-5 m2/s (its constructor's defaults, :104-107).
+void atu::build_turb_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, const atu_turb_opts& o,
+                            TurbTables& t) {
+    const int im = g.im, jm = g.jm;
+    const double C_nue = 0.028, Karman = 0.42, dis_min = 1.0e-10;
+    const double nue_air = 1.8e-5, vel_star_ref = 0.4, z_0 = 0.1;
+    const double free_wall = 1.0e20;
+    std::vector<float> lh(im);
+    const float h = (float)(p.L_atm / (im - 1));
+    for (int i = 0; i < im; i++) lh[i] = (float)i * h;
+    auto height_m = [&](int i) { return (double)((0 > i || i > im - 1) ? -1.0f : lh[i]) * 1.0e3; };
+    auto layer_thickness_m = [&](int i) { return (i < 0 || i > im - 2) ? 0.0 : (double)(lh[i + 1] - lh[i]) * 1.0e3; };
+    const double L_m = p.L_atm * 1.0e3;
+    const double y_mount = height_m(0);
+    t.rm.assign(im, 0); t.inv_rm.assign(im, 0); t.ystar_sst.assign(im, 0); t.yphys_ke.assign(im, 0);
+    t.ystar_ke.assign(im, 0); t.taper.assign(im, 1.0); t.karman_z.assign(im, 0); t.pz.assign(im, 0);
+    // clamp_nue's taper (:567-586)
+    int i_abl = 0;
+    while (i_abl < im - 1 && height_m(i_abl) <= o.abl_height) i_abl++;
+    const double taper_dz = std::max(1.0, height_m(std::min(i_abl, im - 1)) - height_m(std::max(i_abl - 1, 0)));
+    for (int i = 0; i < im; i++) {
+        t.rm[i] = geo.rm[i];
+        t.inv_rm[i] = 1.0 / geo.rm[i];
+        const double dz_layer = std::max(1.0, layer_thickness_m(std::max(i - 1, 0)));
+        const double y_sst = o.free_flow ? free_wall : std::max(height_m(i) - y_mount, dz_layer);   // :1040-1041
+        t.ystar_sst[i] = y_sst / L_m;
+        const double y_ke = o.free_flow ? free_wall : std::max(height_m(i) - y_mount, L_m);        // :896
+        t.yphys_ke[i] = y_ke;
+        t.ystar_ke[i] = y_ke / L_m;
+        if (o.abl_taper) {
+            const double z_agl = height_m(i) - height_m(0);
+            const double f = (o.abl_height + taper_dz - z_agl) / taper_dz;
+            t.taper[i] = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+        }
+        // init_fields (:447-462)
+        const double z_i = height_m(i) - y_mount;
+        const double z_safe = o.free_flow ? free_wall : std::max(z_i, dz_layer);
+        const double z_frac = std::min(z_i, o.abl_height) / o.abl_height;
+        t.karman_z[i] = Karman * z_safe;
+        t.pz[i] = (1.0 - z_frac) * (1.0 - z_frac);   // std::pow(1 - z_frac, 2), formed as the square by g++
+    }
+    t.sin_pair.assign(jm, 0); t.cos_pair.assign(jm, 0); t.sin_alone.assign(jm, 0);
+    for (int j = 0; j < jm; j++) {
+        // compute_sources forms the sin and cos of one angle (:680-682), which its g++ build takes
+        // from one sincos call; compute_k_omega forms the sin alone (:944-945)
+        double sj, cj;
+        sincos(geo.thez[j], &sj, &cj);
+        t.sin_pair[j] = (sj == 0.0) ? 1.0e-5 : sj;
+        t.cos_pair[j] = cj;
+    }
+    for (int j = 0; j < jm; j++) {
+        double sj = std::sin(geo.thez[j]);
+        if (sj == 0.0) sj = 1.0e-5;
+        t.sin_alone[j] = sj;
+    }
+    t.nue_max_nd = o.nue_max / (p.u_0 * L_m);
+    t.k_bg_free_nd = o.k_bg / (p.u_0 * p.u_0);
+    t.kbg_den = std::sqrt(C_nue) * p.u_0 * p.u_0;
+    t.nue_air_nd = nue_air / (p.u_0 * L_m);
+    t.nue_air = nue_air;
+    t.u_0 = p.u_0;
+    t.L_m = L_m;
+    t.re_turb = vel_star_ref * z_0 / nue_air;
+    t.sqrt_bet_star = std::sqrt(0.09);
+    t.sqrt_cd = std::sqrt(0.002);
+    t.vel_star_max = o.vel_star_max;
+    t.inv_u02 = 1.0 / (p.u_0 * p.u_0);
+    t.nd_omega = L_m / p.u_0;
+    t.nd_eps = L_m / (p.u_0 * p.u_0 * p.u_0);
+    t.c_m025 = std::pow(C_nue, -0.25);
+    t.c_075 = std::pow(C_nue, 0.75);
+    t.sqrt_cnue = std::sqrt(C_nue);
+    t.karman_l = Karman * L_m;   // init_fields' Karman * L_bg without TURB_FREE
+    // With TURB_FREE every interior cell of init_fields takes one value (:440-462)
+    {
+        const double k_bg = o.k_bg, L_bg = o.l_bg;
+        const double om_bg = std::pow(C_nue, -0.25) * std::sqrt(k_bg) / (Karman * L_bg);
+        const double eps_bg = std::pow(C_nue, 0.75) * std::pow(k_bg, 1.5) / (Karman * L_bg);
+        const double k_phys = std::max(0.0, k_bg);
+        double dis_nd;
+        if (o.model == ATU_TURB_K_EPSILON) {
+            const double eps_phys = std::pow(C_nue, 0.75) * std::pow(k_phys, 1.5) / (Karman * free_wall);
+            dis_nd = std::max(eps_phys, eps_bg) * t.nd_eps;
+        } else {
+            const double om_phys = std::pow(C_nue, -0.25) * std::sqrt(k_phys) / (Karman * free_wall);
+            dis_nd = std::max(om_phys, om_bg) * t.nd_omega;
+        }
+        t.init_tke = std::max(0.0, k_phys * t.inv_u02);
+        t.init_dis = std::max(dis_min, dis_nd);
+    }
+}
+
+extern "C" int atu_turb_opts_default(atu_turb_opts* o) {
+    if (!o) return ATU_EINVAL;
+    std::memset(o, 0, sizeof *o);
+    o->model = ATU_TURB_K_OMEGA_SST;
+    o->free_flow = 1;
+    o->abl_taper = 0;
+    o->curvature = 1;
+    o->k_bg = 1.0;
+    o->l_bg = 0.0;   // L_atm * 1e3 / (im-1) of the context's grid
+    o->nue_max = 1.0e5;
+    o->vel_star_max = std::sqrt(0.002) * 200.0;
+    o->abl_height = 20000.0;
+    return ATU_OK;
+}
+
 extern "C" int atu_grid_default(int refine, atu_grid* out) {
     if (!out || (refine != 1 && refine != 2)) return ATU_EINVAL;
     std::memset(out, 0, sizeof(*out));

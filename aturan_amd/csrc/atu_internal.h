@@ -55,6 +55,24 @@ struct UranTables {
 };
 void build_uran_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, UranTables& u);
 
+// Tables and scalars of the turbulence closure (planet/Turbulence.h) for one set of knobs, each
+// formed with the reference's own expression: per level the metric radius, the wall distances of
+// the SST and k-epsilon branches, the ABL taper of clamp_nue and init_fields' per-level factors;
+// per row the sin / cos pair of compute_sources and the lone sin of compute_k_omega.
+struct TurbTables {
+    std::vector<double> rm, inv_rm, ystar_sst, yphys_ke, ystar_ke, taper, karman_z, pz;   // [im]
+    std::vector<double> sin_pair, cos_pair, sin_alone;                                     // [jm]
+    double nue_max_nd = 0, k_bg_free_nd = 0, kbg_den = 0, nue_air_nd = 0, nue_air = 0, u_0 = 0, L_m = 0;
+    double re_turb = 0, sqrt_bet_star = 0, sqrt_cd = 0, vel_star_max = 0;
+    double init_tke = 0, init_dis = 0, inv_u02 = 0, nd_omega = 0, nd_eps = 0, c_m025 = 0, c_075 = 0;
+    double sqrt_cnue = 0, karman_l = 0;
+};
+// The knobs with every unset (<= 0) value replaced by the reference's default; false when one is
+// out of range.
+bool resolve_turb_opts(const atu_grid& g, const atu_params& p, const atu_turb_opts& in, atu_turb_opts& out);
+void build_turb_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, const atu_turb_opts& o,
+                       TurbTables& t);
+
 // cp table of H2 for a CpH2 mode (1 normal, 2 equilibrium), J/(kg K).
 const std::vector<double>& cp_h2_table(int mode);
 double cp_lapse(const atu_params& p, double T_K);

@@ -170,6 +170,22 @@ hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, 
 hipError_t launch_damp_wiggles(const Shape& sh, double* field, double* scratch, bool along_i, bool along_j,
                                bool along_k, double strength, int passes, hipStream_t s);
 
+// ---- the turbulence closure (atu_turb.hip): Turbulence<cUranusModel>::init / run ----
+struct TurbArgs {
+    const double *u, *v, *w;
+    double *tke, *dis, *nue, *tke_source, *dis_source;    // context fields
+    double *prod, *tken, *disn;                           // the closure's own fields
+    double* vel_star;                                     // [jm][ldk]
+    const double *rm, *inv_rm, *ystar_sst, *yphys_ke, *ystar_ke, *taper, *karman_z, *pz;   // [im]
+    const double *sin_pair, *cos_pair, *sin_alone;                                          // [jm]
+    int model, free_flow, curvature, pad_;
+    double inv_2dr, inv_2dthe, inv_2dphi, dr, dthe, dphi;
+    double nue_max_nd, k_bg_free_nd, kbg_den, nue_air_nd, nue_air, u_0, L_m, re_turb, sqrt_bet_star;
+    double sqrt_cd, vel_star_max;
+    double init_tke, init_dis, inv_u02, nd_omega, nd_eps, c_m025, c_075, sqrt_cnue, karman_l;
+};
+hipError_t launch_turbulence(const Shape& sh, const TurbArgs& a, bool init, hipStream_t s);
+
 // ---- PressureSolverUran (atu_uran.hip), the reference's default solver ----
 struct UranTabs {
     const double *a_inv_rm, *a_num2, *m_num2;                                  // [im]

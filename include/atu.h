@@ -249,6 +249,50 @@ int atu_pressure_residual(atu_ctx* ctx, double* rel_resid, double* src_norm);
  * after it (atu_apply_bc). */
 int atu_damp_wiggles(atu_ctx* ctx, int field, int along_i, int along_j, int along_k, double strength, int passes);
 
+/* ---- turbulence closure: Turbulence<cUranusModel> (planet/Turbulence.h, planet/TurbulenceUran.h) ----
+ * Fills the eddy viscosity nue*, the production prod and the sources tke_source, dis_source of the
+ * k-omega SST (Menter 1994, the default turb_model), k-omega (Wilcox) or k-epsilon (Chien) closure
+ * from u, v, w, tke, dis: atu_turbulence_run is Turbulence::run (planet/Turbulence.h:306-332), the
+ * reference loop's call after restoreVar (planet/cUranusModel.cpp:1270); atu_turbulence_init is
+ * Turbulence::init (:282-303), which first seeds tke and dis. Both act on the context's fields
+ * ATU_TKE, ATU_DIS, ATU_NUE, ATU_TKE_SOURCE, ATU_DIS_SOURCE and on four arrays of the closure kept
+ * by the context (atu_turb_array): prod, tken, disn and the [jm][km] friction velocity vel_star.
+ * The grid is Uranus's: no solid cells (is_solid false, surface_index 0), phi periodic, no
+ * coordinate stretching, the closure's own boundaries off (bc_turb_active false). The buoyancy
+ * term of the k equation (<TAG>_TURB_BUOY, default 0 on ATURAN) is not built. */
+typedef enum atu_turb_model {
+    ATU_TURB_K_OMEGA_SST = 0,   /* planet/Turbulence.h:1020-1107 (the default)                  */
+    ATU_TURB_K_OMEGA = 1,       /* :929-1017                                                     */
+    ATU_TURB_K_EPSILON = 2      /* :885-927                                                      */
+} atu_turb_model;
+
+/* The knobs of the closure the reference reads from its environment (ATURAN_<NAME>). */
+typedef struct atu_turb_opts {
+    int32_t model;              /* atu_turb_model (ATURAN_TURB_MODEL)                           */
+    int32_t free_flow;          /* TURB_FREE: 1 no wall anywhere (the default), 0 the surface
+                                   boundary-layer form above i = 0 (:131-142)                    */
+    int32_t abl_taper;          /* ABL_TAPER: 1 tapers nue* to 0 above abl_height (:564-589)     */
+    int32_t curvature;          /* TURB_CURV: 1 the curvature terms of the gradient tensor       */
+    double k_bg;                /* TURB_K_BG [m2/s2], background k with free_flow (default 1)    */
+    double l_bg;                /* TURB_L_BG [m], background eddy length (default L_atm/(im-1)) */
+    double nue_max;             /* NUE_MAX [m2/s], eddy-viscosity ceiling (default 1e5)          */
+    double vel_star_max;        /* VEL_STAR_MAX [m/s], u_tau ceiling (default sqrt(0.002) 200)   */
+    double abl_height;          /* boundary-layer height [m] (planet/cUranusModel.h:364: 20000) */
+} atu_turb_opts;
+
+/* The reference's defaults (every knob unset). */
+int atu_turb_opts_default(atu_turb_opts* out);
+int atu_turbulence_init(atu_ctx* ctx, const atu_turb_opts* opts);
+int atu_turbulence_run(atu_ctx* ctx, const atu_turb_opts* opts);
+
+typedef enum atu_turb_array {
+    ATU_TURB_PROD = 0, ATU_TURB_TKEN = 1, ATU_TURB_DISN = 2,   /* [im][jm][km]                   */
+    ATU_TURB_VEL_STAR = 3                                      /* [jm][km]                       */
+} atu_turb_array;
+/* Host copies of the closure's own arrays (zero until the first closure call or a set). */
+int atu_turb_set(atu_ctx* ctx, int which, const double* host);
+int atu_turb_get(atu_ctx* ctx, int which, double* host);
+
 /* Device pointer of a field's storage (for zero-copy interop); `ld_k` receives the k stride in
  * elements, `elem_bytes` 8 or 4. The layout is [im][jm][ld_k]. */
 int atu_field_device_ptr(atu_ctx* ctx, int field, void** dptr, int* ld_k, int* elem_bytes);

@@ -49,6 +49,7 @@ def load() -> C.CDLL:
         "orc_damp_wiggles": (C.c_int, [V, _DP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
         "orc_step_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
                                         C.POINTER(C.c_double)]),
+        "orc_turb": (C.c_int, [V, V, V, C.c_int, _DP, _DP, _DP, _DP, _DP]),
     }
     for n, (res, args) in sig.items():
         f = getattr(lib, n)
@@ -157,3 +158,31 @@ def damp_wiggles(grid, fields: np.ndarray, field: int, along_i: bool, along_j: b
     """AtomUtils::damp_wiggles on one field (no fluid floor, periodic phi seam)."""
     _ok(load().orc_damp_wiggles(_p(grid), _d(fields), field, int(along_i), int(along_j), int(along_k), strength,
                                 passes), "orc_damp_wiggles")
+
+
+class TurbOpts(C.Structure):
+    """atu_turb_opts (include/atu.h) with every default written out."""
+    _fields_ = [("model", C.c_int32), ("free_flow", C.c_int32), ("abl_taper", C.c_int32), ("curvature", C.c_int32),
+                ("k_bg", C.c_double), ("l_bg", C.c_double), ("nue_max", C.c_double), ("vel_star_max", C.c_double),
+                ("abl_height", C.c_double)]
+
+
+_TURB_MODELS = {"k_omega_SST": 0, "k_omega": 1, "k_epsilon": 2}
+
+
+def turb(grid, params, fields: np.ndarray, prod: np.ndarray, tken: np.ndarray, disn: np.ndarray,
+         vel_star: np.ndarray, init: bool = False, model: str = "k_omega_SST", free_flow: bool = True,
+         abl_taper: bool = False, curvature: bool = True, k_bg: float = 1.0, l_bg: float = 0.0,
+         nue_max: float = 1.0e5, vel_star_max: float = 0.0, abl_height: float = 20000.0) -> None:
+    """Turbulence<cUranusModel>::init (init) or ::run on the dense arrays, in place. The defaults are
+    the reference's (planet/Turbulence.h:125-156, 349-352; l_bg 0: L_atm / (im-1) in metres)."""
+    o = TurbOpts()
+    o.model = _TURB_MODELS[model]
+    o.free_flow, o.abl_taper, o.curvature = int(free_flow), int(abl_taper), int(curvature)
+    o.k_bg = k_bg
+    o.l_bg = l_bg if l_bg > 0.0 else params.L_atm * 1.0e3 / float(grid.im - 1)
+    o.nue_max = nue_max
+    o.vel_star_max = vel_star_max if vel_star_max > 0.0 else float(np.sqrt(0.002)) * 200.0
+    o.abl_height = abl_height
+    _ok(load().orc_turb(_p(grid), _p(params), _p(o), int(init), _d(fields), _d(prod), _d(tken), _d(disn),
+                        _d(vel_star)), "orc_turb")

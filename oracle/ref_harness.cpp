@@ -12,8 +12,11 @@
 //   mode step_uran `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolverUran)
 //   mode damp_t   AtomUtils::damp_wiggles on t: along j and k (strength 1), then along i (0.5)
 //   mode damp_uvw AtomUtils::damp_wiggles on u, v, w along i, j and k, strength 1, `sweeps` passes
+//   mode turb_run  TurbulenceUran(m).run()   (planet/Turbulence.h:306-332; knobs from the environment)
+//   mode turb_init TurbulenceUran(m).init()  (planet/Turbulence.h:282-303)
 // in/out: int32 im, jm, km, nfields, then nfields dense [im][jm][km] FP64 arrays in the atu_field
-// order of include/atu.h; out appends im doubles of buoy_ref_level.
+// order of include/atu.h; out appends im doubles of buoy_ref_level. The turb_* modes read and write
+// after those the closure's own arrays prod, tken, disn ([im][jm][km]) and vel_star ([jm][km]).
 
 #include <fenv.h>
 #include <omp.h>
@@ -47,6 +50,7 @@
 #include "ChemistryUran.h"
 #include "PressureSolver.h"
 #include "PressureSolverUran.h"
+#include "TurbulenceUran.h"
 #undef private
 #undef protected
 
@@ -117,6 +121,19 @@ int main(int argc, char** argv) {
             for (int j = 0; j < jm; j++)
                 for (int k = 0; k < km; k++) F[f]->x[i][j][k] = buf[((size_t)i * jm + j) * km + k];
     }
+    const bool turb_mode = mode.compare(0, 5, "turb_") == 0;
+    std::vector<Array*> T3 = {&m.prod, &m.tken, &m.disn};
+    if (turb_mode) {
+        for (Array* a : T3) {
+            in.read((char*)buf.data(), buf.size() * sizeof(double));
+            for (int i = 0; i < im; i++)
+                for (int j = 0; j < jm; j++)
+                    for (int k = 0; k < km; k++) a->x[i][j][k] = buf[((size_t)i * jm + j) * km + k];
+        }
+        in.read((char*)buf.data(), (size_t)jm * km * sizeof(double));
+        for (int j = 0; j < jm; j++)
+            for (int k = 0; k < km; k++) m.vel_star.y[j][k] = buf[(size_t)j * km + k];
+    }
     if (!in) {
         std::fprintf(stderr, "short input\n");
         return 2;
@@ -160,6 +177,10 @@ int main(int argc, char** argv) {
         AtomUtils::damp_wiggles(m.t, nullptr, true, false, false, 0.5, 1);
     } else if (mode == "damp_uvw") {
         for (Array* f : {&m.u, &m.v, &m.w}) AtomUtils::damp_wiggles(*f, nullptr, true, true, true, 1.0, sweeps);
+    } else if (mode == "turb_run") {
+        TurbulenceUran(m).run();
+    } else if (mode == "turb_init") {
+        TurbulenceUran(m).init();
     } else {
         std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
         return 2;
@@ -176,5 +197,16 @@ int main(int argc, char** argv) {
     std::vector<double> lev(im, 0.0);
     for (int i = 0; i < im && i < (int)m.buoy_ref_level.size(); i++) lev[i] = m.buoy_ref_level[i];
     out.write((const char*)lev.data(), lev.size() * sizeof(double));
+    if (turb_mode) {
+        for (Array* a : T3) {
+            for (int i = 0; i < im; i++)
+                for (int j = 0; j < jm; j++)
+                    for (int k = 0; k < km; k++) buf[((size_t)i * jm + j) * km + k] = a->x[i][j][k];
+            out.write((const char*)buf.data(), buf.size() * sizeof(double));
+        }
+        for (int j = 0; j < jm; j++)
+            for (int k = 0; k < km; k++) buf[(size_t)j * km + k] = m.vel_star.y[j][k];
+        out.write((const char*)buf.data(), (size_t)jm * km * sizeof(double));
+    }
     return out ? 0 : 1;
 }

@@ -44,6 +44,32 @@ CASES = {
     "step_uran_2": ("step_uran", 2, 1, True),
     "damp_t": ("damp_t", 1, 1, True),
     "damp_uvw_2": ("damp_uvw", 1, 2, True),
+    # the turbulence closure (planet/Turbulence.h) under the knobs of TURB_ENV
+    "turb_run": ("turb_run", 1, 1, False),
+    "turb_init": ("turb_init", 1, 1, False),
+    "turb_run_komega": ("turb_run", 1, 1, False),
+    "turb_run_keps": ("turb_run", 1, 1, False),
+    "turb_run_wall": ("turb_run", 1, 1, False),
+    "turb_init_wall_keps": ("turb_init", 1, 1, False),
+}
+# The reference reads the closure's knobs from its environment (planet/Turbulence.h:125-176,
+# planet/ATPhys.h env_for): ATURAN_TURB_MODEL, ATURAN_TURB_FREE, ATURAN_ABL_TAPER, ATURAN_TURB_CURV.
+TURB_ENV = {
+    "turb_run": {},
+    "turb_init": {},
+    "turb_run_komega": {"ATURAN_TURB_MODEL": "k_omega"},
+    "turb_run_keps": {"ATURAN_TURB_MODEL": "k_epsilon"},
+    "turb_run_wall": {"ATURAN_TURB_FREE": "0", "ATURAN_ABL_TAPER": "1", "ATURAN_TURB_CURV": "0"},
+    "turb_init_wall_keps": {"ATURAN_TURB_FREE": "0", "ATURAN_TURB_MODEL": "k_epsilon"},
+}
+# The same knobs as keyword arguments of the library's and the oracle's closure calls.
+TURB_KNOBS = {
+    "turb_run": {},
+    "turb_init": {},
+    "turb_run_komega": {"model": "k_omega"},
+    "turb_run_keps": {"model": "k_epsilon"},
+    "turb_run_wall": {"free_flow": False, "abl_taper": True, "curvature": False},
+    "turb_init_wall_keps": {"free_flow": False, "model": "k_epsilon"},
 }
 
 
@@ -87,6 +113,30 @@ def summarize(F: np.ndarray, level: np.ndarray) -> dict:
     return {"samples": F[:, ii, jj, kk].copy(), "sums": sums, "sumsq": sq, "level": level.copy()}
 
 
+def summarize_turb(T3: np.ndarray, vel_star: np.ndarray) -> dict:
+    """The closure's own arrays (prod, tken, disn and the [jm][km] friction velocity) as summarize()."""
+    ii, jj, kk = np.ix_(SAMPLE_I, SAMPLE_J, SAMPLE_K)
+    arrs = list(T3) + [vel_star]
+    return {"turb_samples": T3[:, ii, jj, kk].copy(),
+            "turb_vs_samples": vel_star[np.ix_(SAMPLE_J, SAMPLE_K)].copy(),
+            "turb_sums": np.array([math.fsum(a.ravel().tolist()) for a in arrs]),
+            "turb_sumsq": np.array([math.fsum((a.ravel() ** 2).tolist()) for a in arrs])}
+
+
+def turb_input(grid, seed: int = SEED) -> tuple[np.ndarray, np.ndarray]:
+    """Seeded prod, tken, disn and vel_star inputs of the closure cases (integer hashing only)."""
+    shp = (grid.im, grid.jm, grid.km)
+    T3 = np.stack([1e-3 * hash_uniform(60 + n, shp, seed) for n in range(3)])
+    vs = 0.05 + 0.1 * hash_uniform(63, (grid.jm, grid.km), seed)
+    return T3, vs
+
+
+def write_turb_bin(path: Path, T3: np.ndarray, vs: np.ndarray) -> None:
+    with open(path, "ab") as f:
+        f.write(np.ascontiguousarray(T3).tobytes())
+        f.write(np.ascontiguousarray(vs).tobytes())
+
+
 def write_bin(path: Path, F: np.ndarray) -> None:
     hdr = np.array([F.shape[1], F.shape[2], F.shape[3], F.shape[0]], dtype=np.int32)
     with open(path, "wb") as f:
@@ -101,6 +151,15 @@ def read_bin(path: Path, nf: int, shape) -> tuple[np.ndarray, np.ndarray]:
     F = body[: nf * n3].reshape((nf,) + tuple(shape)).copy()
     level = body[nf * n3: nf * n3 + shape[0]].copy()
     return F, level
+
+
+def read_turb_tail(path: Path, nf: int, shape) -> tuple[np.ndarray, np.ndarray]:
+    body = np.fromfile(path, dtype=np.uint8)[16:].view(np.float64)
+    n3 = int(np.prod(shape))
+    o = nf * n3 + shape[0]
+    T3 = body[o: o + 3 * n3].reshape((3,) + tuple(shape)).copy()
+    vs = body[o + 3 * n3: o + 3 * n3 + shape[1] * shape[2]].reshape(shape[1], shape[2]).copy()
+    return T3, vs
 
 
 def case_input(name: str, grid, params) -> np.ndarray:
@@ -126,13 +185,18 @@ def main() -> None:
             F = case_input(name, g, p)
             fin, fout = Path(td) / "in.bin", Path(td) / "out.bin"
             write_bin(fin, F)
+            if name in TURB_ENV:
+                write_turb_bin(fin, *turb_input(g))
+            env = {"OMP_NUM_THREADS": "8", "PATH": "/usr/bin", **TURB_ENV.get(name, {})}
             rr = subprocess.run([str(harness), str(fin), str(fout), mode, str(steps), str(sweeps)],
-                                capture_output=True, text=True, env={"OMP_NUM_THREADS": "8", "PATH": "/usr/bin"})
+                                capture_output=True, text=True, env=env)
             if rr.returncode != 0:
                 raise SystemExit(f"harness {name} failed:\n{rr.stderr[-2000:]}")
             consts = [ln for ln in rr.stderr.splitlines() if ln.startswith("ref_harness:")]
             G, level = read_bin(fout, A.NFIELDS, shape)
             s = summarize(G, level)
+            if name in TURB_ENV:
+                s.update(summarize_turb(*read_turb_tail(fout, A.NFIELDS, shape)))
             np.savez_compressed(HERE / f"ref_{name}.npz", **s, mode=mode, steps=steps, sweeps=sweeps,
                                 consts=np.array(consts))
             print(f"wrote ref_{name}.npz ({consts[0] if consts else ''})", flush=True)

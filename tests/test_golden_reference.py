@@ -1,9 +1,11 @@
 """Pins the CPU oracle to golden vectors written by the compiled reference (tests/golden/ref_*.npz,
 from tests/golden/make_golden.py): the domain BC, one RungeKuttaUran (hydro prep + 4 stages), the
 red-black PressureSolver at 1 and 20 sweeps, two full steps, and the default PressureSolverUran after
-one RungeKuttaUran and in two full steps, on an input in which every term of
+one RungeKuttaUran and in two full steps, the Shapiro wiggle filter, and the turbulence closure
+(Turbulence::run and ::init under six sets of its knobs), on an input in which every term of
 the path is non-zero. Per field: the sampled cells within 1e-12 of the field's largest sample, and
-the exact sum and sum of squares within 1e-12 relative."""
+the exact sum and sum of squares within 1e-12 relative (the closure's own arrays prod, tken, disn and
+vel_star likewise, and exactly)."""
 import math
 import sys
 from pathlib import Path
@@ -24,6 +26,10 @@ def run_oracle(oracle, name, g, p):
     F = MG.case_input(name, g, p)
     mode, steps, sweeps, _ = MG.CASES[name]
     level = np.zeros(g.im)
+    if mode in ("turb_run", "turb_init"):
+        T3, vs = MG.turb_input(g)
+        oracle.turb(g, p, F, T3[0], T3[1], T3[2], vs, init=(mode == "turb_init"), **MG.TURB_KNOBS[name])
+        return F, level, (T3, vs)
     if mode == "bc":
         oracle.bc(g, p, F)
     elif mode == "rk4":
@@ -53,7 +59,7 @@ def run_oracle(oracle, name, g, p):
             oracle.rk4(g, p, F)
             oracle.bc(g, p, F)
             oracle.pressure(g, p, F, sweeps, 1.0)
-    return F, level
+    return F, level, None
 
 
 @pytest.mark.slow
@@ -62,8 +68,12 @@ def test_oracle_matches_compiled_reference(oracle, name):
     ref = np.load(GOLDEN / f"ref_{name}.npz")
     g = A.default_grid(1)
     p = A.default_params(g)
-    F, level = run_oracle(oracle, name, g, p)
+    F, level, turb = run_oracle(oracle, name, g, p)
     got = MG.summarize(F, level)
+    if turb is not None:
+        t = MG.summarize_turb(*turb)
+        for key in ("turb_samples", "turb_vs_samples", "turb_sums", "turb_sumsq"):
+            assert np.array_equal(t[key], ref[key]), (name, key)
     worst = {}
     for f, fname in enumerate(A.FIELD_NAMES):
         a, b = got["samples"][f], ref["samples"][f]

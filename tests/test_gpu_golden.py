@@ -1,8 +1,9 @@
 """The HIP path against the compiled reference's golden vectors (tests/golden/ref_*.npz) on the
 enriched input in which every term of the step is non-zero: domain BC, one RungeKuttaUran (hydro
 prep + 4 stages), PressureSolver at 1 and 20 sweeps, two full steps, and the default
-PressureSolverUran after one RungeKuttaUran and in two full steps. FP64 bar: 1e-12 of each
-field's largest sampled magnitude, and 1e-12 relative on the exact field sums."""
+PressureSolverUran after one RungeKuttaUran and in two full steps, the Shapiro filter, and the
+turbulence closure. FP64 bar: 1e-12 of each field's largest sampled magnitude, and 1e-12 relative on
+the exact field sums; the closure's own arrays (prod, tken, disn, vel_star) the same."""
 import sys
 from pathlib import Path
 
@@ -28,9 +29,20 @@ def test_gpu_matches_compiled_reference(oracle, name):
     F = MG.case_input(name, g, p)
     mode, steps, sweeps, _ = MG.CASES[name]
     level = np.zeros(g.im)
+    turb = None
     with A.UranusCore(g, p) as core:
         core.set_state(F)
-        if mode == "bc":
+        if mode in ("turb_run", "turb_init"):
+            T3, vs = MG.turb_input(g)
+            for n, a in zip(("prod", "tken", "disn"), T3):
+                core.set_turb_array(n, a)
+            core.set_turb_array("vel_star", vs)
+            if mode == "turb_init":
+                core.Turbulence_init(**MG.TURB_KNOBS[name])
+            else:
+                core.Turbulence_run(**MG.TURB_KNOBS[name])
+            turb = (np.stack([core.turb_array(n) for n in ("prod", "tken", "disn")]), core.turb_array("vel_star"))
+        elif mode == "bc":
             core.boundary_conditions()
         elif mode == "rk4":
             level = core.computeHydrostaticPressure()
@@ -67,4 +79,15 @@ def test_gpu_matches_compiled_reference(oracle, name):
             assert x == y or abs(x - y) <= TOL * abs(y), (name, fname, key, x, y)
     if mode in ("rk4", "rk4_uran"):
         assert np.array_equal(got["level"], ref["level"])
+    if turb is not None:
+        t = MG.summarize_turb(*turb)
+        for q in range(3):
+            b = ref["turb_samples"][q]
+            e = float(np.max(np.abs(t["turb_samples"][q] - b))) / (float(np.max(np.abs(b))) or 1.0)
+            worst = max(worst, e)
+            assert e <= TOL, (name, "turb", q, e)
+        assert np.array_equal(t["turb_vs_samples"], ref["turb_vs_samples"]), name
+        for key in ("turb_sums", "turb_sumsq"):
+            x, y = t[key], ref[key]
+            assert np.all((x == y) | (np.abs(x - y) <= TOL * np.abs(y))), (name, key, x, y)
     print(f"[golden-gpu] {name}: worst sampled difference {worst:.3e}", flush=True)
