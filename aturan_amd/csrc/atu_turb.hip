@@ -12,8 +12,8 @@
 //                     wall copy of i = 0 (:414-418), which only they read
 //   k_turb_clamp      the clamp-out of tke and dis (deferred: neighbours read them in the loop) and
 //                     clamp_nue of the interior columns' end cells
-//   k_turb_edges      the columns outside the loop (the pole rows and the seam meridian): wall copy
-//                     and clamp_nue, one thread per column
+//   k_turb_edge_wall, k_turb_edge_clamp   the columns outside the loop (the pole rows and the seam
+//                     meridian): the wall copy, then clamp_nue
 //   k_turb_seam, k_turb_lid   seam_copy (:542-561) and fill_lid_corners (:510-536)
 // init() (:282-303) adds k_turb_init (init_fields, :426-495), a seam copy and a wall copy before
 // the sources, and takes no wall copy after them.
@@ -66,9 +66,9 @@ __global__ void __launch_bounds__(256) k_turb_vel_star(Shape sh, TurbArgs a) {
 // init_fields (:426-495) over the interior; with TURB_FREE every cell takes the background values
 // the host formed.
 __global__ void __launch_bounds__(256) k_turb_init(Shape sh, TurbArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y + 1, i = blockIdx.z + 1;
-    if (k >= sh.km - 1) return;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1, i = blockIdx.z + 1;
+    if (k >= sh.km - 1 || j > sh.jm - 2) return;
     const size_t q = (size_t)i * sh.plane + (size_t)j * sh.ldk + k;
     double tke, dis;
     if (a.free_flow) {
@@ -133,10 +133,19 @@ __device__ __forceinline__ SrcOut sst_cell(const TurbArgs& a, int i, double rm, 
                              4.0 * sig_w2 * kc / (CD_kw * y_star * y_star));
     const double arg2 = smax(2.0 * sqrt(kc) / (bet_star * dis_safe * y_star),
                              500.0 * a.nue_air_nd / (y_star * y_star * dis_safe));
-    const double F1 = tanh(pow(arg1, 4.0));
-    const double F2 = tanh(arg2 * arg2);   // std::pow(x, 2), which g++ and clang both form as x * x
+    // F1 = tanh(arg1^4) enters only as F1 gam1 + (1 - F1) gam2, F1 bet1 + (1 - F1) bet2 and
+    // 2 (1 - F1). Below arg1 = 2^-15, F1 <= arg1^4 < 2^-60 is under half an ulp of 1, of gam2 and of
+    // bet2, so each of those rounds to its value at F1 = 0: the pow and the tanh are skipped there
+    // (away from walls, e.g. the free-flow default, arg1 ~ 1e-16) with the same bits.
+    const double F1 = (arg1 < 0x1p-15) ? 0.0 : tanh(pow(arg1, 4.0));
+    // F2 = tanh(y), y = std::pow(arg2, 2), which g++ and clang both form as arg2 * arg2. F2 <= y, so
+    // where Omega y is at most half of a1 dis_safe the larger of a1 dis_safe and Omega F2 is
+    // a1 dis_safe whatever the last bit of F2: the tanh is skipped there.
+    const double y2 = arg2 * arg2;
+    const double lhs = a1 * dis_safe;
+    const double rhs = (Omega * y2 <= 0.5 * lhs) ? 0.0 : Omega * tanh(y2);
     SrcOut o;
-    o.nue = a1 * kc / smax(a1 * dis_safe, Omega * F2);
+    o.nue = a1 * kc / smax(lhs, rhs);
     const double tke_safe = smax(kc, 1.0e-20);
     const double P_k = smin(prod, 20.0 * bet_star * tke_safe * dis_safe);
     const double Y_k = bet_star * tke_safe * dis_safe;
@@ -227,10 +236,10 @@ __device__ __forceinline__ SrcOut keps_cell(const TurbArgs& a, int i, double vs,
 // The source loop over the interior (i 1..im-2, j 1..jm-2, k 0..km-2 on the periodic ring).
 template <int MODEL>
 __global__ void __launch_bounds__(256) k_turb_sources(Shape sh, TurbArgs a, int wall_after) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y + 1, i = blockIdx.z + 1;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1, i = blockIdx.z + 1;
     const int km = sh.km;
-    if (k >= km - 1) return;
+    if (k >= km - 1 || j > sh.jm - 2) return;
     const int kw = (k == 0) ? km - 2 : k - 1, ke = (k == km - 2) ? 0 : k + 1;   // the ring of km-1 meridians
     const size_t c = (size_t)i * sh.plane + (size_t)j * sh.ldk;
     const size_t q = c + k;
@@ -337,9 +346,9 @@ __global__ void __launch_bounds__(256) k_turb_sources(Shape sh, TurbArgs a, int 
 // The clamp-out of tke and dis over the interior (:636-647) and clamp_nue of the interior
 // columns' end cells (i = im-1 always, i = 0 in init(), where the wall copy came first).
 __global__ void __launch_bounds__(256) k_turb_clamp(Shape sh, TurbArgs a, int wall_after) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y + 1, i = blockIdx.z;
-    if (k >= sh.km - 1) return;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1, i = blockIdx.z;
+    if (k >= sh.km - 1 || j > sh.jm - 2) return;
     const size_t q = (size_t)i * sh.plane + (size_t)j * sh.ldk + k;
     if (i >= 1 && i <= sh.im - 2) {
         const double vs = a.vel_star[(size_t)j * sh.ldk + k];
@@ -352,12 +361,9 @@ __global__ void __launch_bounds__(256) k_turb_clamp(Shape sh, TurbArgs a, int wa
 }
 
 // The columns the loop does not reach: the pole rows j = 0, jm-1 (every k) and the seam meridian
-// k = km-1 (j = 1..jm-2). In run() the wall copy at i = 0 from i = 1; then clamp_nue down the column.
-__global__ void __launch_bounds__(64) k_turb_edges(Shape sh, TurbArgs a, int wall_after) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// k = km-1 (j = 1..jm-2); column t of the 2 km + jm - 2.
+__device__ __forceinline__ size_t edge_column(const Shape& sh, int t) {
     const int km = sh.km, jm = sh.jm;
-    const int ncol = 2 * km + (jm - 2);
-    if (t >= ncol) return;
     int j, k;
     if (t < 2 * km) {
         j = (t < km) ? 0 : jm - 1;
@@ -366,27 +372,39 @@ __global__ void __launch_bounds__(64) k_turb_edges(Shape sh, TurbArgs a, int wal
         j = 1 + (t - 2 * km);
         k = km - 1;
     }
-    const size_t col = (size_t)j * sh.ldk + k;
-    if (wall_after) {
-        a.tke[col] = a.tke[sh.plane + col];
-        a.dis[col] = a.dis[sh.plane + col];
-        a.nue[col] = a.nue[sh.plane + col];
-    }
-    for (int i = 0; i < sh.im; i++) {
-        const size_t q = (size_t)i * sh.plane + col;
-        a.nue[q] = clamp3(a.nue[q], 0.0, a.nue_max_nd) * a.taper[i];
-    }
+    return (size_t)j * sh.ldk + k;
+}
+
+// run(): the wall copy of the edge columns at i = 0 from i = 1 and clamp_nue of the copied cell
+// (i = 1 is clamped by k_turb_edge_clamp, launched after this kernel).
+__global__ void __launch_bounds__(256) k_turb_edge_wall(Shape sh, TurbArgs a) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * sh.km + (sh.jm - 2)) return;
+    const size_t col = edge_column(sh, t);
+    a.tke[col] = a.tke[sh.plane + col];
+    a.dis[col] = a.dis[sh.plane + col];
+    a.nue[col] = clamp3(a.nue[sh.plane + col], 0.0, a.nue_max_nd) * a.taper[0];
+}
+
+// clamp_nue down the edge columns from level i0 (1 after k_turb_edge_wall, else 0).
+__global__ void __launch_bounds__(256) k_turb_edge_clamp(Shape sh, TurbArgs a, int i0) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y + i0;
+    if (t >= 2 * sh.km + (sh.jm - 2)) return;
+    const size_t q = (size_t)i * sh.plane + edge_column(sh, t);
+    a.nue[q] = clamp3(a.nue[q], 0.0, a.nue_max_nd) * a.taper[i];
 }
 
 // seam_copy (:542-561): k = km-1 takes k = 0 in the closure's eight arrays.
-__global__ void __launch_bounds__(256) k_turb_seam(Shape sh, TurbArgs a) {
+// One thread per (array, row): 8 x im x jm copies.
+__global__ void __launch_bounds__(64) k_turb_seam(Shape sh, TurbArgs a) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
+    const int i = blockIdx.y, n = blockIdx.z;
     if (j >= sh.jm) return;
     const size_t r = (size_t)i * sh.plane + (size_t)j * sh.ldk;
-    double* f[8] = {a.tke, a.dis, a.tken, a.disn, a.nue, a.prod, a.tke_source, a.dis_source};
-#pragma unroll
-    for (int n = 0; n < 8; n++) f[n][r + sh.km - 1] = f[n][r];
+    double* f = n == 0 ? a.tke : n == 1 ? a.dis : n == 2 ? a.tken : n == 3 ? a.disn : n == 4 ? a.nue
+              : n == 5 ? a.prod : n == 6 ? a.tke_source : a.dis_source;
+    f[r + sh.km - 1] = f[r];
 }
 
 // fill_lid_corners with the model's own boundaries off (:515-527): the lid i = im-1 (j = 1..jm-2)
@@ -422,27 +440,32 @@ hipError_t launch_turbulence(const Shape& sh, const TurbArgs& a, bool init, hipS
     const dim3 gk((sh.km + bk - 1) / bk);
     hipLaunchKernelGGL(k_turb_vel_star, dim3(gk.x, sh.jm), dim3(bk), 0, s, sh, a);
     const int wall_after = init ? 0 : 1;
+    // the interior passes: tiles of 64 meridians x 4 rows (the ring of km-1 meridians in 64-wide waves)
+    const dim3 tile(64, 4);
+    const int tx = (sh.km - 1 + 63) / 64, ty = (sh.jm - 2 + 3) / 4;
     if (init) {
-        hipLaunchKernelGGL(k_turb_init, dim3(gk.x, sh.jm - 2, sh.im - 2), dim3(bk), 0, s, sh, a);
-        hipLaunchKernelGGL(k_turb_seam, dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, a);
+        hipLaunchKernelGGL(k_turb_init, dim3(tx, ty, sh.im - 2), tile, 0, s, sh, a);
+        hipLaunchKernelGGL(k_turb_seam, dim3((sh.jm + 63) / 64, sh.im, 8), dim3(64), 0, s, sh, a);
         hipLaunchKernelGGL(k_turb_wall, dim3(gk.x, sh.jm), dim3(bk), 0, s, sh, a);
     }
-    const dim3 gsrc(gk.x, sh.jm - 2, sh.im - 2);
+    const dim3 gsrc(tx, ty, sh.im - 2);
     switch (a.model) {
         case ATU_TURB_K_OMEGA:
-            hipLaunchKernelGGL((k_turb_sources<ATU_TURB_K_OMEGA>), gsrc, dim3(bk), 0, s, sh, a, wall_after);
+            hipLaunchKernelGGL((k_turb_sources<ATU_TURB_K_OMEGA>), gsrc, tile, 0, s, sh, a, wall_after);
             break;
         case ATU_TURB_K_EPSILON:
-            hipLaunchKernelGGL((k_turb_sources<ATU_TURB_K_EPSILON>), gsrc, dim3(bk), 0, s, sh, a, wall_after);
+            hipLaunchKernelGGL((k_turb_sources<ATU_TURB_K_EPSILON>), gsrc, tile, 0, s, sh, a, wall_after);
             break;
         default:
-            hipLaunchKernelGGL((k_turb_sources<ATU_TURB_K_OMEGA_SST>), gsrc, dim3(bk), 0, s, sh, a, wall_after);
+            hipLaunchKernelGGL((k_turb_sources<ATU_TURB_K_OMEGA_SST>), gsrc, tile, 0, s, sh, a, wall_after);
             break;
     }
-    hipLaunchKernelGGL(k_turb_clamp, dim3(gk.x, sh.jm - 2, sh.im), dim3(bk), 0, s, sh, a, wall_after);
+    hipLaunchKernelGGL(k_turb_clamp, dim3(tx, ty, sh.im), tile, 0, s, sh, a, wall_after);
     const int ncol = 2 * sh.km + (sh.jm - 2);
-    hipLaunchKernelGGL(k_turb_edges, dim3((ncol + 63) / 64), dim3(64), 0, s, sh, a, wall_after);
-    hipLaunchKernelGGL(k_turb_seam, dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, a);
+    if (wall_after) hipLaunchKernelGGL(k_turb_edge_wall, dim3((ncol + 255) / 256), dim3(256), 0, s, sh, a);
+    hipLaunchKernelGGL(k_turb_edge_clamp, dim3((ncol + 255) / 256, sh.im - wall_after), dim3(256), 0, s, sh, a,
+                       wall_after);
+    hipLaunchKernelGGL(k_turb_seam, dim3((sh.jm + 63) / 64, sh.im, 8), dim3(64), 0, s, sh, a);
     if (a.free_flow)
         hipLaunchKernelGGL(k_turb_lid, dim3(gk.x, sh.jm + 2 * (sh.im - 1)), dim3(bk), 0, s, sh, a);
     return hipGetLastError();

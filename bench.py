@@ -56,6 +56,7 @@ def parse():
                     help="no HIP events in the timed region and no per-class pass (for runs under a profiler)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="window of the adjoint timing (0: skip)")
     ap.add_argument("--checkpoint-every", type=int, default=10)
+    ap.add_argument("--turb-calls", type=int, default=50, help="turbulence-closure calls timed (0: skip)")
     ap.add_argument("--refined", action="store_true",
                     help="the 81x361x721 workload instead: mixed vs FP64 over --refined-steps steps and the "
                          "mixed vs FP64 adjoint over --refined-window steps")
@@ -166,6 +167,52 @@ def mixed_errors(g, p, F, opts64, optsmx):
             b = {f: c64.get_field(f) for f in set(ERR_FIELDS.values())}
             res[str(n)] = rel_errors(a, b)
     return res
+
+
+def turb_bytes(g) -> int:
+    """Algorithmic HBM bytes of one Turbulence::run (k-omega SST, TURB_FREE 1) on the device: the source
+    loop reads u, v, w, tke, dis and writes prod, nue and the two sources per interior cell; the clamp
+    pass reads and writes tke and dis; the friction velocity, the edge columns, the seam and the lid
+    are the small rest."""
+    im, jm, km = g.im, g.jm, g.km
+    n_int = (im - 2) * (jm - 2) * (km - 1)
+    cols_int = (jm - 2) * (km - 1)
+    b = n_int * (9 * 8 + 4 * 8)                   # k_turb_sources, k_turb_clamp (interior)
+    b += cols_int * 2 * 16 + jm * km * 8 * 2      # clamp of the end cells; vel_star write + read
+    b += (2 * km + jm - 2) * (im * 16 + 6 * 8)    # k_turb_edges
+    b += 8 * im * jm * 16                         # k_turb_seam
+    b += 8 * (jm + 2 * (im - 1)) * km * 16        # k_turb_lid
+    return b
+
+
+def turb_bench(g, p, F, calls, events=True):
+    """Turbulence::run (k-omega SST, the reference's defaults) on the step's state: wall time per
+    call (which includes refreshing the stage buffers of tke and dis) and the closure's kernels."""
+    import aturan_amd as A
+    with A.UranusCore(g, p, "fp64") as core:
+        core.set_state(F)
+        core.Turbulence_init()
+        core.Turbulence_run()
+        core.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            core.Turbulence_run()
+        core.synchronize()
+        wall = (time.perf_counter() - t0) / calls
+        kms = None
+        if events:
+            core.profile_begin("turbulence")
+            for _ in range(calls):
+                core.Turbulence_run()
+            kms = core.profile_query("turbulence")[0] / calls
+            core.profile_end()
+    b = turb_bytes(g)
+    out = {"closure": "k-omega SST, TURB_FREE 1 (Turbulence<cUranusModel>::run)", "calls": calls,
+           "ms_per_call": round(wall * 1e3, 4), "algorithmic_bytes_per_call": b}
+    if kms:
+        out.update(kernels_ms_per_call=round(kms, 4), kernels_GBps=round(b / (kms * 1e-3) / 1e9, 1),
+                   kernels_frac=round(b / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
+    return out
 
 
 def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s, events=True):
@@ -397,6 +444,8 @@ def main():
     if args.adjoint_steps > 0 and args.mode in ("both", "fp64"):
         line["adjoint"] = adjoint_bench(g, p, F, args.adjoint_steps, args.checkpoint_every, args.tol,
                                         line["ms_per_step"] / 1e3 if line["ms_per_step"] else None, not args.no_events)
+    if args.turb_calls > 0 and args.mode in ("both", "fp64"):
+        line["turbulence"] = turb_bench(g, p, F, args.turb_calls, not args.no_events)
     line["config"] = cfg
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds)
