@@ -85,6 +85,19 @@ class TurbOpts(C.Structure):
                 ("abl_height", C.c_double)]
 
 
+class Gas(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t_0", "t_00", "ep", "lv", "ls", "C", "L0", "R", "del_alf", "del_bet",
+                                          "C_i", "L0_i", "del_alf_i", "del_bet_i")]
+
+
+class SatAdjOpts(C.Structure):
+    _fields_ = [("gas", Gas * 4), ("m_nh3", C.c_double), ("m_h2s", C.c_double), ("m_nh4sh", C.c_double),
+                ("gas_mask", C.c_int32), ("nh4sh_eq", C.c_int32), ("conserve", C.c_int32), ("newton", C.c_int32),
+                ("evap", C.c_int32), ("ice_entry", C.c_int32), ("bergeron", C.c_int32), ("freeze", C.c_int32),
+                ("dry_exit", C.c_int32), ("pad_", C.c_int32)]
+
+
+GASES = {"H2O": 0, "H2S": 1, "NH3": 2, "CH4": 3}
 TURB_MODELS = {"k_omega_SST": 0, "k_omega": 1, "k_epsilon": 2}
 TURB_ARRAYS = {"prod": 0, "tken": 1, "disn": 2, "vel_star": 3}
 
@@ -116,6 +129,8 @@ _SIGS = {
     "atu_window_rewind": (C.c_int, [C.c_void_p]),
     "atu_build_info": (C.c_char_p, []),
     "atu_damp_wiggles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
+    "atu_satadj_opts_default": (C.c_int, [C.POINTER(SatAdjOpts)]),
+    "atu_saturation_adjust": (C.c_int, [C.c_void_p, C.POINTER(SatAdjOpts)]),
     "atu_turb_opts_default": (C.c_int, [C.POINTER(TurbOpts)]),
     "atu_turbulence_init": (C.c_int, [C.c_void_p, C.POINTER(TurbOpts)]),
     "atu_turbulence_run": (C.c_int, [C.c_void_p, C.POINTER(TurbOpts)]),

@@ -56,6 +56,19 @@ def cp_table(mode: int = 2) -> np.ndarray:
     return out
 
 
+def satadj_opts(gases=("H2O", "H2S", "NH3", "CH4"), nh4sh_eq: bool = True, **knobs) -> L.SatAdjOpts:
+    """The saturation-adjustment options: the reference's gas constants and knobs, the listed gases."""
+    o = L.SatAdjOpts()
+    _check(L.load().atu_satadj_opts_default(C.byref(o)), None, "atu_satadj_opts_default")
+    o.gas_mask = sum(1 << L.GASES[g] for g in gases)
+    o.nh4sh_eq = int(nh4sh_eq)
+    for k, v in knobs.items():
+        if k not in ("conserve", "newton", "evap", "ice_entry", "bergeron", "freeze", "dry_exit"):
+            raise ValueError(f"unknown saturation-adjustment knob {k}")
+        setattr(o, k, int(v))
+    return o
+
+
 def turb_opts(model: str = "k_omega_SST", free_flow: bool = True, abl_taper: bool = False, curvature: bool = True,
               k_bg: float = 0.0, l_bg: float = 0.0, nue_max: float = 0.0, vel_star_max: float = 0.0,
               abl_height: float = 0.0) -> L.TurbOpts:
@@ -185,6 +198,14 @@ class UranusCore:
         f = L.FIELD[field] if isinstance(field, str) else int(field)
         _check(self._lib.atu_damp_wiggles(self._ctx, f, int(along_i), int(along_j), int(along_k), strength, passes),
                self._ctx, "atu_damp_wiggles")
+
+    # ---- the saturation-adjustment block of the reference loop ----
+    def SaturationAdjustment(self, gases=("H2O", "H2S", "NH3", "CH4"), nh4sh_eq: bool = True, **knobs) -> None:
+        """The loop's four SaturationAdjustmentUran calls (in the loop's order, for the listed gases)
+        and ChemistryUran::EquilibriumNH4SH("loop") (planet/cUranusModel.cpp:997-1033). Knobs: conserve,
+        newton, evap, ice_entry, bergeron, freeze, dry_exit (the reference's ATURAN_SATADJ_*)."""
+        o = satadj_opts(gases, nh4sh_eq, **knobs)
+        _check(self._lib.atu_saturation_adjust(self._ctx, C.byref(o)), self._ctx, "atu_saturation_adjust")
 
     # ---- the turbulence closure, Turbulence<cUranusModel> (planet/Turbulence.h) ----
     def Turbulence_init(self, **knobs) -> None:

@@ -27,8 +27,8 @@ struct ProfEvent {
 };
 
 const char* kClassNames[] = {"rk_stage", "rb_pass", "hydro", "bc", "proj_src", "residual", "line", "baro", "adjoint",
-                             "step", "turbulence"};
-enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_ADJ, PC_STEP, PC_TURB, PC_N };
+                             "step", "turbulence", "satadj"};
+enum { PC_RK = 0, PC_RB, PC_HYDRO, PC_BC, PC_PROJ, PC_RES, PC_LINE, PC_BARO, PC_ADJ, PC_STEP, PC_TURB, PC_SATADJ, PC_N };
 
 }  // namespace
 
@@ -1244,6 +1244,51 @@ int atu_get_field(atu_ctx* ctx, int f, double* host) {
     if (!e) e = get_field(ctx, d, host);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
+}
+
+int atu_saturation_adjust(atu_ctx* ctx, const atu_satadj_opts* opts) {
+    if (!ctx) return ATU_EINVAL;
+    atu_satadj_opts o;
+    if (opts) o = *opts;
+    else atu_satadj_opts_default(&o);
+    if (o.gas_mask < 0 || o.gas_mask > 15 || o.bergeron < 0 || o.bergeron > 2)
+        return ctx->fail(ATU_EINVAL, "atu_saturation_adjust: option out of range");
+    for (int n = 0; n < 4; n++)
+        if ((o.gas_mask & (1 << n)) && !(o.gas[n].t_0 != o.gas[n].t_00))
+            return ctx->fail(ATU_EINVAL, "atu_saturation_adjust: t_0 == t_00");
+    ctx->state_epoch++;
+    SatAdjArgs a{};
+    a.y = ctx->Y;
+    if (ctx->mixed()) {
+        a.y32 = ctx->Y32;
+        a.sa32 = ctx->SA32;
+        a.sb32 = ctx->SB32;
+    } else {
+        a.sa = ctx->SA;
+        a.sb = ctx->SB;
+    }
+    a.p_stat = ctx->PSTAT;
+    a.rho_mix = field_ptr(ctx, ATU_RHO_MIX);
+    a.fs = ctx->sh.fs;
+    for (int n = 0; n < 4; n++) a.gas[n] = o.gas[n];
+    a.gas_mask = o.gas_mask;
+    a.nh4sh_eq = o.nh4sh_eq;
+    a.conserve = o.conserve;
+    a.newton = o.newton;
+    a.evap = o.evap;
+    a.ice_entry = o.ice_entry;
+    a.bergeron = o.bergeron;
+    a.freeze = o.freeze;
+    a.dry_exit = o.dry_exit;
+    a.t_ref = ctx->params.t_ref;
+    a.cp_mix = ctx->params.cp_mix;
+    a.r_mix = ctx->params.r_mix;
+    a.R_mix = ctx->params.R_mix;
+    a.m_nh3 = o.m_nh3;
+    a.m_h2s = o.m_h2s;
+    a.m_nh4sh = o.m_nh4sh;
+    a.nh4sh_heat = 4705.0 * std::log(10.0) * 8314.46 / 51.1114;   // nh4sh_formation_heat, planet/ChemistryUran.h:132
+    return ctx->hip(ctx->timed(PC_SATADJ, [&] { return launch_satadj(ctx->sh, a, ctx->stream); }), "satadj");
 }
 
 int atu_turbulence_init(atu_ctx* ctx, const atu_turb_opts* opts) { return do_turbulence(ctx, opts, true); }

@@ -503,6 +503,37 @@ extern "C" int atu_turb_opts_default(atu_turb_opts* o) {
     return ATU_OK;
 }
 
+// The gas constants the reference loop passes to its four SaturationAdjustmentUran calls
+// (planet/cUranusModel.cpp:998-1029 with planet/cUranusModel.h:880-906, 1094-1192) and the ice sets
+// its dispatch picks (planet/SaturationAdjustmentUran.cpp:222-231; H2S takes its liquid set), with
+// every knob unset (ATURAN_T00_LEGACY 0, ATURAN_LV_H2S unset, ATURAN_SVP_FIT 0).
+extern "C" int atu_satadj_opts_default(atu_satadj_opts* o) {
+    if (!o) return ATU_EINVAL;
+    std::memset(o, 0, sizeof *o);
+    o->gas[ATU_GAS_H2O] = atu_gas{273.15, 210.15, 8.1253, 2.5009e6, 2.8339e6, 25.096, 3148.2, 461.52, 0.0, -8.7e-3,
+                                  28.418, 3567.3, 0.0, -8.7e-3};
+    o->gas[ATU_GAS_H2S] = atu_gas{187.65, 140.0, 14.5192, 5.5e5, 7.4500e5, 17.064, 747.0, 243.96, 0.0, -2.9e-3,
+                                  17.064, 747.0, 0.0, -2.9e-3};
+    o->gas[ATU_GAS_NH3] = atu_gas{195.5, 140.0, 7.6752, 1.3720e6, 1.8320e6, 27.863, 2016.0, 488.21, -0.888, 0.0,
+                                  34.948, 2692.5, -0.888, 0.0};
+    o->gas[ATU_GAS_CH4] = atu_gas{90.69, 67.36, 7.6752, 5.11e5, 5.11e5, 1.627, 553.1, 518.28, 1.002, -4.1e-3,
+                                  1.627, 553.1, 1.002, -4.1e-3};
+    o->m_nh3 = 17.03052;
+    o->m_h2s = 34.08088;
+    o->m_nh4sh = 51.1114;
+    o->gas_mask = 15;
+    o->nh4sh_eq = 1;      // ATURAN_NH4SH_EQ, planet/ChemistryUran.h:134-137
+    // planet/cUranusModel.h:578-589 and planet/SaturationAdjustment.h:286-287, 300-301
+    o->conserve = 1;
+    o->newton = 1;
+    o->evap = 1;
+    o->ice_entry = 0;
+    o->bergeron = 2;
+    o->freeze = 1;
+    o->dry_exit = 1;
+    return ATU_OK;
+}
+
 extern "C" int atu_grid_default(int refine, atu_grid* out) {
     if (!out || (refine != 1 && refine != 2)) return ATU_EINVAL;
     std::memset(out, 0, sizeof(*out));

@@ -186,6 +186,21 @@ struct TurbArgs {
 };
 hipError_t launch_turbulence(const Shape& sh, const TurbArgs& a, bool init, hipStream_t s);
 
+// ---- the saturation-adjustment block (atu_satadj.hip) ----
+struct SatAdjArgs {
+    double* y;                  // FP64 prognostic masters (19 fields, `fs` apart)
+    double *sa, *sb;            // FP64 stage buffers (FP64 context) or null
+    float *y32, *sa32, *sb32;   // FP32 mirrors (mixed context) or null
+    const double *p_stat, *rho_mix;
+    size_t fs;
+    atu_gas gas[4];
+    int gas_mask, nh4sh_eq;
+    int conserve, newton, evap, ice_entry, bergeron, freeze, dry_exit, pad_;
+    double t_ref, cp_mix, r_mix, R_mix;
+    double m_nh3, m_h2s, m_nh4sh, nh4sh_heat;
+};
+hipError_t launch_satadj(const Shape& sh, const SatAdjArgs& a, hipStream_t s);
+
 // ---- PressureSolverUran (atu_uran.hip), the reference's default solver ----
 struct UranTabs {
     const double *a_inv_rm, *a_num2, *m_num2;                                  // [im]

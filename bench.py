@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=50, help="window of the adjoint timing (0: skip)")
     ap.add_argument("--checkpoint-every", type=int, default=10)
     ap.add_argument("--turb-calls", type=int, default=50, help="turbulence-closure calls timed (0: skip)")
+    ap.add_argument("--satadj-calls", type=int, default=20, help="saturation-adjustment calls timed (0: skip)")
     ap.add_argument("--refined", action="store_true",
                     help="the 81x361x721 workload instead: mixed vs FP64 over --refined-steps steps and the "
                          "mixed vs FP64 adjoint over --refined-window steps")
@@ -212,6 +213,56 @@ def turb_bench(g, p, F, calls, events=True):
     if kms:
         out.update(kernels_ms_per_call=round(kms, 4), kernels_GBps=round(b / (kms * 1e-3) / 1e9, 1),
                    kernels_frac=round(b / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4))
+    return out
+
+
+def satadj_state(g, p, F):
+    """The step's state with clouds: every gas's vapour log-uniform in 1e-9..1e-3 kg/m3 (sub- and
+    supersaturated cells), its cloud and ice 1e-5..1e-3 kg/m3 in a quarter of the cells each and both
+    in another quarter, NH4SH in half of them (seeded)."""
+    import aturan_amd as A
+    import numpy as np
+    rng = np.random.default_rng(20261203)
+    F = F.copy()
+    shp = F.shape[1:]
+    for gas in ("h2o", "h2s", "nh3", "ch4"):
+        F[A.FIELD[gas]] = 10.0 ** rng.uniform(-9.0, -3.0, shp)
+        sel = rng.integers(0, 4, shp)
+        F[A.FIELD[gas + "_cloud"]] = np.where((sel == 1) | (sel == 3), 10.0 ** rng.uniform(-5.0, -3.0, shp), 0.0)
+        F[A.FIELD[gas + "_ice"]] = np.where((sel == 2) | (sel == 3), 10.0 ** rng.uniform(-5.0, -3.0, shp), 0.0)
+    F[A.FIELD["nh4sh"]] = np.where(rng.random(shp) < 0.5, 10.0 ** rng.uniform(-6.0, -3.0, shp), 0.0)
+    return F
+
+
+def satadj_bench(g, p, F, calls, events=True):
+    """The loop's saturation-adjustment block (four gases + NH4SH equilibrium), each call from the
+    same cloudy state re-uploaded outside the timed kernels (events) or included (wall)."""
+    import aturan_amd as A
+    Fs = satadj_state(g, p, F)
+    nbytes = 30 * 8 * g.im * g.jm * g.km      # 16 fields read, 14 written
+    out = {"block": "SaturationAdjustmentUran x 4 (H2O, H2S, NH3, CH4) + EquilibriumNH4SH(loop)",
+           "state": "step state with seeded clouds (vapour 1e-9..1e-3 kg/m3, mixed phases)", "calls": calls,
+           "algorithmic_bytes_per_call": nbytes}
+    with A.UranusCore(g, p, "fp64") as core:
+        total = 0.0
+        kms = 0.0
+        for _ in range(calls):
+            core.set_state(Fs)
+            core.synchronize()
+            if events:
+                core.profile_begin("satadj")
+            t0 = time.perf_counter()
+            core.SaturationAdjustment()
+            core.synchronize()
+            total += time.perf_counter() - t0
+            if events:
+                kms += core.profile_query("satadj")[0]
+                core.profile_end()
+    out["ms_per_call"] = round(total / calls * 1e3, 4)
+    if events:
+        out["kernel_ms_per_call"] = round(kms / calls, 4)
+        out["cells_per_s"] = round(g.im * g.jm * g.km / (kms / calls * 1e-3), 1)
+        out["GBps"] = round(nbytes / (kms / calls * 1e-3) / 1e9, 1)
     return out
 
 
@@ -446,6 +497,8 @@ def main():
                                         line["ms_per_step"] / 1e3 if line["ms_per_step"] else None, not args.no_events)
     if args.turb_calls > 0 and args.mode in ("both", "fp64"):
         line["turbulence"] = turb_bench(g, p, F, args.turb_calls, not args.no_events)
+    if args.satadj_calls > 0 and args.mode in ("both", "fp64"):
+        line["saturation_adjustment"] = satadj_bench(g, p, F, args.satadj_calls, not args.no_events)
     line["config"] = cfg
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds)

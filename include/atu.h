@@ -293,6 +293,39 @@ typedef enum atu_turb_array {
 int atu_turb_set(atu_ctx* ctx, int which, const double* host);
 int atu_turb_get(atu_ctx* ctx, int which, double* host);
 
+/* ---- saturation adjustment: the reference loop's block of planet/cUranusModel.cpp:997-1033 ----
+ * The four SaturationAdjustmentUran calls, H2O, H2S, NH3, CH4 in that order, each the shared
+ * mixed-phase adjustment SaturationAdjustment<cUranusModel>::run (Tao et al. 1988 with the Newton
+ * step, planet/SaturationAdjustment.h:256-668, dispatched by planet/SaturationAdjustmentUran.cpp:
+ * 199-239), then ChemistryUran::EquilibriumNH4SH("loop") (planet/ChemistryUran.h:139-255). Acts on
+ * t, p_stat (read), rho_mix (read: the model's rho_at with ATURAN_LOCAL_RHO 1), the vapour, cloud and
+ * ice of each gas and nh3, h2s, nh4sh. Every cell is independent. */
+typedef enum atu_gas_id { ATU_GAS_H2O = 0, ATU_GAS_H2S = 1, ATU_GAS_NH3 = 2, ATU_GAS_CH4 = 3 } atu_gas_id;
+
+/* One gas's constants as the loop passes them (planet/cUranusModel.h:880-906, 1045-1192): the
+ * melting and ice-onset temperatures [K], ep, the latent heats [J/kg] and the liquid and ice
+ * vapour-pressure sets of ATPhys::saturation_vapour_pressure (planet/ATPhys.h:49-52). */
+typedef struct atu_gas {
+    double t_0, t_00, ep, lv, ls;
+    double C, L0, R, del_alf, del_bet;
+    double C_i, L0_i, del_alf_i, del_bet_i;
+} atu_gas;
+
+typedef struct atu_satadj_opts {
+    atu_gas gas[4];             /* indexed by atu_gas_id                                          */
+    double m_nh3, m_h2s, m_nh4sh;   /* molecular weights [kg/kmol] (planet/cUranusModel.h:997-999) */
+    int32_t gas_mask;           /* bit n: adjust gas n                                            */
+    int32_t nh4sh_eq;           /* ATURAN_NH4SH_EQ: the NH4SH equilibrium after the gases         */
+    /* knobs of the shared routine (ATURAN_SATADJ_<NAME>), Uranus defaults in brackets:
+       CONSERVE [1], NEWTON [1], EVAP [1], ICE_ENTRY [0], BERGERON [2], FREEZE [1], DRY_EXIT [1]  */
+    int32_t conserve, newton, evap, ice_entry, bergeron, freeze, dry_exit;
+    int32_t pad_;
+} atu_satadj_opts;
+
+/* The reference's defaults: all four gases, the equilibrium on, every knob unset. */
+int atu_satadj_opts_default(atu_satadj_opts* out);
+int atu_saturation_adjust(atu_ctx* ctx, const atu_satadj_opts* opts);
+
 /* Device pointer of a field's storage (for zero-copy interop); `ld_k` receives the k stride in
  * elements, `elem_bytes` 8 or 4. The layout is [im][jm][ld_k]. */
 int atu_field_device_ptr(atu_ctx* ctx, int field, void** dptr, int* ld_k, int* elem_bytes);

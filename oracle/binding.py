@@ -50,6 +50,8 @@ def load() -> C.CDLL:
         "orc_step_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int),
                                         C.POINTER(C.c_double)]),
         "orc_turb": (C.c_int, [V, V, V, C.c_int, _DP, _DP, _DP, _DP, _DP]),
+        "orc_satadj": (C.c_int, [V, V, V, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                 _DP]),
     }
     for n, (res, args) in sig.items():
         f = getattr(lib, n)
@@ -186,3 +188,35 @@ def turb(grid, params, fields: np.ndarray, prod: np.ndarray, tken: np.ndarray, d
     o.abl_height = abl_height
     _ok(load().orc_turb(_p(grid), _p(params), _p(o), int(init), _d(fields), _d(prod), _d(tken), _d(disn),
                         _d(vel_star)), "orc_turb")
+
+
+class Gas(C.Structure):
+    """atu_gas (include/atu.h)."""
+    _fields_ = [(n, C.c_double) for n in ("t_0", "t_00", "ep", "lv", "ls", "C", "L0", "R", "del_alf", "del_bet",
+                                          "C_i", "L0_i", "del_alf_i", "del_bet_i")]
+
+
+# The constants the reference loop passes (planet/cUranusModel.h:880-906, 1045-1192 and the ice sets
+# of planet/SaturationAdjustmentUran.cpp:222-231: H2S takes its liquid set), in atu_gas order.
+GASES = {
+    "H2O": (273.15, 210.15, 8.1253, 2.5009e6, 2.8339e6, 25.096, 3148.2, 461.52, 0.0, -8.7e-3,
+            28.418, 3567.3, 0.0, -8.7e-3),
+    "H2S": (187.65, 140.0, 14.5192, 5.5e5, 7.45e5, 17.064, 747.0, 243.96, 0.0, -2.9e-3,
+            17.064, 747.0, 0.0, -2.9e-3),
+    "NH3": (195.5, 140.0, 7.6752, 1.3720e6, 1.8320e6, 27.863, 2016.0, 488.21, -0.888, 0.0,
+            34.948, 2692.5, -0.888, 0.0),
+    "CH4": (90.69, 67.36, 7.6752, 5.11e5, 5.11e5, 1.627, 553.1, 518.28, 1.002, -4.1e-3,
+            1.627, 553.1, 1.002, -4.1e-3),
+}
+MOLAR = (17.03052, 34.08088, 51.1114)    # m_nh3, m_h2s, m_nh4sh [kg/kmol]
+
+
+def satadj(grid, params, fields: np.ndarray, gas_mask: int = 15, nh4sh_eq: bool = True, conserve: int = 1,
+           newton: int = 1, evap: int = 1, ice_entry: int = 0, bergeron: int = 2, freeze: int = 1,
+           dry_exit: int = 1) -> None:
+    """The reference loop's saturation-adjustment block on the dense fields, in place (defaults: the
+    reference's)."""
+    gases = (Gas * 4)(*[Gas(*GASES[n]) for n in ("H2O", "H2S", "NH3", "CH4")])
+    kn = (C.c_int * 7)(conserve, newton, evap, ice_entry, bergeron, freeze, dry_exit)
+    _ok(load().orc_satadj(_p(grid), _p(params), C.cast(gases, C.c_void_p), kn, gas_mask, int(nh4sh_eq), *MOLAR,
+                          _d(fields)), "orc_satadj")

@@ -14,6 +14,9 @@
 //   mode damp_uvw AtomUtils::damp_wiggles on u, v, w along i, j and k, strength 1, `sweeps` passes
 //   mode turb_run  TurbulenceUran(m).run()   (planet/Turbulence.h:306-332; knobs from the environment)
 //   mode turb_init TurbulenceUran(m).init()  (planet/Turbulence.h:282-303)
+//   mode satadj    the loop's four SaturationAdjustmentUran calls (H2O, H2S, NH3, CH4) and
+//                  ChemistryUran::EquilibriumNH4SH("loop") (planet/cUranusModel.cpp:997-1033); prints
+//                  the gas constants they pass
 // in/out: int32 im, jm, km, nfields, then nfields dense [im][jm][km] FP64 arrays in the atu_field
 // order of include/atu.h; out appends im doubles of buoy_ref_level. The turb_* modes read and write
 // after those the closure's own arrays prod, tken, disn ([im][jm][km]) and vel_star ([jm][km]).
@@ -51,6 +54,7 @@
 #include "PressureSolver.h"
 #include "PressureSolverUran.h"
 #include "TurbulenceUran.h"
+#include "SaturationAdjustmentUran.h"
 #undef private
 #undef protected
 
@@ -177,6 +181,37 @@ int main(int argc, char** argv) {
         AtomUtils::damp_wiggles(m.t, nullptr, true, false, false, 0.5, 1);
     } else if (mode == "damp_uvw") {
         for (Array* f : {&m.u, &m.v, &m.w}) AtomUtils::damp_wiggles(*f, nullptr, true, true, true, 1.0, sweeps);
+    } else if (mode == "satadj") {
+        std::fprintf(stderr, "ref_satadj: H2O %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+                     m.t_0_h2o, m.t_00_h2o, m.ep_h2o, m.lv_h2o, m.ls_h2o, m.C_h2o, m.L0_h2o, m.R_h2o, m.del_alf_h2o,
+                     m.del_bet_h2o, m.C_h2o_ice, m.L0_h2o_ice, m.del_alf_h2o_ice, m.del_bet_h2o_ice);
+        std::fprintf(stderr, "ref_satadj: H2S %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+                     m.t_0_h2s, m.t_00_h2s, m.ep_h2s, m.lv_h2s, m.ls_h2s, m.C_h2s, m.L0_h2s, m.R_h2s, m.del_alf_h2s,
+                     m.del_bet_h2s, m.C_h2s, m.L0_h2s, m.del_alf_h2s, m.del_bet_h2s);
+        std::fprintf(stderr, "ref_satadj: NH3 %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+                     m.t_0_nh3, m.t_00_nh3, m.ep_nh3, m.lv_nh3, m.ls_nh3, m.C_nh3, m.L0_nh3, m.R_nh3, m.del_alf_nh3,
+                     m.del_bet_nh3, m.C_nh3_ice, m.L0_nh3_ice, m.del_alf_nh3_ice, m.del_bet_nh3_ice);
+        std::fprintf(stderr, "ref_satadj: CH4 %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n",
+                     m.t_0_ch4, m.t_00_ch4, m.ep_ch4, m.lv_ch4, m.ls_ch4, m.C_ch4, m.L0_ch4, m.R_ch4, m.del_alf_ch4,
+                     m.del_bet_ch4, m.C_ch4_ice, m.L0_ch4_ice, m.del_alf_ch4_ice, m.del_bet_ch4_ice);
+        std::fprintf(stderr, "ref_satadj: molar %.17g %.17g %.17g\n", m.m_nh3, m.m_h2s, m.m_nh4sh);
+        SaturationAdjustmentUran(m).run("H2O", m.coeff_h2o_A, m.coeff_h2o_B, m.coeff_h2o_A_i, m.coeff_h2o_B_i,
+                                        m.t_0_h2o, m.t_00_h2o, m.ep_h2o, m.lv_h2o, m.ls_h2o, m.cp_h2o, m.r_h2o,
+                                        m.C_h2o, m.L0_h2o, m.R_h2o, m.del_alf_h2o, m.del_bet_h2o, m.m_h2o,
+                                        m.h2o, m.h2o_cloud, m.h2o_ice);
+        SaturationAdjustmentUran(m).run("H2S", m.coeff_h2s_A, m.coeff_h2s_B, m.coeff_h2s_A_i, m.coeff_h2s_B_i,
+                                        m.t_0_h2s, m.t_00_h2s, m.ep_h2s, m.lv_h2s, m.ls_h2s, m.cp_h2s, m.r_h2s,
+                                        m.C_h2s, m.L0_h2s, m.R_h2s, m.del_alf_h2s, m.del_bet_h2s, m.m_h2s,
+                                        m.h2s, m.h2s_cloud, m.h2s_ice);
+        SaturationAdjustmentUran(m).run("NH3", m.coeff_nh3_A, m.coeff_nh3_B, m.coeff_nh3_A_i, m.coeff_nh3_B_i,
+                                        m.t_0_nh3, m.t_00_nh3, m.ep_nh3, m.lv_nh3, m.ls_nh3, m.cp_nh3, m.r_nh3,
+                                        m.C_nh3, m.L0_nh3, m.R_nh3, m.del_alf_nh3, m.del_bet_nh3, m.m_nh3,
+                                        m.nh3, m.nh3_cloud, m.nh3_ice);
+        SaturationAdjustmentUran(m).run("CH4", m.coeff_ch4_A, m.coeff_ch4_B, m.coeff_ch4_A_i, m.coeff_ch4_B_i,
+                                        m.t_0_ch4, m.t_00_ch4, m.ep_ch4, m.lv_ch4, m.ls_ch4, m.cp_ch4, m.r_ch4,
+                                        m.C_ch4, m.L0_ch4, m.R_ch4, m.del_alf_ch4, m.del_bet_ch4, m.m_ch4,
+                                        m.ch4, m.ch4_cloud, m.ch4_ice);
+        if (ChemistryUran::nh4sh_eq()) ChemistryUran(m).EquilibriumNH4SH("loop");
     } else if (mode == "turb_run") {
         TurbulenceUran(m).run();
     } else if (mode == "turb_init") {

@@ -51,6 +51,9 @@ CASES = {
     "turb_run_keps": ("turb_run", 1, 1, False),
     "turb_run_wall": ("turb_run", 1, 1, False),
     "turb_init_wall_keps": ("turb_init", 1, 1, False),
+    # the loop's saturation adjustment (four gases) and NH4SH equilibrium on the satadj input
+    "satadj": ("satadj", 1, 1, False),
+    "satadj_mix": ("satadj", 1, 1, False),
 }
 # The reference reads the closure's knobs from its environment (planet/Turbulence.h:125-176,
 # planet/ATPhys.h env_for): ATURAN_TURB_MODEL, ATURAN_TURB_FREE, ATURAN_ABL_TAPER, ATURAN_TURB_CURV.
@@ -162,9 +165,27 @@ def read_turb_tail(path: Path, nf: int, shape) -> tuple[np.ndarray, np.ndarray]:
     return T3, vs
 
 
+def satadj_mix(grid, params, F: np.ndarray, seed: int = SEED) -> None:
+    """The saturation-adjustment input with every branch reached: temperatures spread +-40 % around
+    the profile (cells above the melting and below the ice-onset temperature of each gas), vapour
+    from 1e-9 to 1e-3 kg/m3 (sub- and supersaturated cells), and a quarter of the cells of each gas
+    with its cloud, a quarter with its ice, a quarter with both and a quarter with neither."""
+    Fi = A.FIELD
+    shp = F.shape[1:]
+    F[Fi["t"]] *= 1.0 + 0.8 * (hash_uniform(70, shp, seed) - 0.5)
+    for n, gas in enumerate(("h2o", "h2s", "nh3", "ch4")):
+        F[Fi[gas]] = 10.0 ** (-9.0 + 6.0 * hash_uniform(71 + n, shp, seed))
+        sel = (hash_uniform(75 + n, shp, seed) * 4.0).astype(np.int64)
+        F[Fi[gas + "_cloud"]] *= (sel == 1) | (sel == 3)
+        F[Fi[gas + "_ice"]] *= (sel == 2) | (sel == 3)
+    F[Fi["nh4sh"]] *= hash_uniform(79, shp, seed) < 0.5
+
+
 def case_input(name: str, grid, params) -> np.ndarray:
     from oracle import binding as O
     F = enriched_state(grid, params)
+    if name == "satadj_mix":
+        satadj_mix(grid, params, F)
     if CASES[name][3]:
         O.bc(grid, params, F)
     return F
@@ -192,7 +213,7 @@ def main() -> None:
                                 capture_output=True, text=True, env=env)
             if rr.returncode != 0:
                 raise SystemExit(f"harness {name} failed:\n{rr.stderr[-2000:]}")
-            consts = [ln for ln in rr.stderr.splitlines() if ln.startswith("ref_harness:")]
+            consts = [ln for ln in rr.stderr.splitlines() if ln.startswith(("ref_harness:", "ref_satadj:"))]
             G, level = read_bin(fout, A.NFIELDS, shape)
             s = summarize(G, level)
             if name in TURB_ENV:

@@ -1,8 +1,9 @@
 """Pins the CPU oracle to golden vectors written by the compiled reference (tests/golden/ref_*.npz,
 from tests/golden/make_golden.py): the domain BC, one RungeKuttaUran (hydro prep + 4 stages), the
 red-black PressureSolver at 1 and 20 sweeps, two full steps, and the default PressureSolverUran after
-one RungeKuttaUran and in two full steps, the Shapiro wiggle filter, and the turbulence closure
-(Turbulence::run and ::init under six sets of its knobs), on an input in which every term of
+one RungeKuttaUran and in two full steps, the Shapiro wiggle filter, the turbulence closure
+(Turbulence::run and ::init under six sets of its knobs) and the loop's saturation-adjustment block
+(four gases and the NH4SH equilibrium, on two inputs), on an input in which every term of
 the path is non-zero. Per field: the sampled cells within 1e-12 of the field's largest sample, and
 the exact sum and sum of squares within 1e-12 relative (the closure's own arrays prod, tken, disn and
 vel_star likewise, and exactly)."""
@@ -26,6 +27,9 @@ def run_oracle(oracle, name, g, p):
     F = MG.case_input(name, g, p)
     mode, steps, sweeps, _ = MG.CASES[name]
     level = np.zeros(g.im)
+    if mode == "satadj":
+        oracle.satadj(g, p, F)
+        return F, level, None
     if mode in ("turb_run", "turb_init"):
         T3, vs = MG.turb_input(g)
         oracle.turb(g, p, F, T3[0], T3[1], T3[2], vs, init=(mode == "turb_init"), **MG.TURB_KNOBS[name])
@@ -97,6 +101,22 @@ def test_fixture_constants_match_library_params():
     p = A.default_params(A.default_grid(1))
     for key in ("r_mix", "R_mix", "cp_mix", "mue_mix", "r_h2"):
         assert getattr(p, key) == vals[key], key
+
+
+def test_satadj_defaults_match_reference_constants():
+    """atu_satadj_opts_default (and the oracle's table) carry the gas constants the compiled reference's
+    loop passes, as its harness printed them."""
+    from oracle import binding as O
+    ref = np.load(GOLDEN / "ref_satadj.npz")
+    lines = [str(x) for x in ref["consts"] if str(x).startswith("ref_satadj:")]
+    printed = {ln.split()[1]: [float(v) for v in ln.split()[2:]] for ln in lines}
+    o = A.core.satadj_opts()
+    names = ("t_0", "t_00", "ep", "lv", "ls", "C", "L0", "R", "del_alf", "del_bet", "C_i", "L0_i", "del_alf_i",
+             "del_bet_i")
+    for gas, n in A._lib.GASES.items():
+        assert [getattr(o.gas[n], f) for f in names] == printed[gas], gas
+        assert list(O.GASES[gas]) == printed[gas], gas
+    assert [o.m_nh3, o.m_h2s, o.m_nh4sh] == printed["molar"] == list(O.MOLAR)
 
 
 def test_enriched_input_is_deterministic():

@@ -1,8 +1,8 @@
 """The HIP path against the compiled reference's golden vectors (tests/golden/ref_*.npz) on the
 enriched input in which every term of the step is non-zero: domain BC, one RungeKuttaUran (hydro
 prep + 4 stages), PressureSolver at 1 and 20 sweeps, two full steps, and the default
-PressureSolverUran after one RungeKuttaUran and in two full steps, the Shapiro filter, and the
-turbulence closure. FP64 bar: 1e-12 of each field's largest sampled magnitude, and 1e-12 relative on
+PressureSolverUran after one RungeKuttaUran and in two full steps, the Shapiro filter, the
+turbulence closure and the loop's saturation-adjustment block. FP64 bar: 1e-12 of each field's largest sampled magnitude, and 1e-12 relative on
 the exact field sums; the closure's own arrays (prod, tken, disn, vel_star) the same."""
 import sys
 from pathlib import Path
@@ -32,7 +32,9 @@ def test_gpu_matches_compiled_reference(oracle, name):
     turb = None
     with A.UranusCore(g, p) as core:
         core.set_state(F)
-        if mode in ("turb_run", "turb_init"):
+        if mode == "satadj":
+            core.SaturationAdjustment()
+        elif mode in ("turb_run", "turb_init"):
             T3, vs = MG.turb_input(g)
             for n, a in zip(("prod", "tken", "disn"), T3):
                 core.set_turb_array(n, a)
