@@ -94,7 +94,7 @@ class SolveResult:
 
 
 def solver_opts(mode: str = "reference", sweeps: int = 1, omega: float = 1.0, rel_tol: float = 1e-8,
-                max_sweeps: int = 200, warm_start: bool = False) -> L.SolverOpts:
+                max_sweeps: int = 200, warm_start: int = 0) -> L.SolverOpts:
     o = L.SolverOpts()
     o.mode = {"reference": L.SOLVE_REFERENCE, "converge": L.SOLVE_CONVERGE, "uran": L.SOLVE_URAN}[mode]
     o.sweeps = sweeps
@@ -184,9 +184,10 @@ class UranusCore:
         return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)
 
     def step(self, nsteps: int = 1, mode: str = "reference", sweeps: int = 1, omega: float = 1.0,
-             rel_tol: float = 1e-8, max_sweeps: int = 200, warm_start: bool = False) -> SolveResult:
+             rel_tol: float = 1e-8, max_sweeps: int = 200, warm_start: int = 0) -> SolveResult:
         """The benchmark's outer step: hydro prep, RK4, BC, Poisson solve; nsteps times (converge with
-        warm_start: each solve after the first starts from 2 p_n - p_{n-1})."""
+        warm_start 1: each solve after the first starts from 2 p_n - p_{n-1}; 2: after the second from
+        3 p_n - 3 p_{n-1} + p_{n-2})."""
         o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps, warm_start)
         st = L.SolveStats()
         _check(self._lib.atu_step(self._ctx, nsteps, C.byref(o), C.byref(st)), self._ctx, "atu_step")

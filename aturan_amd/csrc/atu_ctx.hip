@@ -67,6 +67,8 @@ struct atu_ctx {
     double* SROWS = nullptr;  // (im-2)*(jm-2)
     double* SROWS2 = nullptr; // the same, for a second sum of one pass
     double* PPREV = nullptr;  // p_dyn as the last warm-started solve found it (interior)
+    double* PPREV2 = nullptr; // and as the one before found it (quadratic warm start)
+    int warm_hist = 0;        // previous solutions held: 0, 1 or 2 (valid while warm_epoch holds)
     double* DAMP = nullptr;   // scratch field of the wiggle filter
     // the turbulence closure: prod, tken, disn, the friction velocity and the tables of the last
     // knobs (allocated by the first closure call or atu_turb_set)
@@ -169,7 +171,7 @@ struct atu_ctx {
 };
 
 atu_ctx::~atu_ctx() {
-    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SROWS2, PPREV, DAMP, SCAL,
+    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SROWS2, PPREV, PPREV2, DAMP, SCAL,
                       BR, BAC, BAS, TURB3, VSTAR, TURBTAB})
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
@@ -541,14 +543,20 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     auto H = [&](hipError_t e, const char* w) { return c->hip(e, w); };
     int e;
     // y.src, the mean of p to hold, the control flags cleared, b and |b|^2; with a warm start the
-    // same pass keeps p_n and starts the iterations from 2 p_n - p_{n-1}
+    // same pass keeps p_n (and p_{n-1}) and starts the iterations from 2 p_n - p_{n-1}, or with
+    // warm_start 2 and three solutions held from 3 p_n - 3 p_{n-1} + p_{n-2}
     const bool record = o->warm_start != 0 && P == c->PDYN;
-    const bool extrap = record && c->warm_epoch == c->state_epoch;
+    const bool quad = record && o->warm_start >= 2;
+    const int hist = (record && c->warm_epoch == c->state_epoch) ? c->warm_hist : 0;
+    const int extrap = hist == 0 ? 0 : ((quad && hist >= 2) ? 2 : 1);
     if (record && !c->PPREV && H(hipMalloc((void**)&c->PPREV, sh.fs * sizeof(double)), "hipMalloc(p_prev)"))
+        return ATU_ENOMEM;
+    if (quad && !c->PPREV2 && H(hipMalloc((void**)&c->PPREV2, sh.fs * sizeof(double)), "hipMalloc(p_prev)"))
         return ATU_ENOMEM;
     if ((e = H(c->timed(PC_RES, [&] { return launch_solve_setup(sh, P, SRCP, c->SROWS, c->SROWS2, c->SCAL,
                                                                  c->dsol.yw, c->RES, c->CTL, record ? c->PPREV : nullptr,
-                                                                 extrap, s); }), "solve setup")))
+                                                                 quad ? c->PPREV2 : nullptr, extrap, s); }),
+               "solve setup")))
         return e;
     c->warm_epoch = ~0ul;
     const int max_it = o->max_sweeps > 0 ? o->max_sweeps : 1;
@@ -684,7 +692,11 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     }
     if (rel > o->rel_tol && o->rel_tol > 0.0)
         return c->fail(ATU_ENOCONV, "converging solve: tolerance not reached within max_sweeps");
-    if (record) c->warm_epoch = c->state_epoch;
+    if (record) {
+        c->warm_epoch = c->state_epoch;
+        // the solutions held for the next solve: this one, and the previous one when PPREV2 took it
+        c->warm_hist = quad ? std::min(hist + 1, 2) : 1;
+    }
     return ATU_OK;
 }
 
@@ -1256,7 +1268,6 @@ int atu_saturation_adjust(atu_ctx* ctx, const atu_satadj_opts* opts) {
     for (int n = 0; n < 4; n++)
         if ((o.gas_mask & (1 << n)) && !(o.gas[n].t_0 != o.gas[n].t_00))
             return ctx->fail(ATU_EINVAL, "atu_saturation_adjust: t_0 == t_00");
-    ctx->state_epoch++;
     SatAdjArgs a{};
     a.y = ctx->Y;
     if (ctx->mixed()) {

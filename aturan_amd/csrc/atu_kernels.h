@@ -156,10 +156,11 @@ hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, d
 // The start of a converging solve in four launches: scal[0] = y . src and scal[1] = the interior sum
 // of p (same orders as launch_dsum), the control flags cleared, b = src - scal[0] / yw and
 // scal[2] = |b|^2. `rows`, `rows2`: (im-2)*(jm-2) doubles each. With `pprev` the first pass also
-// stores p's interior there and, with `extrap`, replaces it by 2 p - pprev (the warm start).
+// stores p's interior there (moving the old pprev to `pprev2`, when given) and replaces p by
+// 2 p - pprev (extrap 1) or 3 p - 3 pprev + pprev2 (extrap 2): the warm start.
 hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, double* rows, double* rows2,
-                              double* scal, double yw, double* b, int* flags, double* pprev, bool extrap,
-                              hipStream_t s);
+                              double* scal, double yw, double* b, int* flags, double* pprev, double* pprev2,
+                              int extrap, hipStream_t s);
 hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done = nullptr);
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,

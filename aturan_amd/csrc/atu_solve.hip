@@ -477,9 +477,10 @@ __device__ __forceinline__ void row_sum_store(double term, double* chunk, double
 }
 
 // The two sums the solve starts with, in one pass: y . src and the interior sum of p. With pprev,
-// the pass also keeps p in pprev and (extrap) starts the iterations from 2 p - pprev.
+// the pass also keeps p in pprev (and the previous pprev in pprev2, when given) and starts the
+// iterations from 2 p - pprev (extrap 1) or 3 p - 3 pprev + pprev2 (extrap 2).
 __global__ void __launch_bounds__(1024) k_dsum_rows_ysrc_p(Shape sh, double* p, const double* src, double* rows_y,
-                                                           double* rows_p, double* pprev, int extrap) {
+                                                           double* rows_p, double* pprev, double* pprev2, int extrap) {
     const int N = sh.km - 1;
     const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
     const int k = threadIdx.x;
@@ -489,7 +490,10 @@ __global__ void __launch_bounds__(1024) k_dsum_rows_ysrc_p(Shape sh, double* p, 
         ty = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * src[x];
         tp = p[x];
         if (pprev) {
-            if (extrap) p[x] = 2.0 * tp - pprev[x];
+            const double p1 = pprev[x];
+            if (extrap == 2) p[x] = 3.0 * tp - 3.0 * p1 + pprev2[x];
+            else if (extrap) p[x] = 2.0 * tp - p1;
+            if (pprev2) pprev2[x] = p1;
             pprev[x] = tp;
         }
     }
@@ -696,14 +700,14 @@ hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, c
 }
 
 hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, double* rows, double* rows2,
-                              double* scal, double yw, double* b, int* flags, double* pprev, bool extrap,
-                              hipStream_t s) {
+                              double* scal, double yw, double* b, int* flags, double* pprev, double* pprev2,
+                              int extrap, hipStream_t s) {
     const int N = sh.km - 1;
     const int threads = ((N + 63) / 64) * 64;
     if (threads > 1024) return hipErrorInvalidValue;
     const dim3 grid(sh.jm - 2, sh.im - 2);
     const int nrows = (sh.im - 2) * (sh.jm - 2);
-    hipLaunchKernelGGL(k_dsum_rows_ysrc_p, grid, dim3(threads), 0, s, sh, p, src, rows, rows2, pprev, extrap ? 1 : 0);
+    hipLaunchKernelGGL(k_dsum_rows_ysrc_p, grid, dim3(threads), 0, s, sh, p, src, rows, rows2, pprev, pprev2, extrap);
     hipLaunchKernelGGL(k_dsum_final2, dim3(1), dim3(kDsumParts), 0, s, nrows, (const double*)rows,
                        (const double*)rows2, scal + 0, scal + 1, flags);
     hipLaunchKernelGGL(k_make_b_sq_rows, grid, dim3(threads), 0, s, sh, src, (const double*)scal, yw, b, rows);

@@ -35,6 +35,8 @@ RK_BYTES_MIXED = (
     _MX_READ + 19 * 4 + 19 * 8 + 19 * 8 + 19 * 4,
     _MX_READ + 19 * 8 + 19 * 8 + 19 * 8 + 19 * 4 + 3 * 4,        # stage 3: y_n, acc; writes y, its mirror, aux
 )
+SOLVE_START = {0: "p_n", 1: "2 p_n - p_(n-1) (linear extrapolation of the last two solutions)",
+               2: "3 p_n - 3 p_(n-1) + p_(n-2) (quadratic extrapolation of the last three solutions)"}
 ERR_FIELDS = {"u_zonal(w)": "w", "v": "v", "w_radial(u)": "u", "T": "t", "CH4": "ch4", "p": "p_dyn"}
 
 
@@ -49,8 +51,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-errors", action="store_true", help="skip the 1/10/100-step mixed-vs-FP64 comparison")
-    ap.add_argument("--cold-start", action="store_true",
-                    help="start every solve from p_n (default: from the extrapolation 2 p_n - p_{n-1})")
+    ap.add_argument("--cold-start", action="store_true", help="start every solve from p_n (= --warm 0)")
+    ap.add_argument("--warm", type=int, choices=[0, 1, 2], default=2,
+                    help="solve start: 0 p_n, 1 2 p_n - p_(n-1), 2 3 p_n - 3 p_(n-1) + p_(n-2) (default)")
     ap.add_argument("--cpu-one-thread", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP events in the timed region and no per-class pass (for runs under a profiler)")
@@ -308,8 +311,9 @@ def refined_bench(args):
     import aturan_amd as A
     g, p, F = initial_state(2)
     n_int = (g.im - 2) * (g.jm - 2) * (g.km - 1)
-    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, warm_start=not args.cold_start)
-    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner, warm_start=not args.cold_start)
+    warm = 0 if args.cold_start else args.warm
+    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, warm_start=warm)
+    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner, warm_start=warm)
     out, fields = {}, {}
 
     def progress(msg):
@@ -377,7 +381,7 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(tol, budget_s):
+def cpu_baseline(tol, budget_s, warm=2):
     """The oracle on the same workload (warm-started solves, as the GPU runs it): one warm-up step,
     then one call of n steps sized to about budget_s; and, in a child process pinned to one OpenMP
     thread, one step."""
@@ -388,11 +392,11 @@ def cpu_baseline(tol, budget_s):
     F = A.synthetic_state(g, p, 20261203)
     O.bc(g, p, F)
     t0 = time.perf_counter()
-    O.step_converge(g, p, F, 1, 200, tol, True)          # warm-up
+    O.step_converge(g, p, F, 1, 200, tol, warm)          # warm-up
     per = time.perf_counter() - t0
     n = max(2, min(50, int(budget_s / max(per, 1e-3))))
     t0 = time.perf_counter()
-    O.step_converge(g, p, F, n, 200, tol, True)
+    O.step_converge(g, p, F, n, 200, tol, warm)
     el = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     one = None
@@ -446,15 +450,16 @@ def main():
         refined_bench(args)
         return
     g, p, F = initial_state()
-    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, warm_start=not args.cold_start)
-    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner, warm_start=not args.cold_start)
+    warm = 0 if args.cold_start else args.warm
+    opts64 = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, warm_start=warm)
+    optsmx = dict(mode="converge", rel_tol=args.tol, max_sweeps=60, sweeps=args.inner, warm_start=warm)
     line = {"metric": "RK4 steps/s at 41x181x361 (plain FP64 GPU step, Poisson to 1e-8)", "value": None,
             "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic"}
     cfg = {"workload": f"41x181x361 shell, seeded synthetic state: hydro prep + RK4 + BC + red-black line/barotropic "
                        f"Poisson solve to relative residual {args.tol:g}, {args.steps} steps",
-           "solve_start": "p_n" if args.cold_start else "2 p_n - p_(n-1) (linear extrapolation of the last two solutions)"}
+           "solve_start": SOLVE_START[warm]}
     if args.mode in ("both", "fp64"):
         with A.UranusCore(g, p, "fp64") as core:
             core.set_state(F)
@@ -501,7 +506,7 @@ def main():
         line["saturation_adjustment"] = satadj_bench(g, p, F, args.satadj_calls, not args.no_events)
     line["config"] = cfg
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds, warm)
     print(json.dumps(line), flush=True)
 
 

@@ -142,9 +142,10 @@ typedef struct atu_solver_opts {
     int32_t sweeps;          /* REFERENCE: sweeps to run. CONVERGE on a mixed context: FP32
                                 smoothing iterations per FP64 refinement (<= 0: 1)            */
     int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing iterations                 */
-    int32_t warm_start;      /* CONVERGE in atu_step: start each solve from the extrapolation
-                                2 p_n - p_{n-1} of the last two solutions (the held mean stays
-                                p_n's). The context forgets p_{n-1} on atu_set_field,
+    int32_t warm_start;      /* CONVERGE in atu_step: 1 starts each solve from the extrapolation
+                                2 p_n - p_{n-1} of the last two solutions, 2 from
+                                3 p_n - 3 p_{n-1} + p_{n-2} once three are held (the held mean
+                                stays p_n's). The context forgets p_{n-1} on atu_set_field,
                                 atu_hydro_prep, atu_rk4, atu_apply_bc, atu_pressure_solve and the
                                 window calls; a caller that writes p_dyn through
                                 atu_field_device_ptr passes 0 on its next atu_step. 0: from p_n  */

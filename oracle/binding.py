@@ -147,8 +147,9 @@ def step(grid, params, fields: np.ndarray, nsteps: int, sweeps: int, omega: floa
 
 def step_converge(grid, params, fields: np.ndarray, nsteps: int, max_iters: int, rel_tol: float,
                   warm: bool) -> tuple[int, float]:
-    """nsteps outer steps with the converging solve (warm: start each solve after the first from
-    2 p_n - p_{n-1}); returns the last solve's iteration count and relative residual."""
+    """nsteps outer steps with the converging solve (warm 1: start each solve after the first from
+    2 p_n - p_{n-1}; 2: after the second from 3 p_n - 3 p_{n-1} + p_{n-2}); returns the last solve's
+    iteration count and relative residual."""
     it, rel = C.c_int(), C.c_double()
     _ok(load().orc_step_converge(_p(grid), _p(params), _d(fields), nsteps, max_iters, rel_tol, int(warm),
                                  C.byref(it), C.byref(rel)), "orc_step_converge")

@@ -981,10 +981,12 @@ static void baro_correct(const orc_baro* B, const orc_geo* G, const orc_pgeo* q,
 }
 
 /* `pprev` (optional, interior cells): the previous solution. When given, it receives p as the
- * solve finds it; with `extrap` the iterations start from 2 p - pprev instead of p (the held mean
- * is p's before the extrapolation). */
+ * solve finds it (and `pprev2`, when given, the previous pprev); with `extrap` 1 the iterations
+ * start from 2 p - pprev instead of p, with 2 from 3 p - 3 pprev + pprev2 (the held mean is p's
+ * before the extrapolation). */
 static int converge_impl(const atu_grid* g, const atu_params* p, double* fields, int max_iters, double rel_tol,
-                         int* iters_out, double* rel_out, double* src_out, double* pprev, int extrap) {
+                         int* iters_out, double* rel_out, double* src_out, double* pprev, double* pprev2,
+                         int extrap) {
     orc_geo G;
     geo_init(&G, g, p);
     orc_pgeo q;
@@ -1016,7 +1018,10 @@ static int converge_impl(const atu_grid* g, const atu_params* p, double* fields,
             for (int j = 1; j < jm - 1; j++)
                 for (int k = 0; k < n; k++) {
                     const double a = pd[IDX(i, j, k)];
-                    if (extrap) pd[IDX(i, j, k)] = 2.0 * a - pprev[IDX(i, j, k)];
+                    const double a1 = pprev[IDX(i, j, k)];
+                    if (extrap == 2) pd[IDX(i, j, k)] = 3.0 * a - 3.0 * a1 + pprev2[IDX(i, j, k)];
+                    else if (extrap) pd[IDX(i, j, k)] = 2.0 * a - a1;
+                    if (pprev2) pprev2[IDX(i, j, k)] = a1;
                     pprev[IDX(i, j, k)] = a;
                 }
     memset(b, 0, sizeof(double) * n3);
@@ -1062,7 +1067,7 @@ static int converge_impl(const atu_grid* g, const atu_params* p, double* fields,
 
 int orc_pressure_converge(const atu_grid* g, const atu_params* p, double* fields, int max_iters, double rel_tol,
                           int* iters_out, double* rel_out, double* src_out) {
-    return converge_impl(g, p, fields, max_iters, rel_tol, iters_out, rel_out, src_out, NULL, 0);
+    return converge_impl(g, p, fields, max_iters, rel_tol, iters_out, rel_out, src_out, NULL, NULL, 0);
 }
 
 /* ------------------------------------------------------------------------------------------- */
@@ -1270,18 +1275,22 @@ int orc_damp_wiggles(const atu_grid* g, double* fields, int field, int along_i, 
     return 0;
 }
 
-/* The same outer step with the converging solve; with `warm`, every solve after the first starts
- * from the linear extrapolation 2 p_n - p_{n-1} of the last two solutions. iters_out / rel_out:
- * the last solve's. */
+/* The same outer step with the converging solve; with `warm` 1, every solve after the first starts
+ * from the linear extrapolation 2 p_n - p_{n-1} of the last two solutions; with 2, every solve after
+ * the second from the quadratic one 3 p_n - 3 p_{n-1} + p_{n-2} of the last three (the second from
+ * the linear one). iters_out / rel_out: the last solve's. */
 int orc_step_converge(const atu_grid* g, const atu_params* p, double* fields, int nsteps, int max_iters,
                       double rel_tol, int warm, int* iters_out, double* rel_out) {
     const size_t n3 = (size_t)g->im * g->jm * g->km;
     double* pprev = warm ? calloc(n3, sizeof(double)) : NULL;
+    double* pprev2 = warm == 2 ? calloc(n3, sizeof(double)) : NULL;
     int e = 0;
     for (int s = 0; s < nsteps && !e; s++) {
         if ((e = orc_hydro(g, p, fields, NULL)) || (e = orc_rk4(g, p, fields)) || (e = orc_bc(g, p, fields))) break;
-        e = converge_impl(g, p, fields, max_iters, rel_tol, iters_out, rel_out, NULL, pprev, warm && s > 0);
+        const int mode = !warm ? 0 : (s == 0 ? 0 : ((warm == 2 && s >= 2) ? 2 : 1));
+        e = converge_impl(g, p, fields, max_iters, rel_tol, iters_out, rel_out, NULL, pprev, pprev2, mode);
     }
     free(pprev);
+    free(pprev2);
     return e;
 }

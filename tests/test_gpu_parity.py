@@ -191,10 +191,11 @@ def test_full_step_matches_oracle(setup, oracle):
     assert report("2 steps", errs) <= TOL
 
 
-@pytest.mark.parametrize("warm", [False, True])
+@pytest.mark.parametrize("warm", [0, 1, 2])
 def test_converge_steps_match_oracle(setup, oracle, warm):
     """The benchmark's outer step (hydro, RK4, BC, converging solve to 1e-8), three steps in one call
-    and one more in a second call; with warm, every solve after the first starts from 2 p_n - p_{n-1}."""
+    and one more in a second call; with warm 1, every solve after the first starts from 2 p_n - p_{n-1},
+    with warm 2 every solve after the second from 3 p_n - 3 p_{n-1} + p_{n-2}."""
     print(f"[case] four converging steps, warm start {warm}", flush=True)
     g, p, F0 = setup
     F = F0.copy()
