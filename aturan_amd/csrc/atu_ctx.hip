@@ -459,11 +459,20 @@ int rk_last_stage_split(atu_ctx* c) {
 }
 
 // The four stages; with `overlap`, the last one as rk_last_stage_split (the caller then runs
-// do_bc_rest and side_scan_after_bc instead of do_bc).
-int do_rk4(atu_ctx* c, bool overlap = false) {
+// do_bc_rest and side_scan_after_bc instead of do_bc). With `hydro_pending` the hydrostatic prep is
+// still running on the side stream: the first stage updates every field but v and w, then waits
+// for it (ev_hydro) and updates v and w, the only fields whose tendencies read p_hydro.
+int do_rk4(atu_ctx* c, bool overlap = false, bool hydro_pending = false) {
     c->rhs_valid = c->keep_rhs && !c->mixed();
     for (int s = 0; s < 4; s++) {
         if (s == 3 && overlap) return rk_last_stage_split(c);
+        if (s == 0 && hydro_pending) {
+            if (c->hip(c->timed(PC_RK, [&] { return launch_stage_part(c, 0, 3); }), "rk_stage") ||
+                c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent") ||
+                c->hip(c->timed(PC_RK, [&] { return launch_stage_part(c, 0, 4); }), "rk_stage"))
+                return ATU_EHIP;
+            continue;
+        }
         if (c->hip(c->timed(PC_RK, [&] { return launch_stage_part(c, s, 0); }), "rk_stage")) return ATU_EHIP;
     }
     return ATU_OK;
@@ -1420,13 +1429,11 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
             pe.b = ctx->take_event();
             hipEventRecord(pe.a, ctx->stream);
         }
-        if (n == 0) {
-            if ((e = do_hydro(ctx, ctx->stream))) return e;
-        } else if ((e = ctx->hip(hipStreamWaitEvent(ctx->stream, ctx->ev_hydro, 0), "hipStreamWaitEvent"))) {
-            return e;
-        }
+        // after the first step the prep is still running on the side stream: do_rk4 waits for it
+        // after the part of the first stage that does not read p_hydro
+        if (n == 0 && (e = do_hydro(ctx, ctx->stream))) return e;
         const bool more = n + 1 < nsteps;
-        if ((e = do_rk4(ctx, more)) || (e = more ? do_bc_rest(ctx) : do_bc(ctx)) ||
+        if ((e = do_rk4(ctx, more, n > 0)) || (e = more ? do_bc_rest(ctx) : do_bc(ctx)) ||
             (more && (e = side_scan_after_bc(ctx))))
             return e;
         if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;

@@ -96,7 +96,9 @@ __device__ __forceinline__ double cp_lapse_dev(const StageArgs<S>& a, double T) 
 
 // The tendencies of one cell in arithmetic precision R from stencil storage S, folded in FP64.
 // PART 0: every field; PART 1: t only; PART 2: every field but t (t's tendency reads no other
-// tendency, so the last stage can update t first and the rest after).
+// tendency, so the last stage can update t first and the rest after); PART 3: every field but v
+// and w, PART 4: v and w (only their tendencies read p_hydro, so the first stage can start before
+// the hydrostatic prep has finished). Each field's fold reads and writes that field alone.
 template <typename R, typename S, int STAGE, int PART>
 __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int k) {
     const int nk = a.sh.km - 1;
@@ -164,7 +166,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     const R re = (R)C.re;
 
     // ---- temperature (:280-296, :298, :333-334, :540-557) ----
-    if constexpr (PART != 2) {
+    if constexpr (PART == 0 || PART == 1 || PART == 3) {
         const D7<R> T = ST(ATU_T);
         R pressure_t = (R)C.coeff_energy_p * scale_p * (u0 * dpdr + v_invrm * dpdthe + w_invrs * dpdphi);
         if ((C.cp_mode != 0 || C.cp_gas != 0) && T.c > R(0.0))
@@ -181,7 +183,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     if constexpr (PART == 1) return;
 
     // ---- radial velocity u (:303-304, :336-338, :723-732) ----
-    {
+    if constexpr (PART != 4) {
         const R transport_u = u0 * U.d1r + v_invrm * U.d1t + w_invrs * U.d1p;
         const R diffusion_u = U.d2r + R(2.0) * u0 * inv_rm2 + U.d2t * inv_rm2 + R(4.0) * U.d1r * inv_rm
                               + U.d1t * cirs + U.d2p * inv_rm2sinthe2;
@@ -194,40 +196,44 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     }
     const R cor_sc = (R)C.Coriolis * (R)C.scale_Cor;
     const double* ph = a.phydro;
-    // ---- meridional velocity v (:305-306, :339-343, :719, :759-765) ----
-    {
-        const R transport_v = u0 * V.d1r + v_invrm * V.d1t + w_invrs * V.d1p
-                              + (u0 * v0 - w0 * w0 * cotanthe) * inv_rm;
-        const R diffusion_v = V.d2r + V.d1r * two_inv_rm + V.d2t * inv_rm2 + V.d1t * cirs - v_metric * v0
-                              + V.d2p * inv_rm2sinthe2 + R(2.0) * U.d1t * inv_rm2
-                              - W.d1p * R(2.0) * costhe * inv_rm2sinthe2;
-        const R dphdthe_term = fp64_diff<R>(ph[c + ldk], ph[c - ldk], G.inv_2dthe) * inv_rm;
-        const R Coriolis_the = (R)C.m_two_omega * costhe * w0;
-        const R rhs_v = -dpdthe_term - dphdthe_term - transport_v + diffusion_v / re + diffusion_v * nue_t
-                        - cor_sc * Coriolis_the;
-        fold<STAGE, CL_NONE>(a, ATU_V, c, Y0(ATU_V, v0), (double)rhs_v);
-        if (STAGE == 3) {
-            a.aux[fs + c] = (S)(rhs_v + dpdthe_term);
-            if (a.rhs3) a.rhs3[fs + c] = (double)rhs_v;
+    if constexpr (PART != 3) {
+        // ---- meridional velocity v (:305-306, :339-343, :719, :759-765) ----
+        {
+            const R transport_v = u0 * V.d1r + v_invrm * V.d1t + w_invrs * V.d1p
+                                  + (u0 * v0 - w0 * w0 * cotanthe) * inv_rm;
+            const R diffusion_v = V.d2r + V.d1r * two_inv_rm + V.d2t * inv_rm2 + V.d1t * cirs - v_metric * v0
+                                  + V.d2p * inv_rm2sinthe2 + R(2.0) * U.d1t * inv_rm2
+                                  - W.d1p * R(2.0) * costhe * inv_rm2sinthe2;
+            const R dphdthe_term = fp64_diff<R>(ph[c + ldk], ph[c - ldk], G.inv_2dthe) * inv_rm;
+            const R Coriolis_the = (R)C.m_two_omega * costhe * w0;
+            const R rhs_v = -dpdthe_term - dphdthe_term - transport_v + diffusion_v / re + diffusion_v * nue_t
+                            - cor_sc * Coriolis_the;
+            fold<STAGE, CL_NONE>(a, ATU_V, c, Y0(ATU_V, v0), (double)rhs_v);
+            if (STAGE == 3) {
+                a.aux[fs + c] = (S)(rhs_v + dpdthe_term);
+                if (a.rhs3) a.rhs3[fs + c] = (double)rhs_v;
+            }
         }
-    }
-    // ---- zonal velocity w (:307-308, :344-348, :720, :780-788) ----
-    {
-        const R transport_w = u0 * W.d1r + v_invrm * W.d1t + w_invrs * W.d1p
-                              + (w0 * u0 + v0 * w0 * cotanthe) * inv_rm;
-        const R diffusion_w = W.d2r + W.d1r * two_inv_rm + W.d2t * inv_rm2 + W.d1t * cirs - v_metric * w0
-                              + W.d2p * inv_rm2sinthe2 + R(2.0) * U.d1p * inv_rm2sinthe
-                              + V.d1p * R(2.0) * costhe * inv_rm2sinthe2;
-        const R dphdphi_term = fp64_diff<R>(ph[ce], ph[cw], G.inv_2dphi) * inv_rmsinthe;
-        const R Coriolis_phi = (R)C.two_omega * (R(1.0) * costhe * v0 + sinthe_bf * u0);
-        const R rhs_w = -dpdphi_term - dphdphi_term - transport_w + diffusion_w / re + diffusion_w * nue_t
-                        - cor_sc * Coriolis_phi;
-        fold<STAGE, CL_NONE>(a, ATU_W, c, Y0(ATU_W, w0), (double)rhs_w);
-        if (STAGE == 3) {
-            a.aux[2 * fs + c] = (S)(rhs_w + dpdphi_term);
-            if (a.rhs3) a.rhs3[2 * fs + c] = (double)rhs_w;
+        // ---- zonal velocity w (:307-308, :344-348, :720, :780-788) ----
+        {
+            const R transport_w = u0 * W.d1r + v_invrm * W.d1t + w_invrs * W.d1p
+                                  + (w0 * u0 + v0 * w0 * cotanthe) * inv_rm;
+            const R diffusion_w = W.d2r + W.d1r * two_inv_rm + W.d2t * inv_rm2 + W.d1t * cirs - v_metric * w0
+                                  + W.d2p * inv_rm2sinthe2 + R(2.0) * U.d1p * inv_rm2sinthe
+                                  + V.d1p * R(2.0) * costhe * inv_rm2sinthe2;
+            const R dphdphi_term = fp64_diff<R>(ph[ce], ph[cw], G.inv_2dphi) * inv_rmsinthe;
+            const R Coriolis_phi = (R)C.two_omega * (R(1.0) * costhe * v0 + sinthe_bf * u0);
+            const R rhs_w = -dpdphi_term - dphdphi_term - transport_w + diffusion_w / re + diffusion_w * nue_t
+                            - cor_sc * Coriolis_phi;
+            fold<STAGE, CL_NONE>(a, ATU_W, c, Y0(ATU_W, w0), (double)rhs_w);
+            if (STAGE == 3) {
+                a.aux[2 * fs + c] = (S)(rhs_w + dpdphi_term);
+                if (a.rhs3) a.rhs3[2 * fs + c] = (double)rhs_w;
+            }
         }
+
     }
+    if constexpr (PART == 4) return;
 
     // ---- species (:310-326, :350-379, :790-913) ----
     const R pc = (R)C.precip_coupling, cr = (R)C.chemical_reaction;
@@ -333,7 +339,14 @@ hipError_t launch_rows(int stage, int part, int rows, const StageArgs<S>& a, hip
     const dim3 block(64, TJ, TI);
     const dim3 grid((nk + 63) / 64, (rows + TJ - 1) / TJ, (a.sh.im - 2 + TI - 1) / TI);
     switch (stage) {
-        case 0: hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI, BAND, 0>), grid, block, 0, s, a); break;
+        case 0:
+            if (part == 3)
+                hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI, BAND, 3>), grid, block, 0, s, a);
+            else if (part == 4)
+                hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI, BAND, 4>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_rk_stage<S, 0, TJ, TI, BAND, 0>), grid, block, 0, s, a);
+            break;
         case 1: hipLaunchKernelGGL((k_rk_stage<S, 1, TJ, TI, BAND, 0>), grid, block, 0, s, a); break;
         case 2: hipLaunchKernelGGL((k_rk_stage<S, 2, TJ, TI, BAND, 0>), grid, block, 0, s, a); break;
         default:
