@@ -527,13 +527,28 @@ __global__ void __launch_bounds__(1024) k_make_b_sq_rows(Shape sh, const double*
 // convergence on the device: the counters take the work just done, then rel = |r| / |b| (|b|^2 in
 // *bb) <= tol sets done.
 constexpr int kDsumParts = 256;
+// Partial t of the 256: rows t, t + 256, t + 512, ... added in sequence; the loads go out in batches
+// of eight ahead of their adds (which keep the order).
+__device__ __forceinline__ double partial_256(int nrows, const double* rows, int t) {
+    double s = 0.0;
+    int q = t;
+    for (; q + 7 * kDsumParts < nrows; q += 8 * kDsumParts) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = rows[q + u * kDsumParts];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += x[u];
+    }
+    for (; q < nrows; q += kDsumParts) s += rows[q];
+    return s;
+}
+
 __global__ void __launch_bounds__(kDsumParts) k_dsum_final(int nrows, const double* rows, double* out, SolveCtl ctl) {
     if (ctl.flags && ctl.flags[CTL_DONE]) return;
     if (ctl.skip && *ctl.skip) return;
     __shared__ double wsum[kDsumParts / 64];
     const int t = threadIdx.x;
-    double s = 0.0;
-    for (int q = t; q < nrows; q += kDsumParts) s += rows[q];
+    double s = partial_256(nrows, rows, t);
     for (int h = 32; h > 0; h >>= 1) s += __shfl_down(s, h, 64);
     if ((t & 63) == 0) wsum[t >> 6] = s;
     __syncthreads();
@@ -562,8 +577,7 @@ __global__ void __launch_bounds__(kDsumParts) k_dsum_final(int nrows, const doub
 // control flags cleared for the iterations that follow.
 __device__ __forceinline__ double total_256(int nrows, const double* rows, double* wsum) {
     const int t = threadIdx.x;
-    double s = 0.0;
-    for (int q = t; q < nrows; q += kDsumParts) s += rows[q];
+    double s = partial_256(nrows, rows, t);
     for (int h = 32; h > 0; h >>= 1) s += __shfl_down(s, h, 64);
     if ((t & 63) == 0) wsum[t >> 6] = s;
     __syncthreads();
