@@ -13,8 +13,8 @@ namespace {
 // anomaly divides by this mean, which magnifies any rounding difference of the sum ~200-fold in
 // p_hydro. So the sum is formed in exactly that order: one workgroup per level. Its lanes load and
 // form 64 consecutive terms at a time (a skipped term contributes an exact +0.0); a group of
-// kLevelGroup such chunks is parked in LDS and added in sequence while the loads of the next group
-// are in flight, the LDS reads of each batch of the chain issued while the previous batch adds.
+// kLevelGroup such chunks is parked in LDS and added in sequence while the loads of the group three
+// ahead are in flight, the LDS reads of each batch of the chain issued while the previous batch adds.
 // Two waves per level: wave 0 runs the chain of the weighted terms, wave 1 that of the weights, each
 // on its own SIMD. The chain is ~65k dependent adds per level; the step runs this kernel on a side
 // stream, concurrently with the previous step's Poisson solve, which it does not depend on.
@@ -24,50 +24,52 @@ __global__ void __launch_bounds__(128) k_level_serial(Shape sh, DevGeo g, StepCo
     const int i = blockIdx.x;
     const int n = sh.jm * sh.km;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;   // wave 0: the sum, wave 1: the weights
-    __shared__ double tv[2][kLevelGroup * 64];
+    constexpr int kG = kLevelGroup * 64;
+    __shared__ double tv[2][kG];
     __shared__ double wsum;
     double* mine = tv[wv];
-    double rt[kLevelGroup], rp[kLevelGroup];   // T and p_stat of the group in flight
-    int rj[kLevelGroup];
-    auto load_group = [&](int q0) {
+    // T, p_stat and sin(theta) of three groups in flight: the loads of a group go out while the
+    // chain of the group three before it runs (the chain shares the HBM with the stage kernels)
+    double tA[kLevelGroup], pA[kLevelGroup], wA[kLevelGroup];
+    double tB[kLevelGroup], pB[kLevelGroup], wB[kLevelGroup];
+    double tC[kLevelGroup], pC[kLevelGroup], wC[kLevelGroup];
+    auto load_group = [&](double (&rt)[kLevelGroup], double (&rp)[kLevelGroup], double (&rw)[kLevelGroup], int q0) {
 #pragma unroll
         for (int u = 0; u < kLevelGroup; u++) {
             const int q = q0 + u * 64 + lane;
-            rj[u] = -1;
-            rt[u] = 0.0;
+            rt[u] = 0.0;   // a cell past the plane: T = 0, skipped
             rp[u] = 0.0;
+            rw[u] = 0.0;
             if (q < n) {
                 const int j = q / sh.km, k = q - j * sh.km;
                 const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-                rj[u] = j;
                 rt[u] = t[x];
                 rp[u] = pstat[x];
+                rw[u] = g.sin_weight[j];
             }
         }
     };
     double acc = 0.0;
-    load_group(0);
-    for (int q0 = 0; q0 < n; q0 += kLevelGroup * 64) {
+    auto step = [&](double (&rt)[kLevelGroup], double (&rp)[kLevelGroup], double (&rw)[kLevelGroup], int q0) {
 #pragma unroll
         for (int u = 0; u < kLevelGroup; u++) {
             double term = 0.0, wt = 0.0;
             const double T = rt[u];
-            if (rj[u] >= 0 && T > 0.0) {
+            if (T > 0.0) {
                 const double b = c.g * rp[u] / (c.hyd_bnum * T * c.t_ref);
                 if (isfinite(b)) {
-                    const double wgt = g.sin_weight[rj[u]];
-                    term = wgt * b;
-                    wt = wgt;
+                    term = rw[u] * b;
+                    wt = rw[u];
                 }
             }
             mine[u * 64 + lane] = (wv == 0) ? term : wt;
         }
         __syncthreads();
-        load_group(q0 + kLevelGroup * 64);      // in flight while the chain below runs
+        load_group(rt, rp, rw, q0 + 3 * kG);
         // the whole group (terms past the plane are +0.0, which leaves the sum unchanged) in
         // batches held in two register sets: the LDS reads of one batch are in flight while the
         // other batch adds
-        constexpr int kBatch = 8, kG = kLevelGroup * 64;
+        constexpr int kBatch = 8;
         double xa[kBatch], xb[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; u++) xa[u] = mine[u];
@@ -84,6 +86,14 @@ __global__ void __launch_bounds__(128) k_level_serial(Shape sh, DevGeo g, StepCo
             for (int u = 0; u < kBatch; u++) acc += xb[u];
         }
         __syncthreads();
+    };
+    load_group(tA, pA, wA, 0);
+    load_group(tB, pB, wB, kG);
+    load_group(tC, pC, wC, 2 * kG);
+    for (int q0 = 0; q0 < n; q0 += 3 * kG) {
+        step(tA, pA, wA, q0);
+        if (q0 + kG < n) step(tB, pB, wB, q0 + kG);
+        if (q0 + 2 * kG < n) step(tC, pC, wC, q0 + 2 * kG);
     }
     if (wv == 1 && lane == 0) wsum = acc;
     __syncthreads();
