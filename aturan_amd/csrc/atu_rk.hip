@@ -51,22 +51,29 @@ __device__ __forceinline__ R fp64_diff(double a, double b, double s) {
     return (R)((a - b) * s);
 }
 
+// Streams touched once per launch (the accumulator, y_n, the next stage input, the step output)
+// bypass the caches' retention so that the stencil rows shared by neighbouring tiles stay in them.
+template <typename T>
+__device__ __forceinline__ T nt_ld(const T* p) { return __builtin_nontemporal_load(p); }
+template <typename T>
+__device__ __forceinline__ void nt_st(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Fold one tendency into the accumulator and form the next input (pass B) or the step (final).
 template <int STAGE, int CLAMP, typename S>
 __device__ __forceinline__ void fold(const StageArgs<S>& a, int f, size_t c, double y0, double r) {
     const size_t q = (size_t)f * a.sh.fs + c;
     if (STAGE < 3) {
         if (STAGE == 0)
-            a.acc[q] = 1.0 * r;
+            nt_st(a.acc + q, 1.0 * r);
         else
-            a.acc[q] = a.acc[q] + 2.0 * r;
+            nt_st(a.acc + q, nt_ld(a.acc + q) + 2.0 * r);
         const double cin = (STAGE == 2) ? a.c.c_full : a.c.c_half;
         double y = y0 + cin * r;
         if (CLAMP == CL_TKE) y = y < 0.0 ? 0.0 : (y > a.c.tke_max ? a.c.tke_max : y);
         if (CLAMP == CL_DIS) y = (y < a.c.dis_min) ? a.c.dis_min : y;
-        a.xnext[q] = (S)y;
+        nt_st(a.xnext + q, (S)y);
     } else {
-        const double s = a.acc[q] + 1.0 * r;
+        const double s = nt_ld(a.acc + q) + 1.0 * r;
         double y = y0 + a.c.dt * s / 6.0;
         if (CLAMP == CL_T) y = y < a.c.t_min ? a.c.t_min : (y > a.c.t_max ? a.c.t_max : y);
         if (CLAMP == CL_TKE) {
@@ -74,8 +81,8 @@ __device__ __forceinline__ void fold(const StageArgs<S>& a, int f, size_t c, dou
             y = y < 0.0 ? 0.0 : (y > a.c.tke_max ? a.c.tke_max : y);
         }
         if (CLAMP == CL_DIS) y = (y < a.c.dis_min) ? a.c.dis_min : y;
-        a.yout[q] = y;
-        if (a.ymirror) a.ymirror[q] = (S)y;
+        nt_st(a.yout + q, y);
+        if (a.ymirror) nt_st(a.ymirror + q, (S)y);
     }
 }
 
@@ -128,8 +135,8 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
     // formed from y_n's FP32 mirror (at stage 0 the stencil centre itself); FP64 y_n elsewhere
     auto Y0 = [&](int f, R center) -> double {
         if (STAGE == 0) return (double)center;
-        if (STAGE < 3 && sizeof(S) == sizeof(float)) return (double)a.yn_lo[(size_t)f * fs + c];
-        return a.yn[(size_t)f * fs + c];
+        if (STAGE < 3 && sizeof(S) == sizeof(float)) return (double)nt_ld(a.yn_lo + (size_t)f * fs + c);
+        return nt_ld(a.yn + (size_t)f * fs + c);
     };
     auto FZ = [&](int f) -> R { return (R)a.frozen[(size_t)(f - ATU_NUE) * fs + c]; };
 
