@@ -138,7 +138,7 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         if (STAGE < 3 && sizeof(S) == sizeof(float)) return (double)nt_ld(a.yn_lo + (size_t)f * fs + c);
         return nt_ld(a.yn + (size_t)f * fs + c);
     };
-    auto FZ = [&](int f) -> R { return (R)a.frozen[(size_t)(f - ATU_NUE) * fs + c]; };
+    auto FZ = [&](int f) -> R { return (R)nt_ld(a.frozen + (size_t)(f - ATU_NUE) * fs + c); };
 
     // velocities and the pressure gradient
     D7<R> U{}, V{}, W{};
@@ -197,8 +197,8 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
         const R rhs_u = -dpdr_term - transport_u + diffusion_u / re + diffusion_u * nue_t - sponge * u0;
         fold<STAGE, CL_NONE>(a, ATU_U, c, Y0(ATU_U, u0), (double)rhs_u);
         if (STAGE == 3) {
-            a.aux[c] = (S)(rhs_u + dpdr_term);
-            if (a.rhs3) a.rhs3[c] = (double)rhs_u;
+            nt_st(a.aux + c, (S)(rhs_u + dpdr_term));
+            if (a.rhs3) nt_st(a.rhs3 + c, (double)rhs_u);
         }
     }
     const R cor_sc = (R)C.Coriolis * (R)C.scale_Cor;
@@ -217,8 +217,8 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
                             - cor_sc * Coriolis_the;
             fold<STAGE, CL_NONE>(a, ATU_V, c, Y0(ATU_V, v0), (double)rhs_v);
             if (STAGE == 3) {
-                a.aux[fs + c] = (S)(rhs_v + dpdthe_term);
-                if (a.rhs3) a.rhs3[fs + c] = (double)rhs_v;
+                nt_st(a.aux + fs + c, (S)(rhs_v + dpdthe_term));
+                if (a.rhs3) nt_st(a.rhs3 + fs + c, (double)rhs_v);
             }
         }
         // ---- zonal velocity w (:307-308, :344-348, :720, :780-788) ----
@@ -234,8 +234,8 @@ __device__ __forceinline__ void rk_cell(const StageArgs<S>& a, int i, int j, int
                             - cor_sc * Coriolis_phi;
             fold<STAGE, CL_NONE>(a, ATU_W, c, Y0(ATU_W, w0), (double)rhs_w);
             if (STAGE == 3) {
-                a.aux[2 * fs + c] = (S)(rhs_w + dpdphi_term);
-                if (a.rhs3) a.rhs3[2 * fs + c] = (double)rhs_w;
+                nt_st(a.aux + 2 * fs + c, (S)(rhs_w + dpdphi_term));
+                if (a.rhs3) nt_st(a.rhs3 + 2 * fs + c, (double)rhs_w);
             }
         }
 
