@@ -10,13 +10,15 @@ eddy viscosity that is negative in places). The enrichment uses integer hashing 
 that re-creates the input on another machine gets the same bits.
 
 A fixture keeps, per case, every field at a fixed sample of cells (the radial, polar and seam
-planes, the neighbours of the sin(theta) floor edge and the interior) plus exact (math.fsum) sums
+planes, the neighbours of the sin(theta) floor edge and the interior) plus exact (correctly rounded, as math.fsum) sums
 and sums of squares of every field, and the buoyancy level means.
 """
 from __future__ import annotations
 
 import math
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
+from fractions import Fraction
 import sys
 import tempfile
 from pathlib import Path
@@ -109,10 +111,41 @@ def enriched_state(grid, params, seed: int = SEED) -> np.ndarray:
     return F
 
 
+def exact_sum(a: np.ndarray) -> float:
+    """The correctly rounded sum of float64 values (math.fsum's result), vectorised: each value is an
+    integer mantissa times a power of two; the mantissas, split into 27- and 26-bit halves, are summed
+    per exponent by bincount (every partial sum is an integer below 2**53, so exact in float64) and the
+    per-exponent totals combined as Python integers."""
+    x = np.ascontiguousarray(a, dtype=np.float64).ravel()
+    if x.size == 0:
+        return 0.0
+    if not np.isfinite(x).all():
+        return math.fsum(x.tolist())
+    m, e = np.frexp(x)
+    mi = (m * 9007199254740992.0).astype(np.int64)   # m * 2**53: the 53-bit integer mantissa
+    hi = mi >> 26
+    lo = mi - (hi << 26)
+    e = e.astype(np.int64) - 53
+    e0 = int(e.min())
+    idx = e - e0
+    hs = np.bincount(idx, weights=hi.astype(np.float64))
+    ls = np.bincount(idx, weights=lo.astype(np.float64))
+    total = 0
+    for b in np.flatnonzero((hs != 0) | (ls != 0)).tolist():
+        total += ((int(hs[b]) << 26) + int(ls[b])) << b
+    return float(Fraction(total) * Fraction(2) ** e0) if e0 < 0 else float(total << e0)
+
+
+def exact_sums(arrs, square: bool = False) -> np.ndarray:
+    """exact_sum of each array (or of its squares), the arrays on 8 threads (numpy releases the GIL)."""
+    with ThreadPoolExecutor(8) as ex:
+        return np.array(list(ex.map(lambda a: exact_sum(np.square(a) if square else a), list(arrs))))
+
+
 def summarize(F: np.ndarray, level: np.ndarray) -> dict:
     ii, jj, kk = np.ix_(SAMPLE_I, SAMPLE_J, SAMPLE_K)
-    sums = np.array([math.fsum(f.ravel().tolist()) for f in F])
-    sq = np.array([math.fsum((f.ravel() ** 2).tolist()) for f in F])
+    sums = exact_sums(F)
+    sq = exact_sums(F, square=True)
     return {"samples": F[:, ii, jj, kk].copy(), "sums": sums, "sumsq": sq, "level": level.copy()}
 
 
@@ -122,8 +155,8 @@ def summarize_turb(T3: np.ndarray, vel_star: np.ndarray) -> dict:
     arrs = list(T3) + [vel_star]
     return {"turb_samples": T3[:, ii, jj, kk].copy(),
             "turb_vs_samples": vel_star[np.ix_(SAMPLE_J, SAMPLE_K)].copy(),
-            "turb_sums": np.array([math.fsum(a.ravel().tolist()) for a in arrs]),
-            "turb_sumsq": np.array([math.fsum((a.ravel() ** 2).tolist()) for a in arrs])}
+            "turb_sums": exact_sums(arrs),
+            "turb_sumsq": exact_sums(arrs, square=True)}
 
 
 def turb_input(grid, seed: int = SEED) -> tuple[np.ndarray, np.ndarray]:

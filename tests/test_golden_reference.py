@@ -127,3 +127,16 @@ def test_enriched_input_is_deterministic():
     b = MG.enriched_state(g, p)
     assert np.array_equal(a, b)
     assert math.isclose(float(np.mean(a[A.FIELD["nue"]] < 0)), 0.1, abs_tol=0.1)
+
+
+def test_exact_sum_matches_fsum():
+    """The fixtures' vectorised exact sums equal math.fsum: wide exponent spread, subnormals,
+    cancellation, zeros and an empty array."""
+    rng = np.random.default_rng(7)
+    cases = [rng.standard_normal(200_000) * 10.0 ** rng.integers(-300, 300, 200_000),
+             rng.standard_normal(100_000), np.array([1e308, -1e308, 1e-320, 3.0]),
+             np.array([1.0, 1e100, 1.0, -1e100]), np.array([0.1] * 10), np.zeros(5), np.array([])]
+    for a in cases:
+        assert MG.exact_sum(a) == math.fsum(a.tolist())
+        if a.size and np.max(np.abs(a)) < 1e150:   # squares stay finite
+            assert MG.exact_sum(np.square(a)) == math.fsum(np.square(a).tolist())
