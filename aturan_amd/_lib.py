@@ -2,10 +2,17 @@
 
 The library is loaded from this package directory only; a missing library raises at import of
 the binding (there is no CPU fallback on the product path).
+
+One HIP runtime per process: the library is built against the system ROCm's libamdhip64, and a
+framework that bundles its own copy (torch's wheel does, under the same soname) must not have
+loaded it first, or the library would run on that copy. load() therefore maps the system runtime
+by its plain names first, so that a framework imported afterwards binds to the same copy, and
+raises when another copy is already in the process or a second one appears.
 """
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libaturan_amd.so"
@@ -115,6 +122,8 @@ _SIGS = {
     "atu_synchronize": (C.c_int, [C.c_void_p]),
     "atu_set_field": (C.c_int, [C.c_void_p, C.c_int, _DP]),
     "atu_get_field": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "atu_set_field_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "atu_get_field_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "atu_hydro_prep": (C.c_int, [C.c_void_p, _DP]),
     "atu_rk4": (C.c_int, [C.c_void_p]),
     "atu_apply_bc": (C.c_int, [C.c_void_p]),
@@ -127,7 +136,12 @@ _SIGS = {
     "atu_window_forward": (C.c_int, [C.c_void_p, C.POINTER(WindowOpts), _DP]),
     "atu_window_backward": (C.c_int, [C.c_void_p, C.c_double, _DP, _DP]),
     "atu_window_rewind": (C.c_int, [C.c_void_p]),
+    "atu_window_backward_device": (C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_int, _DP]),
+    "atu_window_generation": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "atu_build_info": (C.c_char_p, []),
+    "atu_runtime_versions": (C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "atu_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _DP]),
+    "atu_device_memory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "atu_damp_wiggles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]),
     "atu_satadj_opts_default": (C.c_int, [C.POINTER(SatAdjOpts)]),
     "atu_saturation_adjust": (C.c_int, [C.c_void_p, C.POINTER(SatAdjOpts)]),
@@ -143,21 +157,48 @@ _SIGS = {
 }
 
 _lib = None
+RUNTIME_PATH: str | None = None   # the libamdhip64 this process runs the library on
+
+
+def mapped_runtimes() -> list[str]:
+    """Real paths of every libamdhip64 mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            paths = {ln.split()[-1] for ln in f if "libamdhip64" in ln and "/" in ln}
+    except OSError:
+        return []
+    return sorted({os.path.realpath(p) for p in paths})
 
 
 def load() -> C.CDLL:
-    """Load libaturan_amd.so from this package; raise if it has not been built."""
-    global _lib
+    """Load libaturan_amd.so from this package; raise if it has not been built, or if the process
+    would run it on a HIP runtime other than the one it was built for."""
+    global _lib, RUNTIME_PATH
     if _lib is not None:
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is missing: build it with `python aturan_amd/build.py` "
                           "(the HIP library is required; there is no fallback)")
+    before = mapped_runtimes()
+    if not before:
+        # the system runtime under the names a framework's libraries ask for, so that they bind to it
+        for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+            C.CDLL(name, mode=C.RTLD_GLOBAL)
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    after = mapped_runtimes()
+    built, running = C.c_int(0), C.c_int(0)
+    rc = lib.atu_runtime_versions(C.byref(built), C.byref(running))
+    if len(after) > 1 or (rc == 0 and built.value // 100000 != running.value // 100000):
+        raise ImportError(
+            f"libaturan_amd needs the HIP runtime it was built for (HIP {built.value // 10000000}."
+            f"{built.value // 100000 % 100}); this process maps {after} (running HIP {running.value // 10000000}."
+            f"{running.value // 100000 % 100}). Import aturan_amd before any framework that bundles its own ROCm "
+            f"runtime (such as torch).")
+    RUNTIME_PATH = after[0] if after else None
     _lib = lib
     return lib
 

@@ -25,8 +25,7 @@ COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-munsafe-fp-atomics"]
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def _hipcc() -> str:

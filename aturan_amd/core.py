@@ -47,6 +47,27 @@ def synthetic_state(grid: L.Grid, params: L.Params, seed: int = 20261203) -> np.
     return out
 
 
+def runtime_versions() -> tuple[int, int]:
+    """(HIP_VERSION the library was built with, hipRuntimeGetVersion of the runtime it runs on)."""
+    b, r = C.c_int(), C.c_int()
+    _check(L.load().atu_runtime_versions(C.byref(b), C.byref(r)), None, "atu_runtime_versions")
+    return b.value, r.value
+
+
+def device_memory() -> tuple[int, int]:
+    """(free, total) bytes of the current device (atu_device_memory)."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(L.load().atu_device_memory(C.byref(f), C.byref(t)), None, "atu_device_memory")
+    return f.value, t.value
+
+
+def copy_probe(nbytes: int, launches: int = 20) -> float:
+    """Mean milliseconds of a device-to-device copy of nbytes (atu_copy_probe)."""
+    ms = C.c_double()
+    _check(L.load().atu_copy_probe(C.c_size_t(nbytes), launches, C.byref(ms)), None, "atu_copy_probe")
+    return ms.value
+
+
 def cp_table(mode: int = 2) -> np.ndarray:
     lib = L.load()
     n = C.c_int(0)
@@ -150,6 +171,19 @@ class UranusCore:
         out = np.empty(self.shape, dtype=np.float64)
         _check(self._lib.atu_get_field(self._ctx, f, out.ctypes.data_as(L._DP)), self._ctx, "atu_get_field")
         return out
+
+    def set_field_device(self, name: str | int, dptr: int, ld_k: int | None = None) -> None:
+        """Copy a dense FP64 device array [im][jm][ld_k] (an address in this device's memory, e.g. a
+        tensor's data_ptr()) into a field (atu_set_field_device)."""
+        f = L.FIELD[name] if isinstance(name, str) else name
+        _check(self._lib.atu_set_field_device(self._ctx, f, C.c_void_p(dptr), ld_k or self.grid.km), self._ctx,
+               "atu_set_field_device")
+
+    def get_field_device(self, name: str | int, dptr: int, ld_k: int | None = None) -> None:
+        """Copy a field into a dense FP64 device array [im][jm][ld_k] (atu_get_field_device)."""
+        f = L.FIELD[name] if isinstance(name, str) else name
+        _check(self._lib.atu_get_field_device(self._ctx, f, C.c_void_p(dptr), ld_k or self.grid.km), self._ctx,
+               "atu_get_field_device")
 
     def set_state(self, fields: np.ndarray) -> None:
         for f in range(L.NFIELDS):
@@ -267,6 +301,21 @@ class UranusCore:
         _check(self._lib.atu_window_backward(self._ctx, float(dloss), g.ctypes.data_as(L._DP), C.byref(gn)),
                self._ctx, "atu_window_backward")
         return g, gn.value
+
+    def window_backward_device(self, dloss: float, grad_t0_dptr: int, ld_k: int | None = None) -> float:
+        """As window_backward, dL/dT0 written to a dense FP64 device array [im][jm][ld_k]; returns
+        dloss * dL/dnue."""
+        gn = C.c_double()
+        _check(self._lib.atu_window_backward_device(self._ctx, float(dloss), C.c_void_p(grad_t0_dptr),
+                                                    ld_k or self.grid.km, C.byref(gn)),
+               self._ctx, "atu_window_backward_device")
+        return gn.value
+
+    def window_generation(self) -> int:
+        """Identifies the forward the next window_backward differentiates (0: none, or its inputs changed)."""
+        g = C.c_uint64()
+        _check(self._lib.atu_window_generation(self._ctx, C.byref(g)), self._ctx, "atu_window_generation")
+        return g.value
 
     def window_rewind(self) -> None:
         """State (prognostic fields, p_dyn) back to the start of the last window_forward."""

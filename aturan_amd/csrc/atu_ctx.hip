@@ -5,6 +5,7 @@
 #include "atu_kernels.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -13,6 +14,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace atu;
@@ -57,6 +59,7 @@ struct atu_ctx {
     double* SRC = nullptr;
     double* RES = nullptr;    // the consistent source b = src - shift of the converging solve
     double* LEVEL = nullptr;  // im
+    double* LEVROWS = nullptr;// 2 im jm: the row sums of the level means
     double* RPART = nullptr;  // residual partials
     double* RSUM = nullptr;   // 2
     void* geo_blob = nullptr;
@@ -109,7 +112,10 @@ struct atu_ctx {
     // gradients over a window of steps (allocated by the first atu_window_forward)
     struct Window {
         atu_window_opts opts{};
-        bool ready = false;          // a forward run whose checkpoints are held
+        bool held = false;           // the checkpoints of a completed forward are held (rewind)
+        bool ready = false;          // and the inputs the backward recomputes from are unchanged
+        uint64_t generation = 0;     // counts completed forwards; 0 = none held
+        uint64_t forwards = 0;
         int K = 0, nck = 0;          // checkpoint interval, checkpoints held
         size_t tape_steps = 0, ck_slots = 0;
         double* ckpt = nullptr;      // ck_slots x (19 prognostic + p_dyn) fields; the last slot is y_N
@@ -166,12 +172,19 @@ struct atu_ctx {
     }
     // side stream for the hydrostatic prep of the next step, which overlaps the Poisson solve
     hipStream_t side = nullptr;
-    hipEvent_t ev_t3 = nullptr, ev_tbc = nullptr, ev_bc = nullptr, ev_hydro = nullptr;
+    hipEvent_t ev_bc = nullptr, ev_hydro = nullptr;
     double* prog(double* buf, int f) const { return buf + (size_t)f * sh.fs; }
+    // A write of an input the window's backward recomputes from (a frozen field, p_stat) makes the
+    // held forward no longer the one the backward would differentiate: the backward then refuses.
+    // Prognostic fields and p_dyn are restored from the checkpoints and do not invalidate it.
+    void field_changed(int f) {
+        if (f >= ATU_NUE && f != ATU_P_DYN && f != ATU_P_HYDRO && !(f >= ATU_AUX_U && f <= ATU_AUX_W))
+            win.ready = false;
+    }
 };
 
 atu_ctx::~atu_ctx() {
-    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, RPART, RSUM, SROWS, SROWS2, PPREV, PPREV2, DAMP, SCAL,
+    for (double* p : {Y, SA, SB, ACC, FROZEN, PSTAT, PDYN, PHYDRO, AUX, SRC, RES, LEVEL, LEVROWS, RPART, RSUM, SROWS, SROWS2, PPREV, PPREV2, DAMP, SCAL,
                       BR, BAC, BAS, TURB3, VSTAR, TURBTAB})
         if (p) hipFree(p);
     if (solve_blob) hipFree(solve_blob);
@@ -186,8 +199,6 @@ atu_ctx::~atu_ctx() {
         if (p) hipFree(p);
     for (float* p : {Y32, SA32, SB32, FROZEN32, AUX32, E32, R32})
         if (p) hipFree(p);
-    if (ev_t3) hipEventDestroy(ev_t3);
-    if (ev_tbc) hipEventDestroy(ev_tbc);
     if (ev_bc) hipEventDestroy(ev_bc);
     if (ev_hydro) hipEventDestroy(ev_hydro);
     if (side) hipStreamDestroy(side);
@@ -344,6 +355,22 @@ int get_field(atu_ctx* c, const double* src, double* host) {
                   "hipMemcpy2DAsync(D2H)");
 }
 
+// The same with a dense device array of rows of `ld` >= km elements ([im][jm][ld], FP64).
+int put_field_dev(atu_ctx* c, double* dst, const double* src, int ld) {
+    const Shape& s = c->sh;
+    return c->hip(hipMemcpy2DAsync(dst, (size_t)s.ldk * sizeof(double), src, (size_t)ld * sizeof(double),
+                                   (size_t)s.km * sizeof(double), (size_t)s.im * s.jm, hipMemcpyDeviceToDevice,
+                                   c->stream),
+                  "hipMemcpy2DAsync(D2D)");
+}
+int get_field_dev(atu_ctx* c, const double* src, double* dst, int ld) {
+    const Shape& s = c->sh;
+    return c->hip(hipMemcpy2DAsync(dst, (size_t)ld * sizeof(double), src, (size_t)s.ldk * sizeof(double),
+                                   (size_t)s.km * sizeof(double), (size_t)s.im * s.jm, hipMemcpyDeviceToDevice,
+                                   c->stream),
+                  "hipMemcpy2DAsync(D2D)");
+}
+
 double* field_ptr(atu_ctx* c, int f) {
     if (f >= 0 && f < ATU_NPROG) return c->prog(c->Y, f);
     if (f >= ATU_NUE && f < ATU_NUE + ATU_NFROZEN) return c->FROZEN + (size_t)(f - ATU_NUE) * c->sh.fs;
@@ -403,17 +430,6 @@ StageArgs<float> stage_args32(atu_ctx* c, int stage) {
     a.aux = c->AUX32;
     a.ymirror = nullptr;
     a.yn_lo = c->Y32;
-    // rows [jb0, jb1) run in FP32 when the FP64 rows are the two bands next to the poles (sin(theta)
-    // is monotone on either side of the equator); otherwise one launch picks per row (jb0 = jb1 = 0)
-    const std::vector<double>& pr = c->hgeo.pole_row;
-    const int jm = c->sh.jm;
-    int j0 = 1, j1 = jm - 1;
-    while (j0 < jm - 1 && pr[j0] != 0.0) j0++;
-    while (j1 > j0 && pr[j1 - 1] != 0.0) j1--;
-    bool banded = true;
-    for (int j = j0; j < j1; j++) banded = banded && pr[j] == 0.0;
-    a.jb0 = banded ? j0 : 0;
-    a.jb1 = banded ? j1 : 0;
     switch (stage) {
         case 0: a.xs = c->Y32; a.xnext = c->SA32; break;
         case 1: a.xs = c->SA32; a.xnext = c->SB32; break;
@@ -429,86 +445,38 @@ BcTargets bc_targets(atu_ctx* c) {
                       : BcTargets{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
 }
 
-hipError_t launch_stage_part(atu_ctx* c, int s, int part) {
-    return c->mixed() ? launch_rk_stage_mixed(s, stage_args32(c, s), c->stream, part)
-                      : launch_rk_stage_f64(s, stage_args(c, s), c->stream, part);
-}
-
-// The last stage with the next step's hydrostatic prep started early: t's update first, then on
-// the side stream t's BC (its own fields only) and the level means, while the main stream updates
-// the other fields. ev_tbc marks t's BC on the side stream.
-int rk_last_stage_split(atu_ctx* c) {
-    const BcTargets t = bc_targets(c);
-    const hipError_t e = c->timed(PC_RK, [&]() -> hipError_t {
-        hipError_t r;
-        if ((r = launch_stage_part(c, 3, 1)) != hipSuccess || (r = hipEventRecord(c->ev_t3, c->stream)) != hipSuccess ||
-            (r = hipStreamWaitEvent(c->side, c->ev_t3, 0)) != hipSuccess ||
-            (r = c->timed_on(PC_BC, c->side,
-                             [&] { return launch_bc_part(c->sh, c->params, t, ATU_T, ATU_T + 1, false, c->side); })) !=
-                hipSuccess ||
-            (r = hipEventRecord(c->ev_tbc, c->side)) != hipSuccess ||
-            (r = c->timed_on(PC_HYDRO, c->side,
-                             [&] {
-                                 return launch_level(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T), c->PSTAT,
-                                                     c->LEVEL, c->side);
-                             })) != hipSuccess)
-            return r;
-        return launch_stage_part(c, 3, 2);
-    });
-    return c->hip(e, "rk_stage");
-}
-
-// The four stages; with `overlap`, the last one as rk_last_stage_split (the caller then runs
-// do_bc_rest and side_scan_after_bc instead of do_bc). With `hydro_pending` the hydrostatic prep is
-// still running on the side stream: the first stage updates every field but v and w, then waits
-// for it (ev_hydro) and updates v and w, the only fields whose tendencies read p_hydro.
-int do_rk4(atu_ctx* c, bool overlap = false, bool hydro_pending = false) {
-    c->rhs_valid = c->keep_rhs && !c->mixed();
-    for (int s = 0; s < 4; s++) {
-        if (s == 3 && overlap) return rk_last_stage_split(c);
-        if (s == 0 && hydro_pending) {
-            if (c->hip(c->timed(PC_RK, [&] { return launch_stage_part(c, 0, 3); }), "rk_stage") ||
-                c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent") ||
-                c->hip(c->timed(PC_RK, [&] { return launch_stage_part(c, 0, 4); }), "rk_stage"))
-                return ATU_EHIP;
-            continue;
-        }
-        if (c->hip(c->timed(PC_RK, [&] { return launch_stage_part(c, s, 0); }), "rk_stage")) return ATU_EHIP;
-    }
-    return ATU_OK;
+hipError_t launch_stage(atu_ctx* c, int s) {
+    return c->mixed() ? launch_rk_stage_mixed(s, stage_args32(c, s), c->stream)
+                      : launch_rk_stage_f64(s, stage_args(c, s), c->stream);
 }
 
 int do_hydro(atu_ctx* c, hipStream_t s) {
     return c->hip(c->timed_on(PC_HYDRO, s,
                               [&] {
                                   return launch_hydro(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T),
-                                                      c->prog(c->Y, ATU_V), c->prog(c->Y, ATU_W), c->PSTAT, c->LEVEL,
-                                                      c->PHYDRO, s);
+                                                      c->prog(c->Y, ATU_V), c->prog(c->Y, ATU_W), c->PSTAT, c->LEVROWS,
+                                                      c->LEVEL, c->PHYDRO, s);
                               }),
                   "hydro");
 }
 
-// The next step's hydrostatic prep on the side stream, in two parts so that it overlaps the end of
-// this step: the level means from the last RK stage on (rk_last_stage_split), the column scan after
-// the BC.
-// the BC of every field but t (after rk_last_stage_split)
-int do_bc_rest(atu_ctx* c) {
-    const BcTargets t = bc_targets(c);
-    return c->hip(c->timed(PC_BC, [&] { return launch_bc_part(c->sh, c->params, t, ATU_T + 1, ATU_NPROG, true,
-                                                               c->stream); }), "bc");
+// The four stages. With `hydro_pending` this step's hydrostatic prep is still running on the side
+// stream (side_hydro_after_bc): the first stage, which reads p_hydro, waits for it.
+int do_rk4(atu_ctx* c, bool hydro_pending = false) {
+    c->rhs_valid = c->keep_rhs && !c->mixed();
+    if (hydro_pending && c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent")) return ATU_EHIP;
+    for (int s = 0; s < 4; s++)
+        if (c->hip(c->timed(PC_RK, [&] { return launch_stage(c, s); }), "rk_stage")) return ATU_EHIP;
+    return ATU_OK;
 }
-int side_scan_after_bc(atu_ctx* c) {
+
+// The next step's hydrostatic prep on the side stream, once this step's BC is done: it reads t, v and
+// w of y_{n+1}, which the Poisson solve running meanwhile on the main stream does not touch, and
+// writes the level means and p_hydro, which the solve does not read. ev_hydro marks its end.
+int side_hydro_after_bc(atu_ctx* c) {
     int e;
     if ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
-        (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")))
-        return e;
-    if ((e = c->hip(c->timed_on(PC_HYDRO, c->side,
-                                [&] {
-                                    return launch_hydro_scan(c->sh, c->geo, c->consts, c->prog(c->Y, ATU_T),
-                                                             c->prog(c->Y, ATU_V), c->prog(c->Y, ATU_W), c->PSTAT,
-                                                             c->LEVEL, c->PHYDRO, c->side);
-                                }),
-                    "hydro_scan")))
+        (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")) || (e = do_hydro(c, c->side)))
         return e;
     return c->hip(hipEventRecord(c->ev_hydro, c->side), "hipEventRecord");
 }
@@ -581,9 +549,23 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     // host polls an event recorded after the batch (no copy, no blocking wait). mark_ctl records it;
     // read_ctl waits for it and reads the mailbox.
     auto mark_ctl = [&]() -> int { return H(hipEventRecord(c->ev_ctl, s), "hipEventRecord"); };
+    // The wait is bounded: a batch that has not finished after kWaitBase + kWaitPerIter per queued
+    // iteration (far beyond any solve of the grids this library accepts) returns ATU_EHIP instead of
+    // waiting for ever. The host spins for the first 100 us of a wait and then sleeps 20 us between
+    // polls, so a long wait does not hold a host core.
     auto read_ctl = [&]() -> int {
+        constexpr double kWaitBase = 10.0, kWaitPerIter = 0.05;
+        const auto t0 = std::chrono::steady_clock::now();
+        const double limit = kWaitBase + kWaitPerIter * max_it;
         hipError_t q;
         while ((q = hipEventQuery(c->ev_ctl)) == hipErrorNotReady) {
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el > limit) {
+                char m[160];
+                std::snprintf(m, sizeof m, "converging solve: a batch of iterations did not finish within %.1f s", limit);
+                return c->fail(ATU_EHIP, m);
+            }
+            if (el > 1e-4) std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
         if (H(q, "hipEventQuery")) return ATU_EHIP;
         const volatile SolveMailbox* mb = c->MBOX;
@@ -866,7 +848,7 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
     if (slot >= 0 && ((e = dcopy(c, tape_field(c, slot, TP_X0), c->Y, 4)) ||
                       (e = dcopy(c, tape_field(c, slot, TP_P), c->PDYN, 1))))
         return e;
-    if (first) {
+    if (first) {   // else the previous step's side_hydro_after_bc has queued this step's prep
         if ((e = do_hydro(c, c->stream))) return e;
     } else if ((e = c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent"))) {
         return e;
@@ -876,10 +858,6 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
                                  "hipMemcpyAsync(level)")))
         return e;
     for (int s = 0; s < 4; s++) {
-        if (s == 3 && !last) {
-            if ((e = rk_last_stage_split(c))) return e;
-            break;
-        }
         if (!c->mixed()) {
             const StageArgs<double> a = stage_args(c, s);
             if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")))
@@ -895,11 +873,9 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
                 return e;
         }
     }
-    if ((e = last ? do_bc(c) : do_bc_rest(c))) return e;
-    if (slot >= 0 && ((!last && (e = c->hip(hipStreamWaitEvent(c->stream, c->ev_tbc, 0), "hipStreamWaitEvent"))) ||
-                      (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))))
-        return e;
-    if (!last && (e = side_scan_after_bc(c))) return e;
+    if ((e = do_bc(c))) return e;
+    if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))) return e;
+    if (!last && (e = side_hydro_after_bc(c))) return e;
     if ((e = do_pressure(c, &o, nullptr))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_SRC), c->SRC, 1))) return e;
     return ATU_OK;
@@ -1137,6 +1113,19 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         g_create_error = "atu_create: unknown precision";
         return ATU_EINVAL;
     }
+    {   // the HIP runtime of this process must be the one the library was built for: a second copy of
+        // libamdhip64 (a framework's bundled runtime, loaded first under the same soname) is refused
+        int rv = 0;
+        if (hipRuntimeGetVersion(&rv) != hipSuccess || rv / 100000 != HIP_VERSION / 100000) {
+            char m[256];
+            std::snprintf(m, sizeof m,
+                          "atu_create: libaturan_amd was built for HIP %d.%d but runs on HIP runtime %d.%d (another "
+                          "libamdhip64 was loaded into this process first)",
+                          HIP_VERSION_MAJOR, HIP_VERSION_MINOR, rv / 10000000, (rv / 100000) % 100);
+            g_create_error = m;
+            return ATU_EUNSUPPORTED;
+        }
+    }
     std::unique_ptr<atu_ctx> c(new (std::nothrow) atu_ctx());
     if (!c) return ATU_ENOMEM;
     c->grid = *g;
@@ -1159,8 +1148,6 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     }
     c->own_stream = true;
     if ((e = c->hip(hipStreamCreate(&c->side), "hipStreamCreate(side)")) ||
-        (e = c->hip(hipEventCreateWithFlags(&c->ev_t3, hipEventDisableTiming), "hipEventCreate")) ||
-        (e = c->hip(hipEventCreateWithFlags(&c->ev_tbc, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_bc, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_hydro, hipEventDisableTiming), "hipEventCreate"))) {
         g_create_error = c->err;
@@ -1194,6 +1181,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     }
     const size_t nres = (size_t)residual_partials_count(c->sh) * 2;
     if ((e = c->hip(hipMalloc((void**)&c->LEVEL, (size_t)g->im * sizeof(double)), "hipMalloc")) ||
+        (e = c->hip(hipMalloc((void**)&c->LEVROWS, 2 * (size_t)g->im * g->jm * sizeof(double)), "hipMalloc")) ||
         (e = c->hip(hipMalloc((void**)&c->RPART, nres * sizeof(double)), "hipMalloc")) ||
         (e = c->hip(hipMalloc((void**)&c->RSUM, 2 * sizeof(double)), "hipMalloc"))) {
         g_create_error = c->err;
@@ -1247,6 +1235,7 @@ int atu_set_field(atu_ctx* ctx, int f, const double* host) {
     ctx->state_epoch++;
     double* d = field_ptr(ctx, f);
     if (!d) return ctx->fail(ATU_EINVAL, "atu_set_field: unknown field id");
+    ctx->field_changed(f);
     int e = put_field(ctx, d, host);
     if (!e) e = field_written(ctx, f);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
@@ -1263,6 +1252,34 @@ int atu_get_field(atu_ctx* ctx, int f, double* host) {
         e = ctx->hip(launch_convert(ctx->AUX32 + (size_t)(f - ATU_AUX_U) * ctx->sh.fs, ctx->RES, ctx->sh.fs,
                                     ctx->stream), "convert");
     if (!e) e = get_field(ctx, d, host);
+    if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return e;
+}
+
+int atu_set_field_device(atu_ctx* ctx, int f, const double* dsrc, int ld_k) {
+    if (!ctx || !dsrc) return ATU_EINVAL;
+    double* d = field_ptr(ctx, f);
+    if (!d) return ctx->fail(ATU_EINVAL, "atu_set_field_device: unknown field id");
+    if (ld_k < ctx->sh.km) return ctx->fail(ATU_EINVAL, "atu_set_field_device: ld_k < km");
+    ctx->state_epoch++;
+    ctx->field_changed(f);
+    int e = put_field_dev(ctx, d, dsrc, ld_k);
+    if (!e) e = field_written(ctx, f);
+    if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return e;
+}
+
+int atu_get_field_device(atu_ctx* ctx, int f, double* ddst, int ld_k) {
+    if (!ctx || !ddst) return ATU_EINVAL;
+    const bool aux = (f >= ATU_AUX_U && f <= ATU_AUX_W);
+    const double* d = field_ptr(ctx, f);
+    if (!d) return ctx->fail(ATU_EINVAL, "atu_get_field_device: unknown field id");
+    if (ld_k < ctx->sh.km) return ctx->fail(ATU_EINVAL, "atu_get_field_device: ld_k < km");
+    int e = ATU_OK;
+    if (ctx->mixed() && aux)
+        e = ctx->hip(launch_convert(ctx->AUX32 + (size_t)(f - ATU_AUX_U) * ctx->sh.fs, ctx->RES, ctx->sh.fs,
+                                    ctx->stream), "convert");
+    if (!e) e = get_field_dev(ctx, d, ddst, ld_k);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
 }
@@ -1311,8 +1328,15 @@ int atu_saturation_adjust(atu_ctx* ctx, const atu_satadj_opts* opts) {
     return ctx->hip(ctx->timed(PC_SATADJ, [&] { return launch_satadj(ctx->sh, a, ctx->stream); }), "satadj");
 }
 
-int atu_turbulence_init(atu_ctx* ctx, const atu_turb_opts* opts) { return do_turbulence(ctx, opts, true); }
-int atu_turbulence_run(atu_ctx* ctx, const atu_turb_opts* opts) { return do_turbulence(ctx, opts, false); }
+// the closure writes nue and the k / omega sources, inputs of the window's steps
+int atu_turbulence_init(atu_ctx* ctx, const atu_turb_opts* opts) {
+    if (ctx) ctx->field_changed(ATU_NUE);
+    return do_turbulence(ctx, opts, true);
+}
+int atu_turbulence_run(atu_ctx* ctx, const atu_turb_opts* opts) {
+    if (ctx) ctx->field_changed(ATU_NUE);
+    return do_turbulence(ctx, opts, false);
+}
 
 int atu_turb_set(atu_ctx* ctx, int which, const double* host) {
     if (!ctx || !host) return ATU_EINVAL;
@@ -1433,9 +1457,7 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
         // after the part of the first stage that does not read p_hydro
         if (n == 0 && (e = do_hydro(ctx, ctx->stream))) return e;
         const bool more = n + 1 < nsteps;
-        if ((e = do_rk4(ctx, more, n > 0)) || (e = more ? do_bc_rest(ctx) : do_bc(ctx)) ||
-            (more && (e = side_scan_after_bc(ctx))))
-            return e;
+        if ((e = do_rk4(ctx, n > 0)) || (e = do_bc(ctx)) || (more && (e = side_hydro_after_bc(ctx)))) return e;
         if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;
         if (step_ev) {
             hipEventRecord(pe.b, ctx->stream);
@@ -1452,7 +1474,8 @@ int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) 
     if (opts->solve_iters <= 0 && !(opts->rel_tol > 0.0))
         return ctx->fail(ATU_EINVAL, "atu_window_forward: give solve_iters > 0 or rel_tol > 0");
     auto& W = ctx->win;
-    W.ready = false;
+    W.held = W.ready = false;
+    W.generation = 0;
     W.opts = *opts;
     if (W.opts.checkpoint_every <= 0) W.opts.checkpoint_every = 10;
     if (!(W.opts.target_amp > 0.0)) W.opts.target_amp = 1.2;
@@ -1473,15 +1496,22 @@ int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) 
     }
     if ((e = ck_save(ctx, W.nck)) || (e = window_loss(ctx, 1.0, &W.loss))) return e;
     if (loss) *loss = W.loss;
-    W.ready = true;
+    W.held = W.ready = true;
+    W.generation = ++W.forwards;
     return ATU_OK;
 }
 
-int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* grad_nue) {
-    if (!ctx) return ATU_EINVAL;
+}  // extern "C"
+
+// The backward of the held window; dL/dT0 into a dense host array (ld < 0) or a dense device array
+// of rows of `ld` elements.
+static int window_backward(atu_ctx* ctx, double dloss, double* grad_t0, int ld, double* grad_nue) {
     ctx->state_epoch++;
     auto& W = ctx->win;
-    if (!W.ready) return ctx->fail(ATU_EINVAL, "atu_window_backward: no atu_window_forward to differentiate");
+    if (!W.held) return ctx->fail(ATU_EINVAL, "atu_window_backward: no atu_window_forward to differentiate");
+    if (!W.ready)
+        return ctx->fail(ATU_EINVAL, "atu_window_backward: a frozen input of the window (or p_stat) was written after "
+                                     "atu_window_forward; run the forward again");
     const Shape& sh = ctx->sh;
     const atu_solver_opts o = window_solver(W.opts);
     const int N = W.opts.nsteps;
@@ -1505,15 +1535,35 @@ int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* gra
                           "hipMemcpy")))
             return e;
     }
-    if (grad_t0 && (e = get_field(ctx, W.LY + (size_t)ATU_T * sh.fs, grad_t0))) return e;
+    const double* gsrc = W.LY + (size_t)ATU_T * sh.fs;
+    if (grad_t0 && (e = (ld < 0 ? get_field(ctx, gsrc, grad_t0) : get_field_dev(ctx, gsrc, grad_t0, ld)))) return e;
     if ((e = ck_restore(ctx, W.nck))) return e;
     return ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+}
+
+extern "C" {
+
+int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* grad_nue) {
+    if (!ctx) return ATU_EINVAL;
+    return window_backward(ctx, dloss, grad_t0, -1, grad_nue);
+}
+
+int atu_window_backward_device(atu_ctx* ctx, double dloss, double* grad_t0, int ld_k, double* grad_nue) {
+    if (!ctx) return ATU_EINVAL;
+    if (grad_t0 && ld_k < ctx->sh.km) return ctx->fail(ATU_EINVAL, "atu_window_backward_device: ld_k < km");
+    return window_backward(ctx, dloss, grad_t0, ld_k, grad_nue);
+}
+
+int atu_window_generation(const atu_ctx* ctx, uint64_t* generation) {
+    if (!ctx || !generation) return ATU_EINVAL;
+    *generation = (ctx->win.held && ctx->win.ready) ? ctx->win.generation : 0;
+    return ATU_OK;
 }
 
 int atu_window_rewind(atu_ctx* ctx) {
     if (!ctx) return ATU_EINVAL;
     ctx->state_epoch++;
-    if (!ctx->win.ready) return ctx->fail(ATU_EINVAL, "atu_window_rewind: no atu_window_forward to rewind to");
+    if (!ctx->win.held) return ctx->fail(ATU_EINVAL, "atu_window_rewind: no atu_window_forward to rewind to");
     int e = ck_restore(ctx, 0);
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
@@ -1529,8 +1579,30 @@ int atu_field_device_ptr(atu_ctx* ctx, int f, void** dptr, int* ld_k, int* elem_
     return ATU_OK;
 }
 
+#define ATU_STR2(x) #x
+#define ATU_STR(x) ATU_STR2(x)
 const char* atu_build_info(void) {
-    return "libaturan_amd abi " "1" " gfx950 hipcc -O3 -ffp-contract=off";
+    return "libaturan_amd abi " ATU_STR(ATU_ABI_VERSION) " gfx950 hipcc -O3 -ffp-contract=off; HIP " ATU_STR(
+        HIP_VERSION_MAJOR) "." ATU_STR(HIP_VERSION_MINOR) "." ATU_STR(HIP_VERSION_PATCH);
+}
+#undef ATU_STR
+#undef ATU_STR2
+
+int atu_device_memory(size_t* free_bytes, size_t* total_bytes) {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return ATU_EHIP;
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return ATU_OK;
+}
+
+int atu_runtime_versions(int* built, int* running) {
+    if (built) *built = HIP_VERSION;
+    if (running) {
+        *running = 0;
+        if (hipRuntimeGetVersion(running) != hipSuccess) return ATU_EHIP;
+    }
+    return ATU_OK;
 }
 
 int atu_profile_begin(atu_ctx* ctx) { return atu_profile_begin_classes(ctx, nullptr); }

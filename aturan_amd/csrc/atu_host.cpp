@@ -619,7 +619,7 @@ extern "C" int atu_params_default(const atu_grid* g, atu_params* p) {
     p->lid_vw = 2;                                      // BC_LID_VW with GEO_BALANCE (BC_Uran.h:155)
     p->rigid_lid = 1;
     p->radius_positive = 1;                             // BC_Uran.h:70
-    p->species_clamp = 0;
+    p->species_clamp = 1;                               // ATURAN_SPECIES_CLAMP, on by default (BC_Uran.h:111)
     (void)g;
     return ATU_OK;
 }

@@ -9,95 +9,68 @@ namespace {
 // ---- level means: sum over the full (j, k) plane of sin(theta) * b, skipping T <= 0 and
 // non-finite b (computeBuoyancyRefLevel, :18-37).
 //
-// The reference adds the terms of a level one after another in (j, k) order, and the buoyancy
-// anomaly divides by this mean, which magnifies any rounding difference of the sum ~200-fold in
-// p_hydro. So the sum is formed in exactly that order: one workgroup per level. Its lanes load and
-// form 64 consecutive terms at a time (a skipped term contributes an exact +0.0); a group of
-// kLevelGroup such chunks is parked in LDS and added in sequence while the loads of the group three
-// ahead are in flight, the LDS reads of each batch of the chain issued while the previous batch adds.
-// Two waves per level: wave 0 runs the chain of the weighted terms, wave 1 that of the weights, each
-// on its own SIMD. The chain is ~65k dependent adds per level; the step runs this kernel on a side
-// stream, concurrently with the previous step's Poisson solve, which it does not depend on.
-constexpr int kLevelGroup = 8;
-__global__ void __launch_bounds__(128) k_level_serial(Shape sh, DevGeo g, StepConsts c, const double* t,
-                                                      const double* pstat, double* level) {
-    const int i = blockIdx.x;
-    const int n = sh.jm * sh.km;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;   // wave 0: the sum, wave 1: the weights
-    constexpr int kG = kLevelGroup * 64;
-    __shared__ double tv[2][kG];
-    __shared__ double wsum;
-    double* mine = tv[wv];
-    // T, p_stat and sin(theta) of three groups in flight: the loads of a group go out while the
-    // chain of the group three before it runs (the chain shares the HBM with the stage kernels)
-    double tA[kLevelGroup], pA[kLevelGroup], wA[kLevelGroup];
-    double tB[kLevelGroup], pB[kLevelGroup], wB[kLevelGroup];
-    double tC[kLevelGroup], pC[kLevelGroup], wC[kLevelGroup];
-    auto load_group = [&](double (&rt)[kLevelGroup], double (&rp)[kLevelGroup], double (&rw)[kLevelGroup], int q0) {
+// The reference adds the terms of a level one after another in (j, k) order. Here every level sum
+// (and its sum of weights) runs in one fixed parallel order, which the oracle restates term for
+// term: per (i, j) row, the km terms in chunks of 64 consecutive k, each chunk reduced by the halving
+// tree v[l] += v[l + h], h = 32..1 (a k past the row, or a skipped cell, is +0.0), the chunk results
+// of a row added in sequence to 0.0; the jm row sums dealt to 64 partials (partial t adds rows t,
+// t + 64, ... in sequence to 0.0), and those reduced by the halving tree. The row sums run one
+// wavefront per row (k_level_rows), the partials and the tree one wavefront per level
+// (k_level_final).
+__device__ __forceinline__ double tree64(double v) {
 #pragma unroll
-        for (int u = 0; u < kLevelGroup; u++) {
-            const int q = q0 + u * 64 + lane;
-            rt[u] = 0.0;   // a cell past the plane: T = 0, skipped
-            rp[u] = 0.0;
-            rw[u] = 0.0;
-            if (q < n) {
-                const int j = q / sh.km, k = q - j * sh.km;
-                const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-                rt[u] = t[x];
-                rp[u] = pstat[x];
-                rw[u] = g.sin_weight[j];
-            }
-        }
-    };
-    double acc = 0.0;
-    auto step = [&](double (&rt)[kLevelGroup], double (&rp)[kLevelGroup], double (&rw)[kLevelGroup], int q0) {
-#pragma unroll
-        for (int u = 0; u < kLevelGroup; u++) {
-            double term = 0.0, wt = 0.0;
-            const double T = rt[u];
+    for (int h = 32; h >= 1; h >>= 1) v += __shfl_down(v, h, 64);
+    return v;   // the tree's result in lane 0
+}
+
+// The row sums: one wavefront per (i, j) row (four rows per workgroup), into rows[i jm + j] (terms) and
+// rows[im jm + i jm + j] (weights).
+__global__ void __launch_bounds__(256) k_level_rows(Shape sh, DevGeo g, StepConsts c, const double* t,
+                                                    const double* pstat, double* rows) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6), i = blockIdx.y;
+    if (j >= sh.jm) return;
+    const double wgt = g.sin_weight[j];
+    const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
+    double s = 0.0, ws = 0.0;
+    for (int k0 = 0; k0 < sh.km; k0 += 64) {
+        const int k = k0 + lane;
+        double term = 0.0, wt = 0.0;
+        if (k < sh.km) {
+            const double T = t[row + k];
             if (T > 0.0) {
-                const double b = c.g * rp[u] / (c.hyd_bnum * T * c.t_ref);
+                const double b = c.g * pstat[row + k] / (c.hyd_bnum * T * c.t_ref);
                 if (isfinite(b)) {
-                    term = rw[u] * b;
-                    wt = rw[u];
+                    term = wgt * b;
+                    wt = wgt;
                 }
             }
-            mine[u * 64 + lane] = (wv == 0) ? term : wt;
         }
-        __syncthreads();
-        load_group(rt, rp, rw, q0 + 3 * kG);
-        // the whole group (terms past the plane are +0.0, which leaves the sum unchanged) in
-        // batches held in two register sets: the LDS reads of one batch are in flight while the
-        // other batch adds
-        constexpr int kBatch = 8;
-        double xa[kBatch], xb[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) xa[u] = mine[u];
-#pragma unroll
-        for (int l0 = 0; l0 < kG; l0 += 2 * kBatch) {
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) xb[u] = mine[l0 + kBatch + u];
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) acc += xa[u];
-            const int l2 = (l0 + 2 * kBatch < kG) ? l0 + 2 * kBatch : 0;
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) xa[u] = mine[l2 + u];
-#pragma unroll
-            for (int u = 0; u < kBatch; u++) acc += xb[u];
-        }
-        __syncthreads();
-    };
-    load_group(tA, pA, wA, 0);
-    load_group(tB, pB, wB, kG);
-    load_group(tC, pC, wC, 2 * kG);
-    for (int q0 = 0; q0 < n; q0 += 3 * kG) {
-        step(tA, pA, wA, q0);
-        if (q0 + kG < n) step(tB, pB, wB, q0 + kG);
-        if (q0 + 2 * kG < n) step(tC, pC, wC, q0 + 2 * kG);
+        term = tree64(term);
+        wt = tree64(wt);
+        s = s + term;
+        ws = ws + wt;
     }
-    if (wv == 1 && lane == 0) wsum = acc;
-    __syncthreads();
-    if (wv == 0 && lane == 0) level[i] = (wsum > 0.0) ? acc / wsum : 0.0;
+    if (lane == 0) {
+        const size_t q = (size_t)i * sh.jm + j;
+        rows[q] = s;
+        rows[(size_t)sh.im * sh.jm + q] = ws;
+    }
+}
+
+// The level means from the row sums: one wavefront per level.
+__global__ void __launch_bounds__(64) k_level_final(Shape sh, const double* rows, double* level) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const double* rs = rows + (size_t)i * sh.jm;
+    const double* rw = rows + (size_t)sh.im * sh.jm + (size_t)i * sh.jm;
+    double ps = 0.0, pw = 0.0;
+    for (int q = lane; q < sh.jm; q += 64) {
+        ps = ps + rs[q];
+        pw = pw + rw[q];
+    }
+    ps = tree64(ps);
+    pw = tree64(pw);
+    if (lane == 0) level[i] = (pw > 0.0) ? ps / pw : 0.0;
 }
 
 // ---- the column scan: p_hydro[0] = 0, p_hydro[i] = p_hydro[i-1] + 0.5 (f(i-1) + f(i)) dr with
@@ -123,15 +96,24 @@ __global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepCons
         const double vv = v[x], ww = w[x];
         return buoy + (vv * vv + ww * ww) / g.rm[i] + cor * ww;
     };
+    // the f of kScanBatch levels at a time: their loads are independent of the running sum, so
+    // they are all in flight before the sum's dependent adds consume them
+    constexpr int kScanBatch = 8;
     double p = 0.0;
     double f_prev = f_at(0);
     phydro[(size_t)j * sh.ldk + k] = 0.0;
-    for (int i = 1; i < sh.im; i++) {
-        const double fi = f_at(i);
-        p = p + 0.5 * (f_prev + fi) * c.dr;
-        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-        phydro[x] = p;
-        f_prev = fi;
+    for (int i0 = 1; i0 < sh.im; i0 += kScanBatch) {
+        double f[kScanBatch];
+#pragma unroll
+        for (int u = 0; u < kScanBatch; u++) f[u] = (i0 + u < sh.im) ? f_at(i0 + u) : 0.0;
+#pragma unroll
+        for (int u = 0; u < kScanBatch; u++) {
+            const int i = i0 + u;
+            if (i >= sh.im) break;
+            p = p + 0.5 * (f_prev + f[u]) * c.dr;
+            phydro[((size_t)i * sh.jm + j) * sh.ldk + k] = p;
+            f_prev = f[u];
+        }
     }
 }
 
@@ -277,23 +259,25 @@ __global__ void __launch_bounds__(256) k_bc_phi(BcCtx b) {
     bc_put(b, q + sh.km - 1, b.t.y[q]);
 }
 
-// clampNegativeSpecies (planet/BC_Uran.h:110-128): every species value that is not >= 0 -> 0.
+// clampNegativeSpecies (planet/BC_Uran.h:110-128): every species value that is not >= 0 (NaN
+// included) -> 0. A flat pass over each species field's storage, blockIdx.y = species, two values
+// per thread (the padding past km holds zeros, which the test leaves alone); a value is written only
+// where it changes, to the state and to every mirror.
 __global__ void __launch_bounds__(256) k_species_clamp(BcCtx b) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, i = blockIdx.z;
-    if (k >= b.sh.km) return;
-    const size_t x = ((size_t)i * b.sh.jm + j) * b.sh.ldk + k;
-    for (int f = ATU_CH4; f <= ATU_NH4SH; f++) {
-        const size_t q = (size_t)f * b.sh.fs + x;
-        if (!(b.t.y[q] >= 0.0)) bc_put(b, q, 0.0);
-    }
+    const size_t q2 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (2 * q2 >= b.sh.fs) return;
+    const size_t q = (size_t)(ATU_CH4 + blockIdx.y) * b.sh.fs + 2 * q2;
+    const double2 v = *reinterpret_cast<const double2*>(b.t.y + q);
+    if (!(v.x >= 0.0)) bc_put(b, q, 0.0);
+    if (!(v.y >= 0.0)) bc_put(b, q + 1, 0.0);
 }
 
 }  // namespace
 
 hipError_t launch_level(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* pstat,
-                        double* level_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_level_serial, dim3(sh.im), dim3(128), 0, s, sh, g, c, t, pstat, level_out);
+                        double* rows, double* level_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_level_rows, dim3((sh.jm + 3) / 4, sh.im), dim3(256), 0, s, sh, g, c, t, pstat, rows);
+    hipLaunchKernelGGL(k_level_final, dim3(sh.im), dim3(64), 0, s, sh, rows, level_out);
     return hipGetLastError();
 }
 
@@ -306,9 +290,9 @@ hipError_t launch_hydro_scan(const Shape& sh, const DevGeo& g, const StepConsts&
 }
 
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
-                        const double* w, const double* pstat, double* level_out, double* phydro,
+                        const double* w, const double* pstat, double* rows, double* level_out, double* phydro,
                         hipStream_t s) {
-    const hipError_t e = launch_level(sh, g, c, t, pstat, level_out, s);
+    const hipError_t e = launch_level(sh, g, c, t, pstat, rows, level_out, s);
     if (e != hipSuccess) return e;
     return launch_hydro_scan(sh, g, c, t, v, w, pstat, level_out, phydro, s);
 }
@@ -346,8 +330,9 @@ hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets&
     hipLaunchKernelGGL(k_bc_radius, dim3((sh.km + 255) / 256, sh.jm, nz), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_bc_theta, dim3((sh.km + 255) / 256, sh.im, nz), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_bc_phi, dim3((sh.jm + 255) / 256, sh.im, nz), dim3(256), 0, s, b);
-    if (rest && p.species_clamp)
-        hipLaunchKernelGGL(k_species_clamp, dim3((sh.km + 255) / 256, sh.jm, sh.im), dim3(256), 0, s, b);
+    if (rest && p.species_clamp)   // fs is a multiple of ldk, itself a multiple of 32: even
+        hipLaunchKernelGGL(k_species_clamp, dim3((unsigned)((sh.fs / 2 + 255) / 256), ATU_NH4SH - ATU_CH4 + 1),
+                           dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

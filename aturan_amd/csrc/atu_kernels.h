@@ -44,27 +44,23 @@ struct StageArgs {
     // FP32 storage only: y_n rounded to FP32 (its mirror), from which stages 1-2 form the next
     // stage input (stage 0 reads it as its own stencil centre); the final update reads FP64 y_n
     const S* yn_lo;
-    // FP32 storage only: rows [jb0, jb1) are away from the poles and run in FP32 arithmetic, the
-    // two bands outside them in FP64 (one launch each)
-    int jb0, jb1;
     Shape sh;
     DevGeo geo;
     StepConsts c;
 };
 
-// Launchers (stream-ordered, asynchronous).
-// part (stage 3 only): 0 every field, 1 t only, 2 every field but t (1 then 2 is the stage).
-hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s, int part = 0);
-hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream_t s, int part = 0);
+// Launchers (stream-ordered, asynchronous): one RK stage over every interior cell.
+hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s);
+hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream_t s);
 
-// Hydrostatic prep: level means (in the reference's serial summation order) into `level_out` [im],
-// then the column scan into `phydro`.
+// Hydrostatic prep: level means (in a fixed tree order the oracle restates) into `level_out` [im],
+// then the column scan into `phydro`. `rows` holds 2 im jm doubles (the row sums).
 hipError_t launch_hydro(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t,
-                        const double* v, const double* w, const double* pstat, double* level_out,
+                        const double* v, const double* w, const double* pstat, double* rows, double* level_out,
                         double* phydro, hipStream_t s);
 // The two halves: the level means (which read t only) and the column scan (t, v, w).
 hipError_t launch_level(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* pstat,
-                        double* level_out, hipStream_t s);
+                        double* rows, double* level_out, hipStream_t s);
 hipError_t launch_hydro_scan(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* v,
                              const double* w, const double* pstat, const double* level, double* phydro,
                              hipStream_t s);

@@ -6,7 +6,13 @@ to a relative residual of 1e-8. `value` is the plain FP64 GPU step (the benchmar
 configuration); the same line carries the mixed-precision step (FP32 stencils, stage states and
 smoothing; FP64 accumulation, residual and refinement) with its per-field error against the FP64
 run after 1, 10 and 100 steps, the roofline of the dominant kernel measured with HIP events on the
-library's own stream, and the CPU oracle timed on a bounded sample of the same step.
+library's own stream, the HBM bandwidth a plain copy kernel reaches on the same device, the CPU
+oracle timed on a bounded sample of the same step with the GPU's state after the same steps compared
+against it, and the reference's CPU configuration (one RK4 step, then 20 red-black SOR sweeps at
+omega = 1.5) timed on both.
+
+The process loads no framework besides the library: the library runs on the HIP runtime it was
+built for, which the line records.
 """
 from __future__ import annotations
 
@@ -68,30 +74,63 @@ def parse():
     ap.add_argument("--refined-window", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of known size (calibrates the HBM byte counters)")
+    ap.add_argument("--config1-steps", type=int, default=50,
+                    help="GPU steps of the reference's CPU configuration (RK4 + 20 SOR sweeps, omega 1.5); 0: skip")
     return ap.parse_args()
 
 
+COPY_BYTES = 512 * 1024 * 1024               # per buffer: far beyond the 256 MB Infinity Cache
+
+
 def copy_calibration():
-    """A plain copy kernel moving a known number of bytes, for scaling FETCH_SIZE / WRITE_SIZE."""
-    import torch
-    n = 64 * 1024 * 1024                      # 512 MiB per FP64 buffer: far beyond any cache
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
-                                           "launches": 3}}), flush=True)
+    """The library's plain copy kernel (k_copy_probe) over a known number of bytes, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import aturan_amd as A
+    ms = A.core.copy_probe(COPY_BYTES, 3)
+    print(json.dumps({"copy_calibration": {"kernel": "k_copy_probe", "bytes_read_per_launch": COPY_BYTES,
+                                           "bytes_written_per_launch": COPY_BYTES, "launches": 4,
+                                           "ms_per_launch": ms}}), flush=True)
+
+
+def hbm_copy():
+    """The HBM bandwidth a plain streaming copy reaches on this device (read + write bytes / time)."""
+    import aturan_amd as A
+    nbytes = 4 * COPY_BYTES
+    ms = A.core.copy_probe(nbytes, 20)
+    return {"kernel": "k_copy_probe", "bytes_per_buffer": nbytes, "ms_per_copy": round(ms, 4),
+            "GBps": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
+TRAFFIC_FILE = "profiles/traffic_rk_stage.json"
 
 
 def traffic_from_profiles():
-    f = ROOT / "profiles" / "traffic_rk_stage.json"
+    """HBM bytes per stage launch as the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary gives them
+    (not measured in this run), with the commit the summary was made at."""
+    f = ROOT / TRAFFIC_FILE
     if f.exists():
         try:
-            return json.loads(f.read_text()).get("bytes_per_launch")
-        except Exception:
+            return json.loads(f.read_text())
+        except ValueError:
             return None
     return None
+
+
+def runtime_record():
+    import aturan_amd as A
+    from aturan_amd import _lib as L
+    built, running = A.core.runtime_versions()
+    return {"libamdhip64": L.RUNTIME_PATH, "hip_built": built, "hip_running": running,
+            "mapped_runtimes": L.mapped_runtimes()}
+
+
+def check_upload(core, F):
+    """The state read back from the device equals what was uploaded (a guard before timing)."""
+    import numpy as np
+    import aturan_amd as A
+    for n in ("t", "w", "ch4", "nue", "p_stat"):
+        if not np.array_equal(core.get_field(n), F[A.FIELD[n]]):
+            raise RuntimeError(f"bench: field {n} read back from the device differs from the upload")
 
 
 def initial_state(refine=1):
@@ -110,21 +149,17 @@ def initial_state(refine=1):
 PROFILE_CLASSES = ("rk_stage", "line", "baro", "residual", "hydro", "bc", "proj_src")
 
 
-def timed_run(core, steps, warmup, opts, torch, events=True):
+def timed_run(core, steps, warmup, opts, events=True):
     """The timed region: only the stage kernels and the whole steps are bracketed by HIP events (for
     the roofline and the stream time per step)."""
     for _ in range(warmup):
         core.step(1, **opts)
     core.synchronize()
-    if torch is not None:
-        torch.cuda.synchronize()
     if events:
         core.profile_begin("rk_stage,step")
     t0 = time.perf_counter()
     st = core.step(steps, **opts)
     core.synchronize()
-    if torch is not None:
-        torch.cuda.synchronize()
     el = time.perf_counter() - t0
     if not events:
         return el, st, (0.0, 0), 0.0
@@ -381,24 +416,31 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(tol, budget_s, warm=2):
-    """The oracle on the same workload (warm-started solves, as the GPU runs it): one warm-up step,
-    then one call of n steps sized to about budget_s; and, in a child process pinned to one OpenMP
-    thread, one step."""
-    import aturan_amd as A
+def cpu_threads() -> int:
+    """The OpenMP threads the oracle runs on (its OMP_NUM_THREADS, else the cores this process may use)."""
+    try:
+        return int(os.environ["OMP_NUM_THREADS"])
+    except (KeyError, ValueError):
+        return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(g, p, F0, tol, budget_s, warm=2):
+    """The oracle on the same workload (warm-started solves, as the GPU runs it): one warm-up step on a
+    copy of the initial state sizes the sample, then one call of n steps from the initial state (about
+    budget_s); and, in a child process pinned to one OpenMP thread, one step. Returns the record, n and
+    the oracle's state after the n steps."""
     from oracle import binding as O
-    g = A.default_grid(1)
-    p = A.default_params(g)
-    F = A.synthetic_state(g, p, 20261203)
-    O.bc(g, p, F)
+    Fw = F0.copy()
     t0 = time.perf_counter()
-    O.step_converge(g, p, F, 1, 200, tol, warm)          # warm-up
+    O.step_converge(g, p, Fw, 1, 200, tol, warm)         # warm-up, on a copy
     per = time.perf_counter() - t0
+    del Fw
     n = max(2, min(50, int(budget_s / max(per, 1e-3))))
+    F = F0.copy()
     t0 = time.perf_counter()
     O.step_converge(g, p, F, n, 200, tol, warm)
     el = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = cpu_threads()
     one = None
     try:
         env = dict(os.environ, OMP_NUM_THREADS="1")
@@ -407,11 +449,69 @@ def cpu_baseline(tol, budget_s, warm=2):
         one = json.loads(r.stdout.strip().splitlines()[-1])["steps_per_s"] if r.returncode == 0 else None
     except (subprocess.TimeoutExpired, ValueError, IndexError, KeyError):
         one = None
-    return {"value": round(n / el, 4), "unit": "steps/s", "cores": cores, "kind": "port",
-            "cpu": cpu_model(), "one_thread_steps_per_s": one,
-            "sample": f"{n} steps (hydro + RK4 + BC + warm-started Poisson to {tol:g}) of the 41x181x361 synthetic "
-                      f"state in one call, FP64 OpenMP oracle (C, -O2), after 1 warm-up step; one_thread: one "
-                      f"step after one warm-up step with OMP_NUM_THREADS=1"}
+    rec = {"value": round(n / el, 4), "unit": "steps/s", "cores": cores, "kind": "port",
+           "cpu": cpu_model(), "one_thread_steps_per_s": one,
+           "sample": f"{n} steps (hydro + RK4 + BC + warm-started Poisson to {tol:g}) of the 41x181x361 synthetic "
+                     f"state in one call from the initial state, FP64 OpenMP oracle (C, -O2), after 1 warm-up step "
+                     f"on a copy; one_thread: one step after one warm-up step with OMP_NUM_THREADS=1"}
+    return rec, n, F
+
+
+def fp64_vs_oracle(g, p, F0, Fo, n, opts):
+    """The FP64 GPU step run n times from the initial state in one call (the oracle's warm-start history
+    is that of one call) against the oracle's state after its n steps: per-field max |x - x_o| / max |x_o|."""
+    import numpy as np
+    import aturan_amd as A
+    with A.UranusCore(g, p, "fp64") as core:
+        core.set_state(F0)
+        core.step(n, **opts)
+        out = {}
+        for name in ("t", "u", "v", "w", "ch4", "p_dyn", "p_hydro", "tke", "dis", "nh4sh"):
+            a, b = core.get_field(name), Fo[A.FIELD[name]]
+            s = float(np.max(np.abs(b)))
+            out[name] = float(np.max(np.abs(a - b))) / (s if s > 0 else 1.0)
+    return {"steps": n, "max_rel": out, "worst": max(out.values()), "bar": 1e-12}
+
+
+def config1(g, p, F0, gpu_steps, cpu_steps=3):
+    """The reference's CPU configuration: one RK4 step (hydro prep, four stages, BC and species clamp),
+    then 20 red-black SOR sweeps at omega = 1.5. GPU: gpu_steps such steps in one call after 3 warm-up
+    steps; CPU: the oracle, cpu_steps steps (one call) after one warm-up step on a copy; the GPU's
+    state after cpu_steps steps from the initial state against the oracle's."""
+    import numpy as np
+    import aturan_amd as A
+    from oracle import binding as O
+    opts = dict(mode="reference", sweeps=20, omega=1.5)
+    n_int = (g.im - 2) * (g.jm - 2) * (g.km - 1)
+    with A.UranusCore(g, p, "fp64") as core:
+        core.set_state(F0)
+        core.step(cpu_steps, **opts)
+        G = {n: core.get_field(n) for n in ("t", "u", "v", "w", "ch4", "p_dyn")}
+        core.step(3, **opts)
+        core.synchronize()
+        t0 = time.perf_counter()
+        core.step(gpu_steps, **opts)
+        core.synchronize()
+        el = time.perf_counter() - t0
+    Fw = F0.copy()
+    O.step(g, p, Fw, 1, 20, 1.5)
+    del Fw
+    F = F0.copy()
+    t0 = time.perf_counter()
+    O.step(g, p, F, cpu_steps, 20, 1.5)
+    cel = time.perf_counter() - t0
+    err = {}
+    for n, a in G.items():
+        b = F[A.FIELD[n]]
+        s = float(np.max(np.abs(b)))
+        err[n] = float(np.max(np.abs(a - b))) / (s if s > 0 else 1.0)
+    return {"workload": "41x181x361: hydro prep + RK4 + BC + species clamp + 20 red-black SOR sweeps, omega 1.5",
+            "gpu": {"steps": gpu_steps, "steps_per_s": round(gpu_steps / el, 2),
+                    "ms_per_step": round(1e3 * el / gpu_steps, 4),
+                    "cell_updates_per_s": round(n_int * gpu_steps / el, 1)},
+            "cpu": {"steps": cpu_steps, "steps_per_s": round(cpu_steps / cel, 4), "cores": cpu_threads(),
+                    "kind": "port", "cpu": cpu_model()},
+            "gpu_vs_oracle_after_cpu_steps": {"max_rel": err, "worst": max(err.values())}}
 
 
 def cpu_one_thread(tol):
@@ -421,10 +521,14 @@ def cpu_one_thread(tol):
     p = A.default_params(g)
     F = A.synthetic_state(g, p, 20261203)
     O.bc(g, p, F)
-    O.step_converge(g, p, F, 1, 200, tol, True)
+    O.step_converge(g, p, F, 1, 200, tol, 2)
     t0 = time.perf_counter()
-    O.step_converge(g, p, F, 1, 200, tol, True)
+    O.step_converge(g, p, F, 1, 200, tol, 2)
     print(json.dumps({"steps_per_s": round(1.0 / (time.perf_counter() - t0), 4)}), flush=True)
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -439,11 +543,6 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    # under a profiler (--no-events) the run leaves torch out: rocprofv3's kernel tracing with torch's
-    # runtime initialised in the same process has given wrong results from this library's kernels
-    torch = None
-    if not args.no_events:
-        import torch
     import aturan_amd as A
 
     if args.refined:
@@ -457,34 +556,46 @@ def main():
             "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic"}
-    cfg = {"workload": f"41x181x361 shell, seeded synthetic state: hydro prep + RK4 + BC + red-black line/barotropic "
-                       f"Poisson solve to relative residual {args.tol:g}, {args.steps} steps",
+    cfg = {"workload": f"41x181x361 shell, seeded synthetic state: hydro prep + RK4 + BC + species clamp + red-black "
+                       f"line/barotropic Poisson solve to relative residual {args.tol:g}, {args.steps} steps",
            "solve_start": SOLVE_START[warm]}
+    line["runtime"] = runtime_record()
+    copy = hbm_copy()
+    progress(f"copy kernel {copy['GBps']} GB/s")
     if args.mode in ("both", "fp64"):
         with A.UranusCore(g, p, "fp64") as core:
             core.set_state(F)
-            el, st, (rk_ms, rk_n), gpu_ms = timed_run(core, args.steps, args.warmup, opts64, torch, not args.no_events)
+            check_upload(core, F)
+            el, st, (rk_ms, rk_n), gpu_ms = timed_run(core, args.steps, args.warmup, opts64, not args.no_events)
             breakdown = None if args.no_events else kernel_breakdown(core, min(args.steps, 20), opts64)
         sps = args.steps / el
+        progress(f"fp64 {sps:.1f} steps/s")
         # every timed step runs the four stages once (the first and the last as two launches each)
         achieved = sum(RK_PASSES_FP64) * 8 * N_INT * args.steps / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
         line.update(value=round(sps, 3), ms_per_step=round(1e3 * el / args.steps, 4))
         cfg.update(cell_updates_per_s=round(sps * N_INT, 1), solve_iterations_last_step=st.sweeps_done,
                    solve_rel_resid_last_step=st.rel_resid)
+        tr = traffic_from_profiles()
         line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_from_profiles(),
+                            "frac": round(achieved / PEAK_HBM_GBS, 4),
+                            "traffic": tr.get("bytes_per_launch") if tr else None,
+                            "traffic_source": (f"{TRAFFIC_FILE}: rocprofv3 FETCH_SIZE and WRITE_SIZE of "
+                                               f"{tr.get('kernel')}, scaled by the copy calibration; committed "
+                                               f"summary, not measured in this run") if tr else None,
                             "kernel": "k_rk_stage<double> (fused RHS + RK fold)",
                             "avg_stage_ms": round(rk_ms / (4 * args.steps), 5), "launches": rk_n,
-                            "bytes_per_stage": sum(RK_PASSES_FP64) * 8 * N_INT / 4}
+                            "bytes_per_stage": sum(RK_PASSES_FP64) * 8 * N_INT / 4,
+                            "copy_GBps": copy["GBps"], "frac_of_copy": round(achieved / copy["GBps"], 4)}
         line["kernels_ms_per_step"] = breakdown
         line["gpu_ms_per_step"] = round(gpu_ms, 4)   # stream time of each step (events), gaps included
     if args.mode in ("both", "mixed"):
         with A.UranusCore(g, p, "mixed") as core:
             core.set_state(F)
-            el, st, (rk_ms, rk_n), gpu_ms_mx = timed_run(core, args.steps, args.warmup, optsmx, torch,
-                                                        not args.no_events)
+            check_upload(core, F)
+            el, st, (rk_ms, rk_n), gpu_ms_mx = timed_run(core, args.steps, args.warmup, optsmx, not args.no_events)
             breakdown = None if args.no_events else kernel_breakdown(core, min(args.steps, 20), optsmx)
         sps = args.steps / el
+        progress(f"mixed {sps:.1f} steps/s")
         achieved = sum(RK_BYTES_MIXED) * N_INT * args.steps / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
         mixed = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 * el / args.steps, 4),
                  "speedup_vs_fp64": round(sps / line["value"], 4) if line["value"] else None,
@@ -497,17 +608,29 @@ def main():
                         metric="RK4 steps/s at 41x181x361 (mixed FP32/FP64 GPU step, Poisson to 1e-8)")
         if not args.no_errors and args.mode == "both":
             mixed["rel_error_vs_fp64"] = mixed_errors(g, p, F, opts64, optsmx)
+            progress("mixed vs fp64 errors done")
         line["mixed"] = mixed
     if args.adjoint_steps > 0 and args.mode in ("both", "fp64"):
         line["adjoint"] = adjoint_bench(g, p, F, args.adjoint_steps, args.checkpoint_every, args.tol,
                                         line["ms_per_step"] / 1e3 if line["ms_per_step"] else None, not args.no_events)
+        progress("adjoint done")
     if args.turb_calls > 0 and args.mode in ("both", "fp64"):
         line["turbulence"] = turb_bench(g, p, F, args.turb_calls, not args.no_events)
     if args.satadj_calls > 0 and args.mode in ("both", "fp64"):
         line["saturation_adjustment"] = satadj_bench(g, p, F, args.satadj_calls, not args.no_events)
+        progress("closure and saturation adjustment done")
     line["config"] = cfg
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.tol, args.cpu_seconds, warm)
+        rec, n, Fo = cpu_baseline(g, p, F, args.tol, args.cpu_seconds, warm)
+        line["cpu_baseline"] = rec
+        progress(f"cpu baseline {rec['value']} steps/s over {n} steps")
+        if args.mode in ("both", "fp64"):
+            line["fp64_vs_oracle"] = fp64_vs_oracle(g, p, F, Fo, n, opts64)
+            progress(f"fp64 vs oracle after {n} steps: worst {line['fp64_vs_oracle']['worst']:.3e}")
+        del Fo
+        if args.config1_steps > 0:
+            line["config1"] = config1(g, p, F, args.config1_steps)
+            progress("config 1 done")
     print(json.dumps(line), flush=True)
 
 

@@ -103,7 +103,9 @@ typedef struct atu_params {
     int32_t lid_vw;                            /* ATURAN_BC_LID_VW: 0 Neumann, 2 linear, 3 cubic */
     int32_t rigid_lid;                         /* u = 0 on both radial planes                 */
     int32_t radius_positive;                   /* ATURAN_BC_RADIUS_POSITIVE: 0 off, 1 floor at 0 */
-    int32_t species_clamp;                     /* ATURAN_SPECIES_CLAMP after the BC (0 = off)  */
+    int32_t species_clamp;                     /* ATURAN_SPECIES_CLAMP after the BC: 1 (the default,
+                                                  planet/BC_Uran.h:110-128) zeroes every species value
+                                                  that is not >= 0 (NaN included); 0 off            */
     int32_t pad_;
 } atu_params;
 
@@ -207,6 +209,13 @@ int atu_synchronize(atu_ctx* ctx);
  * the reference (planet/cUranusModel.cpp:1638-1685). */
 int atu_set_field(atu_ctx* ctx, int field, const double* host);
 int atu_get_field(atu_ctx* ctx, int field, double* host);
+
+/* The same from and to a dense device array of the context's device, rows of ld_k >= km FP64
+ * elements ([im][jm][ld_k]): a caller's tensor in HBM, no host copy. Both return once the copy has
+ * run on the context's stream; the caller makes sure its own writes of `dsrc` are complete before the
+ * call (no counterpart in the reference: its arrays live on the host). */
+int atu_set_field_device(atu_ctx* ctx, int field, const double* dsrc, int ld_k);
+int atu_get_field_device(atu_ctx* ctx, int field, double* ddst, int ld_k);
 
 /* ---- the step pieces ---- */
 
@@ -331,8 +340,23 @@ int atu_saturation_adjust(atu_ctx* ctx, const atu_satadj_opts* opts);
  * elements, `elem_bytes` 8 or 4. The layout is [im][jm][ld_k]. */
 int atu_field_device_ptr(atu_ctx* ctx, int field, void** dptr, int* ld_k, int* elem_bytes);
 
-/* Library build information string (arch, flags). */
+/* Library build information string (arch, flags, the HIP version it was built with). */
 const char* atu_build_info(void);
+
+/* HIP_VERSION the library was built with and hipRuntimeGetVersion of the runtime it runs on. The two
+ * differ when another copy of libamdhip64 was loaded into the process first (a framework's bundled
+ * runtime shares the soname); atu_create then refuses with ATU_EUNSUPPORTED. */
+int atu_runtime_versions(int* built, int* running);
+
+/* hipMemGetInfo of the current device: free and total bytes (for reporting the memory a context and
+ * its window hold; no counterpart in the reference). */
+int atu_device_memory(size_t* free_bytes, size_t* total_bytes);
+
+/* A device-to-device copy of `bytes` (a multiple of 16) between two buffers it allocates, `launches`
+ * times after one warm-up launch; *ms_per_launch receives the mean duration (HIP events). The
+ * benchmark's measure of the HBM bandwidth a streaming kernel reaches on this device, and the
+ * calibration of the hardware byte counters (kernel k_copy_probe). No counterpart in the reference. */
+int atu_copy_probe(size_t bytes, int launches, double* ms_per_launch);
 
 /* ---- gradients over a window of steps (no counterpart in the reference) ----
  * The loss of a window of `nsteps` outer steps (atu_step's order, FP64 context, converging solve)
@@ -360,8 +384,18 @@ typedef struct atu_window_opts {
 } atu_window_opts;
 
 int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss);
-/* grad_t0: dense [im][jm][km] host array or NULL; grad_nue: scalar or NULL. */
+/* grad_t0: dense [im][jm][km] host array or NULL; grad_nue: scalar or NULL. The backward differentiates
+ * the last atu_window_forward; it returns ATU_EINVAL when a frozen input of the window (fields 19..39)
+ * or p_stat was written since (atu_set_field, atu_set_field_device, the turbulence closure), because
+ * its recomputation would then read other inputs than the forward did. A caller that writes those
+ * fields through atu_field_device_ptr runs the forward again itself. */
 int atu_window_backward(atu_ctx* ctx, double dloss, double* grad_t0, double* grad_nue);
+/* The same with grad_t0 a dense device array of rows of ld_k >= km elements. */
+int atu_window_backward_device(atu_ctx* ctx, double dloss, double* grad_t0, int ld_k, double* grad_nue);
+/* A number identifying the forward the next atu_window_backward would differentiate: it changes with
+ * every atu_window_forward and is 0 when none is held or its inputs were written since (a binding
+ * stores it with its forward and checks it before its backward). */
+int atu_window_generation(const atu_ctx* ctx, uint64_t* generation);
 /* Put the state (prognostic fields and p_dyn) back to the start of the last atu_window_forward,
  * from its first checkpoint on the device (for repeated evaluations such as finite differences:
  * rewind, change a field with atu_set_field, run the window again). */

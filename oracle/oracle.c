@@ -33,6 +33,7 @@
 
 static double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
 static double dclamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+static void halving_tree(double* t, int w);
 
 /* ------------------------------------------------------------------------------------------- */
 /* cp of H2 and cp_lapse                                                                        */
@@ -447,21 +448,54 @@ int orc_hydro(const atu_grid* g, const atu_params* p, double* fields, double* le
     const double* ps = fields + (size_t)ATU_P_STAT * n3;
     double* ph = fields + (size_t)ATU_P_HYDRO * n3;
     double* ref = malloc(sizeof(double) * im);
+    /* The level means (:18-37) with each sum in the fixed order the GPU uses (its k_level_tree),
+     * instead of the reference's serial (j, k) order: per (i, j) row the km terms in chunks of 64
+     * consecutive k, each chunk reduced by the halving tree (a k past the row, or a skipped cell --
+     * T <= 0 or a non-finite b, which the reference skips -- is +0.0), the chunk results added in
+     * sequence to 0.0; the jm row sums dealt to 64 partials (partial t adds rows t, t + 64, ... in
+     * sequence to 0.0) reduced by the halving tree; the weights the same way. */
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < im; i++) {
-        double sum = 0.0, wsum = 0.0;
+        double* rs = malloc(sizeof(double) * 2 * jm);
         for (int j = 0; j < jm; j++) {
             const double wgt = sin(G.thez[j]);
-            for (int k = 0; k < km; k++) {
-                const double T = t[IDX(i, j, k)];
-                if (!(T > 0.0)) continue;
-                const double b = p->g * ps[IDX(i, j, k)] / (p->r_mix * p->R_mix * T * p->t_ref);
-                if (!isfinite(b)) continue;
-                sum += wgt * b;
-                wsum += wgt;
+            double s = 0.0, ws = 0.0;
+            for (int k0 = 0; k0 < km; k0 += 64) {
+                double tv[64], wv[64];
+                for (int l = 0; l < 64; l++) {
+                    const int k = k0 + l;
+                    tv[l] = 0.0;
+                    wv[l] = 0.0;
+                    if (k >= km) continue;
+                    const double T = t[IDX(i, j, k)];
+                    if (!(T > 0.0)) continue;
+                    const double b = p->g * ps[IDX(i, j, k)] / (p->r_mix * p->R_mix * T * p->t_ref);
+                    if (!isfinite(b)) continue;
+                    tv[l] = wgt * b;
+                    wv[l] = wgt;
+                }
+                halving_tree(tv, 64);
+                halving_tree(wv, 64);
+                s = s + tv[0];
+                ws = ws + wv[0];
             }
+            rs[j] = s;
+            rs[jm + j] = ws;
         }
-        ref[i] = (wsum > 0.0) ? sum / wsum : 0.0;
+        double pt[64], pw[64];
+        for (int l = 0; l < 64; l++) {
+            double a = 0.0, b = 0.0;
+            for (int q = l; q < jm; q += 64) {
+                a = a + rs[q];
+                b = b + rs[jm + q];
+            }
+            pt[l] = a;
+            pw[l] = b;
+        }
+        free(rs);
+        halving_tree(pt, 64);
+        halving_tree(pw, 64);
+        ref[i] = (pw[0] > 0.0) ? pt[0] / pw[0] : 0.0;
     }
     const double nd_rel = p->g * (p->L_atm * 1.0e3) / (p->u_0 * p->u_0);
     const double dr = g->dr;

@@ -5,11 +5,13 @@
 //
 // usage: ref_harness <in.bin> <out.bin> <mode> [steps] [sweeps]
 //   mode bc       BC_Uran bcRadius, bcTheta, bcPhi                       (planet/BC_Uran.h:140-173)
+//   mode bc_clamp the same, then clampNegativeSpecies, as the loop runs them (planet/cUranusModel.cpp:
+//                 1243-1248; planet/BC_Uran.h:110-128, ATURAN_SPECIES_CLAMP at its default 1)
 //   mode rk4      restoreVar(1) + RungeKuttaUran (hydro prep + 4 stages) (planet/RungeKutta_Uran_Turb.cpp)
 //   mode pressure PressureSolver<cUranusModel>::run with ATURAN_PRESS_SWEEPS=sweeps
-//   mode step     `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolver)
+//   mode step     `steps` x (restoreVar, RungeKuttaUran, BC, clampNegativeSpecies, PressureSolver)
 //   mode rk4_uran restoreVar(1) + RungeKuttaUran + PressureSolverUran::run (the default solver)
-//   mode step_uran `steps` x (restoreVar, RungeKuttaUran, BC, PressureSolverUran)
+//   mode step_uran `steps` x (restoreVar, RungeKuttaUran, BC, clampNegativeSpecies, PressureSolverUran)
 //   mode damp_t   AtomUtils::damp_wiggles on t: along j and k (strength 1), then along i (0.5)
 //   mode damp_uvw AtomUtils::damp_wiggles on u, v, w along i, j and k, strength 1, `sweeps` passes
 //   mode turb_run  TurbulenceUran(m).run()   (planet/Turbulence.h:306-332; knobs from the environment)
@@ -153,6 +155,9 @@ int main(int argc, char** argv) {
     AtomUtils::set_phi_seam_periodic(ATPhys::phi_periodic<cUranusModel>());
     if (mode == "bc") {
         bc();
+    } else if (mode == "bc_clamp") {
+        bc();
+        m.clampNegativeSpecies();
     } else if (mode == "rk4") {
         m.restoreVar(1.0);
         m.RungeKuttaUran();
@@ -163,6 +168,7 @@ int main(int argc, char** argv) {
             m.restoreVar(1.0);
             m.RungeKuttaUran();
             bc();
+            m.clampNegativeSpecies();
             PressureSolver<cUranusModel>(m).run();
         }
     } else if (mode == "rk4_uran") {
@@ -174,6 +180,7 @@ int main(int argc, char** argv) {
             m.restoreVar(1.0);
             m.RungeKuttaUran();
             bc();
+            m.clampNegativeSpecies();
             PressureSolverUran(m).run();
         }
     } else if (mode == "damp_t") {

@@ -36,8 +36,12 @@ SAMPLE_I = [0, 1, 2, 20, 30, 38, 39, 40]
 SAMPLE_J = [0, 1, 2, 23, 24, 90, 156, 157, 178, 179, 180]
 SAMPLE_K = [0, 1, 2, 180, 358, 359, 360]
 CASES = {
-    # name: (harness mode, steps, sweeps, apply BC to the input first)
+    # name: (harness mode, steps, sweeps, apply BC (and the species clamp) to the input first)
+    # the BC alone (params.species_clamp = 0): its radial and polar extrapolations leave negative
+    # species values in the corner rows, which the loop's clamp then zeroes
     "bc": ("bc", 1, 1, False),
+    # the BC then clampNegativeSpecies on an input with negative and NaN species values
+    "bc_clamp": ("bc_clamp", 1, 1, False),
     "rk4": ("rk4", 1, 1, True),
     "pressure_1": ("pressure", 1, 1, True),
     "pressure_20": ("pressure", 1, 20, True),
@@ -214,13 +218,26 @@ def satadj_mix(grid, params, F: np.ndarray, seed: int = SEED) -> None:
     F[Fi["nh4sh"]] *= hash_uniform(79, shp, seed) < 0.5
 
 
+def species_negatives(F: np.ndarray, seed: int = SEED) -> None:
+    """The clamp input: in each of the 13 species fields about 8 % of the cells negative (-1e-4 to 0)
+    and 2 % NaN, spread over the interior and every boundary plane (integer hashing only)."""
+    Fi = A.FIELD
+    shp = F.shape[1:]
+    for f in range(Fi["ch4"], Fi["nh4sh"] + 1):
+        h = hash_uniform(90 + f, shp, seed)
+        F[f] = np.where(h < 0.08, -1e-4 * hash_uniform(120 + f, shp, seed), F[f])
+        F[f] = np.where((h >= 0.08) & (h < 0.10), np.nan, F[f])
+
+
 def case_input(name: str, grid, params) -> np.ndarray:
     from oracle import binding as O
     F = enriched_state(grid, params)
     if name == "satadj_mix":
         satadj_mix(grid, params, F)
+    if name == "bc_clamp":
+        species_negatives(F)
     if CASES[name][3]:
-        O.bc(grid, params, F)
+        O.bc(grid, params, F)       # with the species clamp of the default parameters
     return F
 
 

@@ -69,6 +69,8 @@ def test_params_mixture_and_anchor():
     assert 180.0 < T < 232.0
     assert p.coriolis_scale == 2250.0
     assert p.turb_active == 1 and p.lid_vw == 2 and p.rigid_lid == 1
+    # clampNegativeSpecies after the BC is on by default (planet/BC_Uran.h:110-111)
+    assert p.species_clamp == 1 and p.radius_positive == 1
 
 
 def test_cp_table_known_values(oracle):
@@ -145,3 +147,24 @@ def test_create_rejects_bad_grid():
     assert b"invalid grid" in A.load().atu_last_error(None)
     with pytest.raises(A.AtuError):
         A.UranusCore(g, p)
+
+
+def _mapped_after(code: str):
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       cwd=str(ROOT))
+    return r.returncode, r.stdout, r.stderr
+
+
+def test_one_hip_runtime_per_process():
+    """Loaded before torch, the library maps the system HIP runtime and torch binds to the same copy;
+    torch imported first (its wheel bundles its own libamdhip64 under the same soname) makes load()
+    refuse instead of running the library on that copy."""
+    rc, out, err = _mapped_after("import aturan_amd as A; from aturan_amd import _lib as L; A.load(); import torch; "
+                                 "print(L.mapped_runtimes())")
+    assert rc == 0, err
+    paths = eval(out.strip().splitlines()[-1])
+    assert len(paths) == 1 and "torch" not in paths[0], paths
+    rc, out, err = _mapped_after("import torch; import aturan_amd as A; A.load()")
+    assert rc != 0 and "Import aturan_amd before" in err

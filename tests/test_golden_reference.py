@@ -21,6 +21,14 @@ sys.path.insert(0, str(GOLDEN))
 import make_golden as MG  # noqa: E402
 
 TOL = 1e-12
+# p_hydro integrates the buoyancy anomaly (b - ref) / ref, which magnifies a rounding difference of the
+# level mean ref by nd_rel / |anomaly| (~1e4 on these states). The level sums here run in a fixed tree
+# order (closer to the exact sum than the reference's serial order, test_level_means_track_exact_sum),
+# so p_hydro differs from the reference's by that magnified rounding: ~1e-11 of its largest value, and
+# its sum, which cancels to ~1/300 of its magnitude, by ~1e-9 relative. Its bar is therefore 1e-10 of
+# its largest sample, and the sums 1e-10 of sqrt(N * sum of squares); every other field keeps 1e-12.
+PH_TOL = 1e-10
+PH = A.FIELD["p_hydro"]
 
 
 def run_oracle(oracle, name, g, p):
@@ -34,8 +42,10 @@ def run_oracle(oracle, name, g, p):
         T3, vs = MG.turb_input(g)
         oracle.turb(g, p, F, T3[0], T3[1], T3[2], vs, init=(mode == "turb_init"), **MG.TURB_KNOBS[name])
         return F, level, (T3, vs)
-    if mode == "bc":
-        oracle.bc(g, p, F)
+    if mode in ("bc", "bc_clamp"):
+        pb = A.default_params(g)
+        pb.species_clamp = int(mode == "bc_clamp")      # "bc" is the BC alone
+        oracle.bc(g, pb, F)
     elif mode == "rk4":
         level = oracle.hydro(g, p, F)
         oracle.rk4(g, p, F)
@@ -79,14 +89,18 @@ def test_oracle_matches_compiled_reference(oracle, name):
         for key in ("turb_samples", "turb_vs_samples", "turb_sums", "turb_sumsq"):
             assert np.array_equal(t[key], ref[key]), (name, key)
     worst = {}
+    n = F[0].size
     for f, fname in enumerate(A.FIELD_NAMES):
         a, b = got["samples"][f], ref["samples"][f]
         scale = float(np.max(np.abs(b))) or 1.0
         worst[fname] = float(np.max(np.abs(a - b))) / scale
         for key in ("sums", "sumsq"):
             x, y = got[key][f], ref[key][f]
-            assert abs(x - y) <= TOL * max(abs(y), 1e-300) or x == y, (name, fname, key, x, y)
-    bad = {k: v for k, v in worst.items() if v > TOL}
+            bar = TOL * max(abs(y), 1e-300)
+            if f == PH:
+                bar = PH_TOL * (math.sqrt(n * ref["sumsq"][f]) if key == "sums" else abs(y))
+            assert abs(x - y) <= bar or x == y, (name, fname, key, x, y)
+    bad = {k: v for k, v in worst.items() if v > (PH_TOL if k == "p_hydro" else TOL)}
     print(f"[golden] {name}: worst sampled difference {max(worst.values()):.3e}")
     assert not bad, bad
     if MG.CASES[name][0] in ("rk4", "step", "rk4_uran", "step_uran"):
@@ -140,3 +154,23 @@ def test_exact_sum_matches_fsum():
         assert MG.exact_sum(a) == math.fsum(a.tolist())
         if a.size and np.max(np.abs(a)) < 1e150:   # squares stay finite
             assert MG.exact_sum(np.square(a)) == math.fsum(np.square(a).tolist())
+
+
+def test_level_means_track_exact_sum(oracle):
+    """The level means' fixed tree order against the correctly rounded sums (math.fsum of the same
+    terms): within a few units of the last place, where the reference's serial order drifts further
+    (its fixture's level means are the serial ones)."""
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F = MG.case_input("rk4", g, p)
+    lev = oracle.hydro(g, p, F.copy())
+    t, ps = F[A.FIELD["t"]], F[A.FIELD["p_stat"]]
+    wgt = np.sin(np.cumsum(np.r_[0.0, np.full(g.jm - 1, g.dthe)]))[None, :, None] * np.ones_like(t)
+    b = p.g * ps / (p.r_mix * p.R_mix * t * p.t_ref)
+    ok = (t > 0) & np.isfinite(b)
+    exact = np.array([MG.exact_sum((wgt[i] * b[i])[ok[i]]) / MG.exact_sum(wgt[i][ok[i]]) for i in range(g.im)])
+    serial = np.load(GOLDEN / "ref_rk4.npz")["level"]
+    e_tree = np.max(np.abs(lev - exact) / np.abs(exact))
+    e_serial = np.max(np.abs(serial - exact) / np.abs(exact))
+    print(f"[level] max relative error vs the exact mean: tree {e_tree:.2e}, reference serial {e_serial:.2e}")
+    assert e_tree <= 1e-14 and e_tree < e_serial

@@ -5,7 +5,10 @@ window. Gradients: with respect to the initial temperature field (checked along 
 directions d: dL/dT0 . d against (L(T0 + eps d) - L(T0 - eps d)) / (2 eps), eps = 1e-6) and the
 uniform eddy viscosity nue (relative step 1e-6). The solves run a fixed iteration count, so the
 forward is a smooth map; the adjoint differentiates the solve as exact, which the fixed count
-reaches to the solver's rounding floor. Bar: relative error <= 1e-4.
+reaches to the solver's rounding floor. Bar: |FD - AD| <= 1e-4 of the scale of the directional
+derivative, max(|AD|, ||dL/dT0||_2): for d drawn from N(0, I), dL/dT0 . d has standard deviation
+||dL/dT0||_2, and a direction drawn nearly orthogonal to the gradient (|AD| far below that) would
+otherwise measure the central difference's rounding floor (~1e-16 L / eps), not the gradient.
 """
 import time
 
@@ -46,13 +49,15 @@ def _fd_check(g, p, F, opts, ndir, seed):
             return core.window_forward(**opts)
 
         worst = 0.0
+        gnorm = float(np.linalg.norm(gT))
         for q in range(ndir):
             d = rng.standard_normal(F[T].shape)
             fd = (loss(F[T] + EPS * d, F[nue]) - loss(F[T] - EPS * d, F[nue])) / (2 * EPS)
             ad = float(np.sum(gT * d))
-            e = abs(fd - ad) / max(abs(ad), 1e-300)
+            e = abs(fd - ad) / max(abs(ad), gnorm, 1e-300)
             worst = max(worst, e)
-            print(f"[adjoint] dir {q}: AD {ad:+.12e} FD {fd:+.12e} rel {e:.2e}", flush=True)
+            print(f"[adjoint] dir {q}: AD {ad:+.12e} FD {fd:+.12e} rel {abs(fd - ad) / max(abs(ad), 1e-300):.2e} "
+                  f"(of the scale {max(abs(ad), gnorm):.3e}: {e:.2e})", flush=True)
         nu = float(F[nue].flat[0])
         h = EPS * nu
         fdn = (loss(F[T], F[nue] + h) - loss(F[T], F[nue] - h)) / (2 * h)

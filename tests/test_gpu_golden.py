@@ -18,6 +18,11 @@ sys.path.insert(0, str(GOLDEN))
 import make_golden as MG  # noqa: E402
 
 TOL = 1e-12
+# p_hydro magnifies the rounding of the level means, which sum in a fixed tree order here and in the
+# reference's serial order there (tests/test_golden_reference.py gives the derivation): 1e-10 of its
+# largest sample, its sums 1e-10 of sqrt(N * sum of squares).
+PH_TOL = 1e-10
+PH = A.FIELD["p_hydro"]
 
 
 @pytest.mark.parametrize("name", list(MG.CASES))
@@ -30,7 +35,10 @@ def test_gpu_matches_compiled_reference(oracle, name):
     mode, steps, sweeps, _ = MG.CASES[name]
     level = np.zeros(g.im)
     turb = None
-    with A.UranusCore(g, p) as core:
+    pc = A.default_params(g)
+    if mode == "bc":
+        pc.species_clamp = 0                 # the BC alone; "bc_clamp" runs the default clamp after it
+    with A.UranusCore(g, pc) as core:
         core.set_state(F)
         if mode == "satadj":
             core.SaturationAdjustment()
@@ -44,7 +52,7 @@ def test_gpu_matches_compiled_reference(oracle, name):
             else:
                 core.Turbulence_run(**MG.TURB_KNOBS[name])
             turb = (np.stack([core.turb_array(n) for n in ("prod", "tken", "disn")]), core.turb_array("vel_star"))
-        elif mode == "bc":
+        elif mode in ("bc", "bc_clamp"):
             core.boundary_conditions()
         elif mode == "rk4":
             level = core.computeHydrostaticPressure()
@@ -70,17 +78,24 @@ def test_gpu_matches_compiled_reference(oracle, name):
         G = core.get_state()
     got = MG.summarize(G, level)
     worst = 0.0
+    n = G[0].size
     for f, fname in enumerate(A.FIELD_NAMES):
         a, b = got["samples"][f], ref["samples"][f]
         scale = float(np.max(np.abs(b))) or 1.0
         e = float(np.max(np.abs(a - b))) / scale
-        worst = max(worst, e)
-        assert e <= TOL, (name, fname, e)
+        if f != PH:
+            worst = max(worst, e)
+        assert e <= (PH_TOL if f == PH else TOL), (name, fname, e)
         for key in ("sums", "sumsq"):
             x, y = got[key][f], ref[key][f]
-            assert x == y or abs(x - y) <= TOL * abs(y), (name, fname, key, x, y)
+            bar = TOL * abs(y)
+            if f == PH:
+                bar = PH_TOL * ((n * ref["sumsq"][f]) ** 0.5 if key == "sums" else abs(y))
+            assert x == y or abs(x - y) <= bar, (name, fname, key, x, y)
     if mode in ("rk4", "rk4_uran"):
-        assert np.array_equal(got["level"], ref["level"])
+        # the level sums run in a fixed parallel order (the oracle's too), not the reference's serial
+        # (j, k) order: equal to the reference to rounding
+        assert np.max(np.abs(got["level"] - ref["level"])) <= TOL * np.max(np.abs(ref["level"]))
     if turb is not None:
         t = MG.summarize_turb(*turb)
         for q in range(3):

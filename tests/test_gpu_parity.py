@@ -3,7 +3,8 @@
 FP64 bar: every prognostic field, p_dyn, p_hydro and aux within 1e-12 relative
 (max |x - x_ref| / max |x_ref|) of the oracle on the same seeded input. The stencil, fold, BC and
 red-black kernels keep the reference's operand order with FMA contraction off, so most of these
-comparisons are exact; the level means of the hydrostatic prep sum in a different (fixed) order.
+comparisons are exact; the level means of the hydrostatic prep sum in a fixed parallel order that the
+oracle restates, so they are exact too.
 """
 import numpy as np
 import pytest
@@ -70,7 +71,7 @@ def test_hydro_matches_oracle(setup, oracle):
     e_lev = rel_err(lev_gpu, lev)
     e_ph = rel_err(ph_gpu, F[A.FIELD["p_hydro"]])
     print(f"[parity] hydro: level means {e_lev:.3e}, p_hydro {e_ph:.3e}", flush=True)
-    # the level sums run in the reference's serial order, so both are exact
+    # the level sums run in one fixed parallel order shared with the oracle, so both are exact
     assert e_lev == 0.0
     assert e_ph == 0.0
 
