@@ -42,6 +42,7 @@ def load() -> C.CDLL:
         "orc_hydro": (C.c_int, [V, V, _DP, _DP]),
         "orc_bc": (C.c_int, [V, V, _DP]),
         "orc_pressure": (C.c_int, [V, V, _DP, C.c_int, C.c_double, _DP]),
+        "orc_pressure_flags": (C.c_int, [V, V, _DP, C.c_int, C.c_double, C.c_int, _DP]),
         "orc_residual": (C.c_int, [V, V, _DP, _DP, _DP, _DP]),
         "orc_pressure_converge": (C.c_int, [V, V, _DP, C.c_int, C.c_double, C.POINTER(C.c_int),
                                             C.POINTER(C.c_double), _DP]),
@@ -120,9 +121,12 @@ def bc(grid, params, fields: np.ndarray) -> None:
     _ok(load().orc_bc(_p(grid), _p(params), _d(fields)), "orc_bc")
 
 
-def pressure(grid, params, fields: np.ndarray, sweeps: int, omega: float) -> np.ndarray:
+def pressure(grid, params, fields: np.ndarray, sweeps: int, omega: float, flags: int = 0) -> np.ndarray:
+    """The red-black PressureSolver; flags 0x100 (PRESS_LINE) and 0x200 (PRESS_BAROTROPIC) add the
+    reference's radial line pass and barotropic correction to every sweep."""
     src = np.zeros(fields.shape[1:])
-    _ok(load().orc_pressure(_p(grid), _p(params), _d(fields), sweeps, omega, _d(src)), "orc_pressure")
+    _ok(load().orc_pressure_flags(_p(grid), _p(params), _d(fields), sweeps, omega, flags, _d(src)),
+        "orc_pressure_flags")
     return src
 
 

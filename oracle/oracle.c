@@ -34,6 +34,8 @@
 static double dmax(double a, double b) { return (a < b) ? b : a; }   /* std::max */
 static double dclamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
 static void halving_tree(double* t, int w);
+#define ORC_PRESS_LINE 0x100
+#define ORC_PRESS_BAROTROPIC 0x200
 
 /* ------------------------------------------------------------------------------------------- */
 /* cp of H2 and cp_lapse                                                                        */
@@ -726,9 +728,234 @@ static void wall_passes(int im, int jm, int km, double* pd) {
         for (int j = 0; j < jm; j++) pd[IDX(i, j, km - 1)] = pd[IDX(i, j, 0)];
 }
 
+/* PRESS_LINE (planet/PressureSolver.h:497-570): every (j, k) column of one colour ((j + k) & 1)
+ * relaxed exactly in r by the Thomas algorithm, the theta / phi neighbours at their current values,
+ * the two radial end rows folded with the 2-point Neumann relation; the column's weights are the
+ * line pass's own expressions (:522-525). */
+static void ref_line_pass(const orc_geo* G, const atu_params* p, const double* st, const double* src, double* pd) {
+    const int im = G->im, jm = G->jm, km = G->km, n = km - 1;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    for (int colour = 0; colour < 2; colour++) {
+#pragma omp parallel
+        {
+            double* aa = malloc(sizeof(double) * im);
+            double* bb = malloc(sizeof(double) * im);
+            double* cc = malloc(sizeof(double) * im);
+            double* dd = malloc(sizeof(double) * im);
+            double* cp = malloc(sizeof(double) * im);
+            double* dp = malloc(sizeof(double) * im);
+#pragma omp for collapse(2) schedule(static)
+            for (int j = 1; j < jm - 1; j++)
+                for (int k = 0; k < n; k++) {
+                    if (((j + k) & 1) != colour) continue;
+                    const int kw = (k == 0) ? n - 1 : k - 1, ke = (k == n - 1) ? 0 : k + 1;
+                    for (int i = 1; i < im - 1; i++) {
+                        const double rm = metric_radius(p, G->radz[i]);
+                        const double erm = 1.0;
+                        const double num1 = erm * erm * G->inv_dr2;
+                        const double num2 = G->inv_dthe2 / (rm * rm);
+                        const double num3 = G->inv_dphi2 / (rm * rm * st[j] * st[j]);
+                        const double acc_h = num2 * (pd[IDX(i, j + 1, k)] + pd[IDX(i, j - 1, k)])
+                                             + num3 * (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)]);
+                        const double den = 2.0 * (num1 + num2 + num3);
+                        aa[i] = -num1;
+                        bb[i] = den;
+                        cc[i] = -num1;
+                        dd[i] = acc_h - src[IDX(i, j, k)];
+                    }
+                    const int lo = 1, hi = im - 2;
+                    if (hi > lo) {
+                        bb[lo] += aa[lo] * c43;
+                        cc[lo] -= aa[lo] * c13;
+                        bb[hi] += cc[hi] * c43;
+                        aa[hi] -= cc[hi] * c13;
+                    }
+                    cp[lo] = cc[lo] / bb[lo];
+                    dp[lo] = dd[lo] / bb[lo];
+                    for (int q = lo + 1; q <= hi; q++) {
+                        const double dn = bb[q] - aa[q] * cp[q - 1];
+                        cp[q] = cc[q] / dn;
+                        dp[q] = (dd[q] - aa[q] * dp[q - 1]) / dn;
+                    }
+                    double x = dp[hi];
+                    if (isfinite(x)) pd[IDX(hi, j, k)] = x;
+                    for (int q = hi - 1; q >= lo; q--) {
+                        x = dp[q] - cp[q] * x;
+                        if (isfinite(x)) pd[IDX(q, j, k)] = x;
+                    }
+                }
+            free(aa); free(bb); free(cc); free(dd); free(cp); free(dp);
+        }
+    }
+}
+
+/* PRESS_BAROTROPIC (planet/PressureSolver.h:572-708): the column sums of the residual (src - Lc p,
+ * the end rows read as the wall pass would set them from the interior) divided by the sum of
+ * 1/rm^2, a DFT in phi (modes m > 0 of round-off size dropped), per mode a tridiagonal solve in theta
+ * with folded ends (mode 0: the consistency shift, q pinned at j = 1, zero sin-weighted mean), the
+ * inverse DFT added to every level of the column. Every sum in the reference's serial order. */
+static void ref_baro_pass(const orc_geo* G, const atu_params* p, const double* st, const double* src, double* pd) {
+    const int im = G->im, jm = G->jm, km = G->km, N = km - 1, J = jm - 2, NM = N / 2;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    double A2 = 0.0;
+    for (int i = 1; i < im - 1; i++) {
+        const double rm = metric_radius(p, G->radz[i]);
+        A2 += 1.0 / (rm * rm);
+    }
+    double* R = malloc(sizeof(double) * (size_t)J * N);
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < jm - 1; j++)
+        for (int k = 0; k < N; k++) {
+            const int kw = (k == 0) ? N - 1 : k - 1, ke = (k == N - 1) ? 0 : k + 1;
+            double sum = 0.0;
+            for (int i = 1; i < im - 1; i++) {
+                const double rm = metric_radius(p, G->radz[i]);
+                const double erm = 1.0;
+                const double num1 = erm * erm * G->inv_dr2;
+                const double num2 = G->inv_dthe2 / (rm * rm);
+                const double num3 = G->inv_dphi2 / (rm * rm * st[j] * st[j]);
+#define PJ(jn) ((jn) == 0 ? c43 * pd[IDX(i, 1, k)] - c13 * pd[IDX(i, 2, k)] \
+                : ((jn) == jm - 1 ? c43 * pd[IDX(i, jm - 2, k)] - c13 * pd[IDX(i, jm - 3, k)] : pd[IDX(i, jn, k)]))
+#define PI(in) ((in) == 0 ? c43 * pd[IDX(1, j, k)] - c13 * pd[IDX(2, j, k)] \
+                : ((in) == im - 1 ? c43 * pd[IDX(im - 2, j, k)] - c13 * pd[IDX(im - 3, j, k)] : pd[IDX(in, j, k)]))
+                const double p0 = pd[IDX(i, j, k)];
+                const double lc = num1 * (PI(i + 1) + PI(i - 1) - 2.0 * p0) + num2 * (PJ(j + 1) + PJ(j - 1) - 2.0 * p0)
+                                  + num3 * (pd[IDX(i, j, ke)] + pd[IDX(i, j, kw)] - 2.0 * p0);
+#undef PJ
+#undef PI
+                sum += src[IDX(i, j, k)] - lc;
+            }
+            R[(size_t)(j - 1) * N + k] = sum / A2;
+        }
+    double* ct = malloc(sizeof(double) * (size_t)(NM + 1) * N);
+    double* sn = malloc(sizeof(double) * (size_t)(NM + 1) * N);
+    for (int mm = 0; mm <= NM; mm++)
+        for (int k = 0; k < N; k++) {
+            const double a = 2.0 * M_PI * (double)((long)mm * k % N) / N;
+            ct[(size_t)mm * N + k] = cos(a);
+            sn[(size_t)mm * N + k] = sin(a);
+        }
+    double* Ac = malloc(sizeof(double) * (size_t)(NM + 1) * J);
+    double* As = malloc(sizeof(double) * (size_t)(NM + 1) * J);
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int mm = 0; mm <= NM; mm++)
+        for (int jj = 0; jj < J; jj++) {
+            double sc = 0.0, ss = 0.0;
+            const double* r = R + (size_t)jj * N;
+            for (int k = 0; k < N; k++) {
+                sc += r[k] * ct[(size_t)mm * N + k];
+                ss += r[k] * sn[(size_t)mm * N + k];
+            }
+            if (mm > 0) {
+                double ra = 0.0;
+                for (int k = 0; k < N; k++) ra += fabs(r[k]);
+                const double tol = 1.0e-12 * ra;
+                if (fabs(sc) < tol) sc = 0.0;
+                if (fabs(ss) < tol) ss = 0.0;
+            }
+            Ac[(size_t)mm * J + jj] = sc;
+            As[(size_t)mm * J + jj] = ss;
+        }
+    const double e = G->inv_dthe2;
+#pragma omp parallel
+    {
+        double* a = malloc(sizeof(double) * J);
+        double* b = malloc(sizeof(double) * J);
+        double* c = malloc(sizeof(double) * J);
+        double* cp = malloc(sizeof(double) * J);
+        double* dp = malloc(sizeof(double) * J);
+#pragma omp for schedule(static)
+        for (int mm = 0; mm <= NM; mm++) {
+            const double sl = sin(M_PI * mm / N);
+            const double lam = 4.0 * sl * sl * G->inv_dphi2;
+            for (int jj = 0; jj < J; jj++) {
+                const double s_ = st[jj + 1];
+                a[jj] = e;
+                c[jj] = e;
+                b[jj] = -2.0 * e - lam / (s_ * s_);
+            }
+            a[0] = 0.0;
+            b[0] += c43 * e;
+            c[0] -= c13 * e;
+            c[J - 1] = 0.0;
+            b[J - 1] += c43 * e;
+            a[J - 1] -= c13 * e;
+            for (int part = 0; part < 2; part++) {
+                if (mm == 0 && part == 1) continue;
+                if (2 * mm == N && part == 1) continue;
+                double* d = (part == 0 ? Ac : As) + (size_t)mm * J;
+                double b0 = b[0], c0 = c[0];
+                if (mm == 0) {
+                    double yr = 0.0, ys = 0.0;
+                    for (int jj = 0; jj < J; jj++) {
+                        const double y = (jj == 0 || jj == J - 1) ? 1.5 : 1.0;
+                        const double s_ = fabs(sin(G->thez[jj + 1]));
+                        yr += y * d[jj];
+                        ys += y * s_;
+                    }
+                    for (int jj = 0; jj < J; jj++) d[jj] -= yr / ys * fabs(sin(G->thez[jj + 1]));
+                    b0 = 1.0;
+                    c0 = 0.0;
+                    d[0] = 0.0;
+                }
+                cp[0] = c0 / b0;
+                dp[0] = d[0] / b0;
+                for (int jj = 1; jj < J; jj++) {
+                    const double dn = b[jj] - a[jj] * cp[jj - 1];
+                    cp[jj] = c[jj] / dn;
+                    dp[jj] = (d[jj] - a[jj] * dp[jj - 1]) / dn;
+                }
+                d[J - 1] = dp[J - 1];
+                for (int jj = J - 2; jj >= 0; jj--) d[jj] = dp[jj] - cp[jj] * d[jj + 1];
+                if (mm == 0) {
+                    double sq = 0.0, sw = 0.0;
+                    for (int jj = 0; jj < J; jj++) {
+                        const double s_ = fabs(sin(G->thez[jj + 1]));
+                        sq += s_ * d[jj];
+                        sw += s_;
+                    }
+                    for (int jj = 0; jj < J; jj++) d[jj] -= sq / sw;
+                }
+            }
+        }
+        free(a); free(b); free(c); free(cp); free(dp);
+    }
+#pragma omp parallel for schedule(static)
+    for (int jj = 0; jj < J; jj++)
+        for (int k = 0; k < N; k++) {
+            double q = 0.0;
+            for (int mm = 0; mm <= NM; mm++) {
+                const int edge = (mm == 0 || 2 * mm == N);
+                const double w_ = edge ? 1.0 : 2.0;
+                q += w_ * (Ac[(size_t)mm * J + jj] * ct[(size_t)mm * N + k]
+                           + (edge ? 0.0 : As[(size_t)mm * J + jj] * sn[(size_t)mm * N + k]));
+            }
+            q /= N;
+            for (int i = 0; i < im; i++) pd[IDX(i, jj + 1, k)] += q;
+        }
+    free(R); free(ct); free(sn); free(Ac); free(As);
+}
+
+/* The interior mean of p (i = 1..im-2, j = 1..jm-2, k = 0..km-2) summed in (i, j, k) order, as the
+ * reference's reduction does on one thread (planet/PressureSolver.h:309-316). */
+static double interior_mean_serial(int im, int jm, int km, const double* pd) {
+    double s = 0.0;
+    long n = 0;
+    for (int i = 1; i < im - 1; i++)
+        for (int j = 1; j < jm - 1; j++)
+            for (int k = 0; k < km - 1; k++) {
+                s += pd[IDX(i, j, k)];
+                n++;
+            }
+    return n ? s / n : 0.0;
+}
+
 /* `sweeps` red-black sweeps at relaxation factor omega; src_out (optional, dense) receives the
- * clamped source. */
-int orc_pressure(const atu_grid* g, const atu_params* p, double* fields, int sweeps, double omega, double* src_out) {
+ * clamped source. flags: ORC_PRESS_LINE and / or ORC_PRESS_BAROTROPIC, the reference's PRESS_LINE and
+ * PRESS_BAROTROPIC after the pointwise passes of every sweep, before its wall passes; with
+ * PRESS_LINE the interior mean of p is held after every sweep (:748-755). */
+int orc_pressure_flags(const atu_grid* g, const atu_params* p, double* fields, int sweeps, double omega, int flags,
+                       double* src_out) {
     orc_geo G;
     geo_init(&G, g, p);
     orc_pgeo q;
@@ -738,6 +965,14 @@ int orc_pressure(const atu_grid* g, const atu_params* p, double* fields, int swe
     double* src = src_out ? src_out : malloc(sizeof(double) * n3);
     proj_prep_src(&G, &q, p, fields, src);
     double* pd = fields + (size_t)ATU_P_DYN * n3;
+    double* st = malloc(sizeof(double) * jm);
+    for (int j = 0; j < jm; j++) {
+        st[j] = sin(G.thez[j]);
+        if (st[j] < p->sinthe_min) st[j] = p->sinthe_min;
+        if (st[j] == 0.0) st[j] = 1.0e-5;
+    }
+    const int line = (flags & ORC_PRESS_LINE) != 0, baro = (flags & ORC_PRESS_BAROTROPIC) != 0;
+    const double mean0 = line ? interior_mean_serial(im, jm, km, pd) : 0.0;
     if (sweeps < 1) sweeps = 1;
     for (int s = 0; s < sweeps; s++) {
         for (int colour = 0; colour < 2; colour++) {
@@ -757,12 +992,23 @@ int orc_pressure(const atu_grid* g, const atu_params* p, double* fields, int swe
                     }
                 }
         }
+        if (line) ref_line_pass(&G, p, st, src, pd);
+        if (baro) ref_baro_pass(&G, p, st, src, pd);
         wall_passes(im, jm, km, pd);
+        if (line) {
+            const double shift = interior_mean_serial(im, jm, km, pd) - mean0;
+            for (size_t x = 0; x < n3; x++) pd[x] -= shift;
+        }
     }
+    free(st);
     if (!src_out) free(src);
     pgeo_free(&q);
     geo_free(&G);
     return 0;
+}
+
+int orc_pressure(const atu_grid* g, const atu_params* p, double* fields, int sweeps, double omega, double* src_out) {
+    return orc_pressure_flags(g, p, fields, sweeps, omega, 0, src_out);
 }
 
 /* ------------------------------------------------------------------------------------------- */

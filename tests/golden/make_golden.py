@@ -45,6 +45,12 @@ CASES = {
     "rk4": ("rk4", 1, 1, True),
     "pressure_1": ("pressure", 1, 1, True),
     "pressure_20": ("pressure", 1, 20, True),
+    # the solver's PRESS_LINE / PRESS_BAROTROPIC options (PRESS_ENV), on one thread
+    "pressure_line_1": ("pressure", 1, 1, True),
+    "pressure_line_20": ("pressure", 1, 20, True),
+    "pressure_baro_1": ("pressure", 1, 1, True),
+    "pressure_baro_20": ("pressure", 1, 20, True),
+    "pressure_lb_20": ("pressure", 1, 20, True),
     "step_2x3": ("step", 2, 3, True),
     "rk4_uran": ("rk4_uran", 1, 1, True),
     "step_uran_2": ("step_uran", 2, 1, True),
@@ -71,6 +77,20 @@ TURB_ENV = {
     "turb_run_wall": {"ATURAN_TURB_FREE": "0", "ATURAN_ABL_TAPER": "1", "ATURAN_TURB_CURV": "0"},
     "turb_init_wall_keps": {"ATURAN_TURB_FREE": "0", "ATURAN_TURB_MODEL": "k_epsilon"},
 }
+# The red-black solver's two options (planet/PressureSolver.h:298-300), read from the environment:
+# the radial line pass and the barotropic column-mean correction after the pointwise passes of every
+# sweep. PRESS_LINE holds the interior mean of p with an OpenMP reduction whose order follows the
+# thread count, so these cases run on one thread: the serial (i, j, k) order.
+PRESS_ENV = {
+    "pressure_line_1": {"ATURAN_PRESS_LINE": "1"},
+    "pressure_line_20": {"ATURAN_PRESS_LINE": "1"},
+    "pressure_baro_1": {"ATURAN_PRESS_BAROTROPIC": "1"},
+    "pressure_baro_20": {"ATURAN_PRESS_BAROTROPIC": "1"},
+    "pressure_lb_20": {"ATURAN_PRESS_LINE": "1", "ATURAN_PRESS_BAROTROPIC": "1"},
+}
+# The same options as the library's solve-mode bits (ATU_SOLVE_PRESS_LINE, ATU_SOLVE_PRESS_BAROTROPIC).
+PRESS_FLAGS = {n: (0x100 if "ATURAN_PRESS_LINE" in e else 0) | (0x200 if "ATURAN_PRESS_BAROTROPIC" in e else 0)
+               for n, e in PRESS_ENV.items()}
 # The same knobs as keyword arguments of the library's and the oracle's closure calls.
 TURB_KNOBS = {
     "turb_run": {},
@@ -258,7 +278,8 @@ def main() -> None:
             write_bin(fin, F)
             if name in TURB_ENV:
                 write_turb_bin(fin, *turb_input(g))
-            env = {"OMP_NUM_THREADS": "8", "PATH": "/usr/bin", **TURB_ENV.get(name, {})}
+            env = {"OMP_NUM_THREADS": "1" if name in PRESS_ENV else "8", "PATH": "/usr/bin",
+                   **TURB_ENV.get(name, {}), **PRESS_ENV.get(name, {})}
             rr = subprocess.run([str(harness), str(fin), str(fout), mode, str(steps), str(sweeps)],
                                 capture_output=True, text=True, env=env)
             if rr.returncode != 0:

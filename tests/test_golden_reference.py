@@ -1,6 +1,7 @@
 """Pins the CPU oracle to golden vectors written by the compiled reference (tests/golden/ref_*.npz,
 from tests/golden/make_golden.py): the domain BC, one RungeKuttaUran (hydro prep + 4 stages), the
-red-black PressureSolver at 1 and 20 sweeps, two full steps, and the default PressureSolverUran after
+red-black PressureSolver at 1 and 20 sweeps (alone, with its PRESS_LINE radial line pass, with its
+PRESS_BAROTROPIC column-mean correction, and with both), two full steps, and the default PressureSolverUran after
 one RungeKuttaUran and in two full steps, the Shapiro wiggle filter, the turbulence closure
 (Turbulence::run and ::init under six sets of its knobs) and the loop's saturation-adjustment block
 (four gases and the NH4SH equilibrium, on two inputs), on an input in which every term of
@@ -50,7 +51,7 @@ def run_oracle(oracle, name, g, p):
         level = oracle.hydro(g, p, F)
         oracle.rk4(g, p, F)
     elif mode == "pressure":
-        oracle.pressure(g, p, F, sweeps, 1.0)
+        oracle.pressure(g, p, F, sweeps, 1.0, MG.PRESS_FLAGS.get(name, 0))
     elif mode == "rk4_uran":
         level = oracle.hydro(g, p, F)
         rhs3 = oracle.rk4_rhs(g, p, F)
