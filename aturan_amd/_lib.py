@@ -42,6 +42,9 @@ PREC_MIXED = 1
 SOLVE_REFERENCE = 0
 SOLVE_CONVERGE = 1
 SOLVE_URAN = 2
+SOLVE_MODE_MASK = 0xFF
+SOLVE_PRESS_LINE = 0x100
+SOLVE_PRESS_BAROTROPIC = 0x200
 
 STATUS = {0: "ok", -1: "invalid argument", -2: "HIP error", -3: "out of memory", -4: "non-finite value",
           -5: "no convergence", -6: "unsupported"}

@@ -115,9 +115,13 @@ class SolveResult:
 
 
 def solver_opts(mode: str = "reference", sweeps: int = 1, omega: float = 1.0, rel_tol: float = 1e-8,
-                max_sweeps: int = 200, warm_start: int = 0) -> L.SolverOpts:
+                max_sweeps: int = 200, warm_start: int = 0, line: bool = False,
+                barotropic: bool = False) -> L.SolverOpts:
+    """line / barotropic: the reference solver's PRESS_LINE / PRESS_BAROTROPIC options (mode
+    "reference" only; the library refuses them with another mode)."""
     o = L.SolverOpts()
     o.mode = {"reference": L.SOLVE_REFERENCE, "converge": L.SOLVE_CONVERGE, "uran": L.SOLVE_URAN}[mode]
+    o.mode |= (L.SOLVE_PRESS_LINE if line else 0) | (L.SOLVE_PRESS_BAROTROPIC if barotropic else 0)
     o.sweeps = sweeps
     o.max_sweeps = max_sweeps
     o.warm_start = int(warm_start)
@@ -211,18 +215,22 @@ class UranusCore:
         _check(self._lib.atu_apply_bc(self._ctx), self._ctx, "atu_apply_bc")
 
     def PressureSolver(self, mode: str = "reference", sweeps: int = 1, omega: float = 1.0,
-                       rel_tol: float = 1e-8, max_sweeps: int = 200) -> SolveResult:
-        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps)
+                       rel_tol: float = 1e-8, max_sweeps: int = 200, line: bool = False,
+                       barotropic: bool = False) -> SolveResult:
+        """PressureSolver::run: the projection source, then the solve of `mode`; with mode "reference",
+        `line` and `barotropic` switch on the reference's PRESS_LINE and PRESS_BAROTROPIC."""
+        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps, line=line, barotropic=barotropic)
         st = L.SolveStats()
         _check(self._lib.atu_pressure_solve(self._ctx, C.byref(o), C.byref(st)), self._ctx, "atu_pressure_solve")
         return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)
 
     def step(self, nsteps: int = 1, mode: str = "reference", sweeps: int = 1, omega: float = 1.0,
-             rel_tol: float = 1e-8, max_sweeps: int = 200, warm_start: int = 0) -> SolveResult:
+             rel_tol: float = 1e-8, max_sweeps: int = 200, warm_start: int = 0, line: bool = False,
+             barotropic: bool = False) -> SolveResult:
         """The benchmark's outer step: hydro prep, RK4, BC, Poisson solve; nsteps times (converge with
         warm_start 1: each solve after the first starts from 2 p_n - p_{n-1}; 2: after the second from
         3 p_n - 3 p_{n-1} + p_{n-2})."""
-        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps, warm_start)
+        o = solver_opts(mode, sweeps, omega, rel_tol, max_sweeps, warm_start, line, barotropic)
         st = L.SolveStats()
         _check(self._lib.atu_step(self._ctx, nsteps, C.byref(o), C.byref(st)), self._ctx, "atu_step")
         return SolveResult(st.sweeps_done, st.outer_iters, st.rel_resid, st.src_norm)

@@ -94,6 +94,10 @@ struct atu_ctx {
     UranTables huran;
     UranTabs duran{};
     void* uran_blob = nullptr;
+    // the red-black solver's PRESS_LINE / PRESS_BAROTROPIC options: their weights and factors
+    PressRefTables hpref;
+    DevPressRef dpref{};
+    void* pref_blob = nullptr;
     double* RHS3 = nullptr;   // 3 fields, allocated when first kept
     bool keep_rhs = false;    // stage 3 writes RHS3
     bool rhs_valid = false;   // RHS3 holds the tendencies of the last RK4
@@ -192,6 +196,7 @@ atu_ctx::~atu_ctx() {
     if (MBOX) hipHostFree(MBOX);
     if (ev_ctl) hipEventDestroy(ev_ctl);
     if (uran_blob) hipFree(uran_blob);
+    if (pref_blob) hipFree(pref_blob);
     if (RHS3) hipFree(RHS3);
     for (double* p : {win.ckpt, win.tape, win.tape_level, win.LY, win.LYN, win.LS, win.LK, win.LK2, win.LAUX, win.LP,
                       win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.B, win.LSRC, win.GREF, win.TGT, win.ROWS,
@@ -323,6 +328,33 @@ int upload_uran_tables(atu_ctx* c) {
     t.two_dthe = u.two_dthe;
     t.two_dphi = u.two_dphi;
     t.src_scale = u.src_scale;
+    return ATU_OK;
+}
+
+int upload_press_ref_tables(atu_ctx* c) {
+    const PressRefTables& t = c->hpref;
+    const std::vector<const std::vector<double>*> parts = {&t.num2, &t.num3, &t.line_cp, &t.line_dn, &t.abs_sin};
+    std::vector<double> blob;
+    std::vector<size_t> off;
+    for (auto* v : parts) {
+        off.push_back(blob.size());
+        blob.insert(blob.end(), v->begin(), v->end());
+    }
+    if (c->hip(hipMalloc(&c->pref_blob, blob.size() * sizeof(double)), "hipMalloc(line and barotropic tables)") ||
+        c->hip(hipMemcpy(c->pref_blob, blob.data(), blob.size() * sizeof(double), hipMemcpyHostToDevice),
+               "hipMemcpy(line and barotropic tables)"))
+        return ATU_EHIP;
+    const double* d = (const double*)c->pref_blob;
+    DevPressRef& r = c->dpref;
+    const double** dst[] = {&r.num2, &r.num3, &r.line_cp, &r.line_dn, &r.abs_sin};
+    for (size_t q = 0; q < parts.size(); q++) *dst[q] = d + off[q];
+    r.num1 = t.num1;
+    r.a_mid = t.a_mid;
+    r.a_hi = t.a_hi;
+    r.A2 = t.A2;
+    r.ys = t.ys;
+    r.sw = t.sw;
+    r.ni = t.ni;
     return ATU_OK;
 }
 
@@ -713,25 +745,71 @@ int do_uran(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
     return ATU_OK;
 }
 
-int do_pressure_fp64(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
-    if (o->mode == ATU_SOLVE_CONVERGE) return do_converge(c, o, st, c->PDYN, c->SRC);
-    if (o->mode == ATU_SOLVE_URAN) return do_uran(c, o, st);
-    if (o->mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EINVAL, "unknown solve mode");
-    const int sweeps = o->sweeps > 0 ? o->sweeps : 1;   // PressureSolver.h:174-175
-    for (int s = 0; s < sweeps; s++) {
-        for (int colour = 0; colour < 2; colour++) {
-            if (c->hip(c->timed(PC_RB,
-                                [&] {
-                                    return launch_rb_pass<double>(c->sh, c->geo, c->consts, c->PDYN, c->SRC, colour,
-                                                                  o->omega, c->stream);
-                                }),
-                       "rb_pass"))
-                return ATU_EHIP;
-        }
-    }
-    if (c->hip(c->timed(PC_RB, [&] { return launch_p_corners<double>(c->sh, c->consts, c->PDYN, c->stream); }),
-               "p_corners"))
+// The red-black sweeps with the reference's PRESS_LINE and / or PRESS_BAROTROPIC options
+// (planet/PressureSolver.h:318-757): per sweep the two pointwise colour passes (walls left for
+// later), the radial line pass, the barotropic correction, the wall passes and, with PRESS_LINE, the
+// interior mean of p held at its value before the first sweep.
+int do_reference_options(atu_ctx* c, const atu_solver_opts* o, int sweeps, int flags) {
+    const Shape& sh = c->sh;
+    hipStream_t s = c->stream;
+    if (sh.im < 4 || sh.jm < 4 || sh.km < 3)
+        return c->fail(ATU_EINVAL, "PRESS_LINE / PRESS_BAROTROPIC need im, jm >= 4 and km >= 3");
+    const bool line = (flags & ATU_SOLVE_PRESS_LINE) != 0, baro = (flags & ATU_SOLVE_PRESS_BAROTROPIC) != 0;
+    double* P = c->PDYN;
+    auto H = [&](hipError_t e, const char* what) { return c->hip(e, what); };
+    if (line && H(c->timed(PC_LINE, [&] { return launch_serial_interior_sum(sh, P, c->SCAL + 5, s); }), "mean"))
         return ATU_EHIP;
+    for (int q = 0; q < sweeps; q++) {
+        for (int colour = 0; colour < 2; colour++)
+            if (H(c->timed(PC_RB, [&] { return launch_rb_pass<double>(sh, c->geo, c->consts, P, c->SRC, colour,
+                                                                      o->omega, s, 0); }), "rb_pass"))
+                return ATU_EHIP;
+        if (line)
+            for (int colour = 0; colour < 2; colour++)
+                if (H(c->timed(PC_LINE, [&] { return launch_rline(sh, c->dpref, P, c->SRC, c->RES, colour, s); }),
+                      "line pass"))
+                    return ATU_EHIP;
+        if (baro && H(c->timed(PC_BARO, [&] { return launch_rbaro(sh, c->dpref, c->dsol, P, c->SRC, c->BR, c->BAC,
+                                                                  c->BAS, s); }), "barotropic correction"))
+            return ATU_EHIP;
+        if (H(c->timed(PC_RB, [&] { return launch_walls<double>(sh, P, s); }), "walls") ||
+            H(c->timed(PC_RB, [&] { return launch_p_corners<double>(sh, c->consts, P, s); }), "p_corners") ||
+            H(c->timed(PC_RB, [&] { return launch_seam<double>(sh, P, s); }), "seam"))
+            return ATU_EHIP;
+        if (line && (H(c->timed(PC_LINE, [&] { return launch_serial_interior_sum(sh, P, c->SCAL + 6, s); }), "mean") ||
+                     H(launch_shift_all<double>(sh, c->SCAL, 5, 6, c->dsol.ncell, P, s), "shift")))
+            return ATU_EHIP;
+    }
+    return ATU_OK;
+}
+
+int do_pressure_fp64(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
+    const int mode = o->mode & ATU_SOLVE_MODE_MASK, flags = o->mode & ~ATU_SOLVE_MODE_MASK;
+    if (flags && (mode != ATU_SOLVE_REFERENCE || (flags & ~(ATU_SOLVE_PRESS_LINE | ATU_SOLVE_PRESS_BAROTROPIC))))
+        return c->fail(ATU_EINVAL, "PRESS_LINE / PRESS_BAROTROPIC are options of ATU_SOLVE_REFERENCE only");
+    if (mode == ATU_SOLVE_CONVERGE) return do_converge(c, o, st, c->PDYN, c->SRC);
+    if (mode == ATU_SOLVE_URAN) return do_uran(c, o, st);
+    if (mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EINVAL, "unknown solve mode");
+    const int sweeps = o->sweeps > 0 ? o->sweeps : 1;   // PressureSolver.h:174-175
+    if (flags) {
+        const int e = do_reference_options(c, o, sweeps, flags);
+        if (e) return e;
+    } else {
+        for (int s = 0; s < sweeps; s++) {
+            for (int colour = 0; colour < 2; colour++) {
+                if (c->hip(c->timed(PC_RB,
+                                    [&] {
+                                        return launch_rb_pass<double>(c->sh, c->geo, c->consts, c->PDYN, c->SRC,
+                                                                      colour, o->omega, c->stream);
+                                    }),
+                           "rb_pass"))
+                    return ATU_EHIP;
+            }
+        }
+        if (c->hip(c->timed(PC_RB, [&] { return launch_p_corners<double>(c->sh, c->consts, c->PDYN, c->stream); }),
+                   "p_corners"))
+            return ATU_EHIP;
+    }
     if (st) {
         st->sweeps_done = sweeps;
         st->outer_iters = 1;
@@ -1141,6 +1219,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     build_step_consts(*g, *p, c->consts);
     build_solve_tables(*g, *p, c->hgeo, c->htab);
     build_uran_tables(*g, *p, c->hgeo, c->huran);
+    build_press_ref_tables(*g, *p, c->hgeo, c->hpref);
     int e;
     if ((e = c->hip(hipStreamCreate(&c->stream), "hipStreamCreate"))) {
         g_create_error = c->err;
@@ -1154,6 +1233,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         return e;
     }
     if ((e = upload_geometry(c.get())) || (e = upload_solve_tables(c.get())) || (e = upload_uran_tables(c.get())) ||
+        (e = upload_press_ref_tables(c.get())) ||
         (e = alloc_fields(c.get(), &c->Y, ATU_NPROG)) ||
         (e = alloc_fields(c.get(), &c->ACC, ATU_NPROG)) || (e = alloc_fields(c.get(), &c->FROZEN, ATU_NFROZEN)) ||
         (e = alloc_fields(c.get(), &c->PSTAT, 1)) || (e = alloc_fields(c.get(), &c->PDYN, 1)) ||

@@ -270,6 +270,71 @@ void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& 
     }
 }
 
+void build_press_ref_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, PressRefTables& t) {
+    const int im = g.im, jm = g.jm, ni = im - 2, J = jm - 2;
+    const double c43 = 4.0 / 3.0, c13 = 1.0 / 3.0;
+    t.ni = ni;
+    std::vector<double> st(jm);   // the solver's sine table, planet/PressureSolver.h:156-162
+    for (int j = 0; j < jm; j++) {
+        double s = std::sin(geo.thez[j]);
+        if (s < p.sinthe_min) s = p.sinthe_min;
+        if (s == 0.0) s = 1.0e-5;
+        st[j] = s;
+    }
+    const double erm = 1.0;                       // no coordinate stretching
+    t.num1 = erm * erm * geo.inv_dr2;
+    t.num2.assign(im, 0.0);
+    t.num3.assign((size_t)im * jm, 0.0);
+    for (int i = 0; i < im; i++) {
+        const double rm = geo.rm[i];
+        t.num2[i] = geo.inv_dthe2 / (rm * rm);
+        for (int j = 0; j < jm; j++) t.num3[(size_t)i * jm + j] = geo.inv_dphi2 / (rm * rm * st[j] * st[j]);
+    }
+    // a row's system: a = c = -num1, b = 2 (num1 + num2 + num3), the ends folded (:547-550)
+    t.a_mid = -t.num1;
+    t.a_hi = -t.num1 - (-t.num1) * c13;
+    t.line_cp.assign((size_t)jm * ni, 0.0);
+    t.line_dn.assign((size_t)jm * ni, 0.0);
+    for (int j = 1; j < jm - 1; j++) {
+        std::vector<double> aa(ni, -t.num1), bb(ni), cc(ni, -t.num1);
+        for (int s = 0; s < ni; s++) {
+            const size_t q = (size_t)(s + 1) * jm + j;
+            bb[s] = 2.0 * (t.num1 + t.num2[s + 1] + t.num3[q]);
+        }
+        if (ni > 1) {
+            bb[0] += aa[0] * c43;
+            cc[0] -= aa[0] * c13;
+            bb[ni - 1] += cc[ni - 1] * c43;
+            aa[ni - 1] -= cc[ni - 1] * c13;
+        }
+        double* cp = &t.line_cp[(size_t)j * ni];
+        double* dn = &t.line_dn[(size_t)j * ni];
+        dn[0] = bb[0];
+        cp[0] = cc[0] / bb[0];
+        for (int s = 1; s < ni; s++) {
+            dn[s] = bb[s] - aa[s] * cp[s - 1];
+            cp[s] = cc[s] / dn[s];
+        }
+    }
+    double A2 = 0.0;
+    for (int i = 1; i < im - 1; i++) {
+        const double rm = geo.rm[i];
+        A2 += 1.0 / (rm * rm);
+    }
+    t.A2 = A2;
+    t.abs_sin.assign(J > 0 ? J : 0, 0.0);
+    double ys = 0.0, sw = 0.0;
+    for (int jj = 0; jj < J; jj++) {
+        const double y = (jj == 0 || jj == J - 1) ? 1.5 : 1.0;
+        const double s_ = std::abs(std::sin(geo.thez[jj + 1]));
+        t.abs_sin[jj] = s_;
+        ys += y * s_;
+    }
+    for (int jj = 0; jj < J; jj++) sw += t.abs_sin[jj];
+    t.ys = ys;
+    t.sw = sw;
+}
+
 // PressureSolverUran::run's weights at PRESS_CONSISTENT (planet/PressureSolverUran.h:40-46, 67-68,
 // 146-158 for the source, 180-195 for the sweep). Its sine table is a lone sin() per row, floored.
 void build_uran_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, UranTables& u) {

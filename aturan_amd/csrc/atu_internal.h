@@ -46,6 +46,21 @@ struct SolveTables {
 };
 void build_solve_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, SolveTables& t);
 
+// The red-black solver's PRESS_LINE pass and PRESS_BAROTROPIC correction (planet/PressureSolver.h:
+// 497-708), each weight formed with the reference's own expression: the line rows' num2 and num3
+// (:522-524, a separate set from the pointwise pass's), the Thomas factors of every row j's folded
+// radial system (:518-555; the same for every column of the row), the column weight
+// A2 = sum_i 1 / rm^2 (:580) and the mode-0 weights |sin theta| with their sums (:657-669).
+struct PressRefTables {
+    int ni = 0;
+    std::vector<double> num2;              // [im]
+    std::vector<double> num3;              // [im][jm]
+    std::vector<double> line_cp, line_dn;  // [jm][ni]
+    std::vector<double> abs_sin;           // [J], |sin(theta_{jj+1})|
+    double num1 = 0, a_mid = 0, a_hi = 0, A2 = 0, ys = 0, sw = 0;
+};
+void build_press_ref_tables(const atu_grid& g, const atu_params& p, const Geometry& geo, PressRefTables& t);
+
 // Weights of PressureSolverUran (planet/PressureSolverUran.h): the defect-correction source's set
 // (a_*) and the sweep's set (m_*), each formed with the reference's own expressions.
 struct UranTables {

@@ -99,9 +99,11 @@ hipError_t launch_proj_src(const Shape& sh, const DevGeo& g, const StepConsts& c
 // One colour pass of the red-black relaxation on p (FP64 or FP32 storage, FP64 source). The black
 // pass (colour 1) also writes the radial and polar wall values and the seam copy that the
 // reference's boundary passes form after each sweep.
+// With form_walls 0 the black pass leaves the walls to a separate pass (launch_walls, launch_p_corners,
+// launch_seam), for passes that run between the colour passes and the walls.
 template <typename P>
 hipError_t launch_rb_pass(const Shape& sh, const DevGeo& g, const StepConsts& c, P* p, const double* src,
-                          int colour, double omega, hipStream_t s);
+                          int colour, double omega, hipStream_t s, int form_walls = 1);
 // The four corner rows (i = 0, im-1 at j = 0, jm-1) and their seam copies, after the last sweep.
 template <typename P>
 hipError_t launch_p_corners(const Shape& sh, const StepConsts& c, P* p, hipStream_t s, const int* skip = nullptr);
@@ -161,6 +163,25 @@ hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hip
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
                             hipStream_t s, const int* skip = nullptr);
+
+// ---- the red-black solver's PRESS_LINE and PRESS_BAROTROPIC options (atu_press_ref.hip) ----
+struct DevPressRef {
+    const double *num2, *num3, *line_cp, *line_dn, *abs_sin;
+    double num1, a_mid, a_hi, A2, ys, sw;
+    int ni;
+};
+// One colour ((j + k) & 1) of the radial line pass on p; `scratch` holds a field (its contents are
+// not kept).
+hipError_t launch_rline(const Shape& sh, const DevPressRef& r, double* p, const double* src, double* scratch,
+                        int colour, hipStream_t s);
+// The interior sum of p (i = 1..im-2, j = 1..jm-2, k = 0..km-2) in the serial (i, j, k) order, into
+// *out (device).
+hipError_t launch_serial_interior_sum(const Shape& sh, const double* p, double* out, hipStream_t s);
+// The barotropic correction added to p: column residuals into R (J*N), their DFT into Ac / As
+// ((NM+1)*J each), the per-mode theta solves, the inverse DFT added to every level. The DFT's
+// cosines and sines and the modes' Thomas factors are the converging solve's tables `t`.
+hipError_t launch_rbaro(const Shape& sh, const DevPressRef& r, const DevSolve& t, double* p, const double* src,
+                        double* R, double* Ac, double* As, hipStream_t s);
 
 // ---- the Shapiro wiggle filter (atu_damp.hip): AtomUtils::damp_wiggles on one field; `scratch`
 // holds a field (its contents are not kept)

@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) k_src(Shape sh, DevGeo g, StepConsts c, c
 // ---- one colour pass; colour 1 also forms the wall values ----
 template <typename P>
 __global__ void __launch_bounds__(256) k_rb_pass(Shape sh, DevGeo g, double c43, double c13, P* p,
-                                                 const double* src, int colour, double omega) {
+                                                 const double* src, int colour, double omega, int form_walls) {
     const int nk = sh.km - 1;
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
     const int i = 1 + blockIdx.z;
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_rb_pass(Shape sh, DevGeo g, double c43,
     const P pv = (P)pn;
     p[x] = pv;
     if (k == 0) p[x + nk] = pv;
-    if (colour != 1) return;
+    if (colour != 1 || !form_walls) return;
     // wall values of this sweep (planet/PressureSolver.h:718-746)
     auto put = [&](size_t q, double val) {
         p[q] = (P)val;
@@ -260,10 +260,10 @@ hipError_t launch_proj_src(const Shape& sh, const DevGeo& g, const StepConsts& c
 
 template <typename P>
 hipError_t launch_rb_pass(const Shape& sh, const DevGeo& g, const StepConsts& c, P* p, const double* src, int colour,
-                          double omega, hipStream_t s) {
+                          double omega, hipStream_t s, int form_walls) {
     const int half = (sh.km - 1 + 1) / 2;
     hipLaunchKernelGGL((k_rb_pass<P>), dim3((half + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0, s, sh,
-                       g, c.c43, c.c13, p, src, colour, omega);
+                       g, c.c43, c.c13, p, src, colour, omega, form_walls);
     return hipGetLastError();
 }
 
@@ -288,9 +288,9 @@ template hipError_t launch_proj_src<double>(const Shape&, const DevGeo&, const S
 template hipError_t launch_proj_src<float>(const Shape&, const DevGeo&, const StepConsts&, float*, double*,
                                            hipStream_t);
 template hipError_t launch_rb_pass<double>(const Shape&, const DevGeo&, const StepConsts&, double*, const double*, int,
-                                           double, hipStream_t);
+                                           double, hipStream_t, int);
 template hipError_t launch_rb_pass<float>(const Shape&, const DevGeo&, const StepConsts&, float*, const double*, int,
-                                          double, hipStream_t);
+                                          double, hipStream_t, int);
 template hipError_t launch_p_corners<double>(const Shape&, const StepConsts&, double*, hipStream_t, const int*);
 template hipError_t launch_p_corners<float>(const Shape&, const StepConsts&, float*, hipStream_t, const int*);
 template hipError_t launch_residual<double>(const Shape&, const DevGeo&, const double*, const double*, double*, double*,

@@ -139,8 +139,29 @@ typedef enum atu_solve_mode {
                                 walls of p. FP64 contexts only.                                */
 } atu_solve_mode;
 
+/* Options OR'ed into atu_solver_opts.mode with ATU_SOLVE_REFERENCE: the reference's environment
+ * switches <TAG>_PRESS_LINE and <TAG>_PRESS_BAROTROPIC (planet/PressureSolver.h:284-303), run after
+ * the pointwise colour passes of every sweep and before its wall passes, in this order:
+ *   ATU_SOLVE_PRESS_LINE        every (j, k) column of one colour ((j + k) & 1, colour 0 then 1)
+ *                               relaxed exactly in r by the Thomas algorithm, the radial ends folded
+ *                               with the 2-point Neumann relation; only finite values are written
+ *                               (:497-570). The interior mean of p is held at its value before the
+ *                               first sweep (:309-316, 748-755), summed in the serial (i, j, k) order
+ *                               of the reference's reduction on one thread (its order on more threads
+ *                               follows the thread count).
+ *   ATU_SOLVE_PRESS_BAROTROPIC  the column-mean correction: column residuals, a DFT in phi, one
+ *                               tridiagonal theta solve per mode (mode 0 made consistent, pinned at
+ *                               j = 1, zero sin-weighted mean), the inverse DFT added to every level
+ *                               (:572-708). The reference runs it only with a periodic seam and no
+ *                               solid body, which is this model's configuration.
+ * Any other bit, or either bit with another mode, is ATU_EINVAL. */
+#define ATU_SOLVE_MODE_MASK 0xff
+#define ATU_SOLVE_PRESS_LINE 0x100
+#define ATU_SOLVE_PRESS_BAROTROPIC 0x200
+
 typedef struct atu_solver_opts {
-    int32_t mode;            /* atu_solve_mode                                                */
+    int32_t mode;            /* atu_solve_mode, with ATU_SOLVE_REFERENCE optionally OR'ed with
+                                ATU_SOLVE_PRESS_LINE and / or ATU_SOLVE_PRESS_BAROTROPIC        */
     int32_t sweeps;          /* REFERENCE: sweeps to run. CONVERGE on a mixed context: FP32
                                 smoothing iterations per FP64 refinement (<= 0: 1)            */
     int32_t max_sweeps;      /* CONVERGE: upper bound on smoothing iterations                 */

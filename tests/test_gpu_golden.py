@@ -1,6 +1,7 @@
 """The HIP path against the compiled reference's golden vectors (tests/golden/ref_*.npz) on the
 enriched input in which every term of the step is non-zero: domain BC, one RungeKuttaUran (hydro
-prep + 4 stages), PressureSolver at 1 and 20 sweeps, two full steps, and the default
+prep + 4 stages), PressureSolver at 1 and 20 sweeps (alone and with its PRESS_LINE and
+PRESS_BAROTROPIC options), two full steps, and the default
 PressureSolverUran after one RungeKuttaUran and in two full steps, the Shapiro filter, the
 turbulence closure and the loop's saturation-adjustment block. FP64 bar: 1e-12 of each field's largest sampled magnitude, and 1e-12 relative on
 the exact field sums; the closure's own arrays (prod, tken, disn, vel_star) the same."""
@@ -58,7 +59,9 @@ def test_gpu_matches_compiled_reference(oracle, name):
             level = core.computeHydrostaticPressure()
             core.RungeKuttaUran()
         elif mode == "pressure":
-            core.PressureSolver("reference", sweeps=sweeps, omega=1.0)
+            fl = MG.PRESS_FLAGS.get(name, 0)
+            core.PressureSolver("reference", sweeps=sweeps, omega=1.0, line=bool(fl & 0x100),
+                                barotropic=bool(fl & 0x200))
         elif mode == "rk4_uran":
             core.keep_stage_rhs(True)
             level = core.computeHydrostaticPressure()
