@@ -134,6 +134,8 @@ _SIGS = {
     "atu_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(SolverOpts), C.POINTER(SolveStats)]),
     "atu_keep_stage_rhs": (C.c_int, [C.c_void_p, C.c_int]),
     "atu_pressure_residual": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "atu_pressure_residual_static": (C.c_int, [C.POINTER(Grid), C.POINTER(Params), C.c_void_p, C.c_void_p, C.c_int,
+                                               C.c_void_p, _DP, _DP]),
     "atu_field_device_ptr": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]),
     "atu_window_forward": (C.c_int, [C.c_void_p, C.POINTER(WindowOpts), _DP]),

@@ -68,6 +68,18 @@ def copy_probe(nbytes: int, launches: int = 20) -> float:
     return ms.value
 
 
+def pressure_residual_static(grid: L.Grid, params: L.Params, p_dyn_dptr: int, src_dptr: int, ld_k: int,
+                             stream: int | None = None) -> tuple[float, float]:
+    """atu_pressure_residual_static: (relative residual, source norm) of device arrays p_dyn and src
+    ([im][jm][ld_k] FP64) without a context, allocating nothing."""
+    lib = L.load()
+    r, s = C.c_double(), C.c_double()
+    _check(lib.atu_pressure_residual_static(C.byref(grid), C.byref(params), C.c_void_p(p_dyn_dptr),
+                                            C.c_void_p(src_dptr), int(ld_k), C.c_void_p(stream), C.byref(r),
+                                            C.byref(s)), None, "atu_pressure_residual_static")
+    return r.value, s.value
+
+
 def cp_table(mode: int = 2) -> np.ndarray:
     lib = L.load()
     n = C.c_int(0)
@@ -280,6 +292,14 @@ class UranusCore:
             raise ValueError(f"{name}: expected shape {shape}, got {a.shape}")
         _check(self._lib.atu_turb_set(self._ctx, L.TURB_ARRAYS[name], a.ctypes.data_as(L._DP)), self._ctx,
                "atu_turb_set")
+
+    def field_device_ptr(self, name: str | int) -> tuple[int, int]:
+        """(device address, ld_k) of one field (atu_field_device_ptr)."""
+        f = L.FIELD[name] if isinstance(name, str) else int(name)
+        ptr, ld = C.c_void_p(), C.c_int()
+        _check(self._lib.atu_field_device_ptr(self._ctx, f, C.byref(ptr), C.byref(ld), None), self._ctx,
+               "atu_field_device_ptr")
+        return int(ptr.value), ld.value
 
     def pressure_residual(self) -> tuple[float, float]:
         r, s = C.c_double(), C.c_double()

@@ -21,7 +21,9 @@ using namespace atu;
 
 namespace {
 
-std::string g_create_error;
+// The message of the last failed context-free call (atu_create, atu_pressure_residual_static): a
+// fixed buffer, so that setting it allocates nothing.
+char g_free_error[512];
 
 struct ProfEvent {
     int cls;
@@ -1178,17 +1180,19 @@ int do_turbulence(atu_ctx* c, const atu_turb_opts* in, bool init) {
 }  // namespace
 
 // =============================================================================================
+void atu::set_free_error(const char* msg) { std::snprintf(g_free_error, sizeof g_free_error, "%s", msg ? msg : ""); }
+
 extern "C" {
 
 int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** out) {
     if (!out) return ATU_EINVAL;
     *out = nullptr;
     if (!g || !p || !check_finite_grid(g)) {
-        g_create_error = "atu_create: invalid grid (need im,jm,km >= 6 and km-1 even)";
+        set_free_error("atu_create: invalid grid (need im,jm,km >= 6 and km-1 even)");
         return ATU_EINVAL;
     }
     if (precision != ATU_PREC_FP64 && precision != ATU_PREC_MIXED) {
-        g_create_error = "atu_create: unknown precision";
+        set_free_error("atu_create: unknown precision");
         return ATU_EINVAL;
     }
     {   // the HIP runtime of this process must be the one the library was built for: a second copy of
@@ -1200,7 +1204,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
                           "atu_create: libaturan_amd was built for HIP %d.%d but runs on HIP runtime %d.%d (another "
                           "libamdhip64 was loaded into this process first)",
                           HIP_VERSION_MAJOR, HIP_VERSION_MINOR, rv / 10000000, (rv / 100000) % 100);
-            g_create_error = m;
+            set_free_error(m);
             return ATU_EUNSUPPORTED;
         }
     }
@@ -1222,14 +1226,14 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
     build_press_ref_tables(*g, *p, c->hgeo, c->hpref);
     int e;
     if ((e = c->hip(hipStreamCreate(&c->stream), "hipStreamCreate"))) {
-        g_create_error = c->err;
+        set_free_error(c->err.c_str());
         return e;
     }
     c->own_stream = true;
     if ((e = c->hip(hipStreamCreate(&c->side), "hipStreamCreate(side)")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_bc, hipEventDisableTiming), "hipEventCreate")) ||
         (e = c->hip(hipEventCreateWithFlags(&c->ev_hydro, hipEventDisableTiming), "hipEventCreate"))) {
-        g_create_error = c->err;
+        set_free_error(c->err.c_str());
         return e;
     }
     if ((e = upload_geometry(c.get())) || (e = upload_solve_tables(c.get())) || (e = upload_uran_tables(c.get())) ||
@@ -1239,7 +1243,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         (e = alloc_fields(c.get(), &c->PSTAT, 1)) || (e = alloc_fields(c.get(), &c->PDYN, 1)) ||
         (e = alloc_fields(c.get(), &c->PHYDRO, 1)) || (e = alloc_fields(c.get(), &c->SRC, 1)) ||
         (e = alloc_fields(c.get(), &c->RES, 1))) {
-        g_create_error = c->err;
+        set_free_error(c->err.c_str());
         return e;
     }
     if (precision == ATU_PREC_FP64) {
@@ -1256,7 +1260,7 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         if (!e) e = alloc_fields(c.get(), &c->R32, 1);
     }
     if (e) {
-        g_create_error = c->err;
+        set_free_error(c->err.c_str());
         return e;
     }
     const size_t nres = (size_t)residual_partials_count(c->sh) * 2;
@@ -1264,11 +1268,11 @@ int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** 
         (e = c->hip(hipMalloc((void**)&c->LEVROWS, 2 * (size_t)g->im * g->jm * sizeof(double)), "hipMalloc")) ||
         (e = c->hip(hipMalloc((void**)&c->RPART, nres * sizeof(double)), "hipMalloc")) ||
         (e = c->hip(hipMalloc((void**)&c->RSUM, 2 * sizeof(double)), "hipMalloc"))) {
-        g_create_error = c->err;
+        set_free_error(c->err.c_str());
         return e;
     }
     if ((e = c->hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) {
-        g_create_error = c->err;
+        set_free_error(c->err.c_str());
         return e;
     }
     *out = c.release();
@@ -1282,7 +1286,7 @@ void atu_destroy(atu_ctx* ctx) {
     delete ctx;
 }
 
-const char* atu_last_error(const atu_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+const char* atu_last_error(const atu_ctx* ctx) { return ctx ? ctx->err.c_str() : g_free_error; }
 
 static int sync_all(atu_ctx* ctx) {
     if (ctx->hip(hipStreamSynchronize(ctx->side), "hipStreamSynchronize(side)")) return ATU_EHIP;

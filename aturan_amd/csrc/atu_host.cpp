@@ -125,7 +125,6 @@ void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo) {
         geo.two_inv_rm[i] = 2.0 * geo.inv_rm[i];
         const double frac = std::max(0.0, (double)(i - (im - 1) * 3 / 4) / (double)((im - 1) / 4));
         geo.sponge[i] = p.alpha_sponge * frac * frac;
-        geo.num2[i] = geo.inv_rm2[i] * geo.inv_dthe2;
     }
 
     // per j: RK tables (RungeKutta_Uran_Turb.cpp:192-197), the body-force true sine
@@ -175,15 +174,40 @@ void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo) {
             geo.inv_rm2sinthe2[q] = inv_rm2 / geo.sinthe2[j];
             geo.costhe_inv_rm2sinthe[q] = geo.costhe[j] * geo.inv_rm2sinthe[q];
             geo.v_metric[q] = (1.0 + geo.costhe[j] / geo.sinthe2[j]) * inv_rm2;
-            // solver weights (PressureSolver.h:328-353) from its own sine table
-            const double ss = sin_solver[j];
-            const double ss2 = ss * ss;
-            const double n3 = (inv_rm2 / ss2) * geo.inv_dphi2;
-            geo.num3[q] = n3;
-            geo.den[q] = 2.0 * geo.num1 + 2.0 * geo.num2[i] + 2.0 * n3;
-            geo.inv_rmsinthe_s[q] = 1.0 / (rm * ss);
+            geo.inv_rmsinthe_s[q] = 1.0 / (rm * sin_solver[j]);
         }
     }
+    solver_weights(g, p, geo.num2.data(), geo.num3.data(), geo.den.data());
+}
+
+double solver_weights(const atu_grid& g, const atu_params& p, double* num2, double* num3, double* den) {
+    const int im = g.im, jm = g.jm;
+    const double inv_dr2 = 1.0 / (g.dr * g.dr);
+    const double inv_dthe2 = 1.0 / (g.dthe * g.dthe);
+    const double inv_dphi2 = 1.0 / (g.dphi * g.dphi);
+    const double exp_rm = 1.0, exp_2_rm = exp_rm * exp_rm;   // no coordinate stretching
+    const double num1 = exp_2_rm * inv_dr2;
+    const double shift = (p.R_metric_km > 0.0) ? (p.R_metric_km / p.L_atm - 1.0) : 0.0;
+    double r = g.r0;                      // the coordinates by cumulative addition, as build_geometry
+    for (int i = 0; i < im; i++) {
+        if (i > 0) r = r + g.dr;
+        const double rm = (p.R_metric_km > 0.0) ? r + shift : r;
+        const double rm2 = rm * rm;
+        const double inv_rm2 = 1.0 / rm2;
+        num2[i] = inv_rm2 * inv_dthe2;
+        double th = g.the0;
+        for (int j = 0; j < jm; j++) {
+            if (j > 0) th = th + g.dthe;
+            double ss = std::sin(th);     // the solver's own sine table, PressureSolver.h:156-162
+            if (ss < p.sinthe_min) ss = p.sinthe_min;
+            if (ss == 0.0) ss = 1.0e-5;
+            const double ss2 = ss * ss;
+            const double n3 = (inv_rm2 / ss2) * inv_dphi2;
+            num3[(size_t)i * jm + j] = n3;
+            den[(size_t)i * jm + j] = 2.0 * num1 + 2.0 * num2[i] + 2.0 * n3;
+        }
+    }
+    return num1;
 }
 
 // The converging solve works on the system of the red-black sweep with every wall folded in

@@ -31,6 +31,10 @@ struct Geometry {
 };
 
 void build_geometry(const atu_grid& g, const atu_params& p, Geometry& geo);
+// The red-black sweep's weights (planet/PressureSolver.h:156-162, 327-353) written into caller
+// arrays, allocating nothing: num2 [im], num3 and den [im][jm]; returns num1. build_geometry's
+// num2, num3 and den are these.
+double solver_weights(const atu_grid& g, const atu_params& p, double* num2, double* num3, double* den);
 
 // Tables of the converging Poisson solve: Thomas factors of the radial lines (per (j, s), s = i-1,
 // the walls folded), of the per-mode theta systems of the barotropic correction (per (m, jj)), the
@@ -110,6 +114,9 @@ struct StepConsts {
     double p_dyn_cap, c43, c13;
 };
 void build_step_consts(const atu_grid& g, const atu_params& p, StepConsts& c);
+
+// Sets the message atu_last_error(NULL) returns (context-free calls); allocates nothing.
+void set_free_error(const char* msg);
 
 // Padded k stride of device fields: km rounded up to a multiple of 32 elements.
 inline int padded_ldk(int km) { return (km + 31) / 32 * 32; }

@@ -219,7 +219,9 @@ int atu_synthetic_state(const atu_grid* g, const atu_params* p, uint64_t seed, d
  * arrays of cUranusModel::resetArrays (planet/cUranusModel.cpp:1416-1634). */
 int atu_create(const atu_grid* g, const atu_params* p, int precision, atu_ctx** out);
 void atu_destroy(atu_ctx* ctx);
-const char* atu_last_error(const atu_ctx* ctx);   /* NULL ctx: the last create error */
+const char* atu_last_error(const atu_ctx* ctx);   /* NULL ctx: the last error of a context-free
+                                                     call (atu_create,
+                                                     atu_pressure_residual_static)            */
 
 /* Attach the context to a HIP stream (hipStream_t passed as void*; NULL = its own stream). */
 int atu_set_stream(atu_ctx* ctx, void* hip_stream);
@@ -271,6 +273,19 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
 /* Relative residual ||src - Lc p||_2 / ||src||_2 of the current p_dyn against the current source
  * (FP64; the source must have been prepared by a solve). */
 int atu_pressure_residual(atu_ctx* ctx, double* rel_resid, double* src_norm);
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): atu_pressure_residual's
+ * result for caller-held device arrays p_dyn and src ([im][jm][ld_k] FP64, k fastest, ld_k >= km)
+ * on grid g with parameters p, without a context. The same weights and kernels as a context's
+ * residual, so the result is bit-identical to it for the same arrays. The weights and block partials
+ * live in a static device array of 4 MiB per GPU (held until the process ends); 81x361x721 needs
+ * 1.8 MB of it, and a grid needing more than 4 MiB is ATU_EINVAL, with the need stated in
+ * atu_last_error(NULL). Allocates nothing; runs on `hip_stream` (hipStream_t as void*, NULL: the
+ * default stream) and returns when the result is on the host. Calls are serialised by a lock; a
+ * call overlapping another on one GPU would race on the array. */
+int atu_pressure_residual_static(const atu_grid* g, const atu_params* p, const double* p_dyn, const double* src,
+                                 int ld_k, void* hip_stream, double* rel_resid, double* src_norm);
 
 /* Shapiro (1-2-1) wiggle damping of one prognostic field: AtomUtils::damp_wiggles
  * (lib/Utils.cpp:224-306) with i_surface = nullptr, as the reference's loop calls it after the RK
