@@ -86,7 +86,7 @@ class SolveStats(C.Structure):
 
 class WindowOpts(C.Structure):
     _fields_ = [("nsteps", C.c_int32), ("checkpoint_every", C.c_int32), ("solve_iters", C.c_int32),
-                ("pad_", C.c_int32), ("rel_tol", C.c_double), ("target_amp", C.c_double)]
+                ("rb_sweeps", C.c_int32), ("rel_tol", C.c_double), ("target_amp", C.c_double)]
 
 
 class TurbOpts(C.Structure):

@@ -312,11 +312,12 @@ class UranusCore:
 
     # ---- gradients over a window of steps ----
     def window_forward(self, nsteps: int, checkpoint_every: int = 10, solve_iters: int = 0,
-                       rel_tol: float = 1e-10, target_amp: float = 1.2) -> float:
+                       rel_tol: float = 1e-10, target_amp: float = 1.2, rb_sweeps: int = 0) -> float:
         """Run `nsteps` steps from the current state, checkpointing every `checkpoint_every` steps, and
-        return the zonal-wind misfit L of the final state (atu_window_forward)."""
+        return the zonal-wind misfit L of the final state (atu_window_forward). rb_sweeps > 0: each
+        solve is that many red-black sweeps of the reference, differentiated sweep by sweep."""
         o = L.WindowOpts()
-        o.nsteps, o.checkpoint_every, o.solve_iters = nsteps, checkpoint_every, solve_iters
+        o.nsteps, o.checkpoint_every, o.solve_iters, o.rb_sweeps = nsteps, checkpoint_every, solve_iters, rb_sweeps
         o.rel_tol, o.target_amp = rel_tol, target_amp
         loss = C.c_double()
         _check(self._lib.atu_window_forward(self._ctx, C.byref(o), C.byref(loss)), self._ctx, "atu_window_forward")

@@ -542,6 +542,101 @@ __global__ void __launch_bounds__(256) k_adj_div(Shape sh, DevGeo g, const doubl
     laux[2 * fs + x] = lw;
 }
 
+// ---- adjoint of the red-black sweeps (ATU_SOLVE_REFERENCE, planet/PressureSolver.h:318-746) ----
+// The forward sweep is: red pass, black pass (each cell of the colour set from its six neighbours,
+// acc = num1 (p_r+ + p_r-) + num2_i (p_t+ + p_t-) + num3_ij (p_f+ + p_f-), p = (1 - w) p + w (acc - src)
+// / den), then the radial and polar walls from the interior; after the last sweep the corner rows
+// and the seam copy. Its adjoint runs the pieces in reverse: seam, corners, then per sweep (last
+// first) the walls folded onto the rows they are formed from, the black pass transposed, the red
+// pass transposed. A colour pass transposed: a cell x of the colour passes mu_x = w lambda_x / den_x
+// to every neighbour it read, with the weight it read it with (num1, num2 of its own i, num3 of its
+// own (i, j): the coefficients transposed), keeps (1 - w) lambda_x, and gives -mu_x to lambda_src.
+
+// the seam copy p(k = km-1) = p(k = 0), folded back; one thread per (i, j)
+__global__ void __launch_bounds__(256) k_seam_adj(Shape sh, double* lp) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (j >= sh.jm) return;
+    const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
+    lp[row] += lp[row + sh.km - 1];
+    lp[row + sh.km - 1] = 0.0;
+}
+
+// the corner rows (i = 0, im-1 at j = 0, jm-1) formed from the radial walls at j = 1, 2 and
+// jm-2, jm-3, folded back; one thread per k < km-1
+__global__ void __launch_bounds__(256) k_corners_adj(Shape sh, double c43, double c13, double* lp) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= sh.km - 1) return;
+    const size_t ldk = sh.ldk;
+    const int J = sh.jm - 1;
+    for (int s = 0; s < 2; s++) {
+        const size_t b = (size_t)(s == 0 ? 0 : sh.im - 1) * sh.plane + k;
+        const double lo = lp[b], hi = lp[b + (size_t)J * ldk];
+        lp[b] = 0.0;
+        lp[b + (size_t)J * ldk] = 0.0;
+        lp[b + ldk] += c43 * lo;
+        lp[b + 2 * ldk] -= c13 * lo;
+        lp[b + (size_t)(J - 1) * ldk] += c43 * hi;
+        lp[b + (size_t)(J - 2) * ldk] -= c13 * hi;
+    }
+}
+
+// one colour pass transposed, out of place (lin -> lout); lsrc accumulates
+__global__ void __launch_bounds__(256) k_rb_pass_adj(Shape sh, DevGeo g, double omega, int colour, const double* lin,
+                                                     double* lout, double* lsrc) {
+    const int N = sh.km - 1;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= sh.ldk) return;
+    const size_t ldk = sh.ldk, P = sh.plane;
+    const size_t x = ((size_t)i * sh.jm + j) * ldk + k;
+    if (k >= N) {   // the seam and the padding: read by no pass
+        lout[x] = lin[x];
+        return;
+    }
+    const int jm = sh.jm;
+    if (is_interior(sh, i, j) && ((i + j + k) & 1) == colour) {
+        const double mu = omega * lin[x] / g.den[(size_t)i * jm + j];
+        lsrc[x] -= mu;
+        lout[x] = (1.0 - omega) * lin[x];
+        return;
+    }
+    // mu of a neighbour that is an interior cell of the colour (its update read this cell), else 0
+    auto mu = [&](int ii, int jj, int kk, size_t xx) -> double {
+        if (!is_interior(sh, ii, jj) || ((ii + jj + kk) & 1) != colour) return 0.0;
+        return omega * lin[xx] / g.den[(size_t)ii * jm + jj];
+    };
+    double l = lin[x];
+    double r = 0.0;
+    if (i + 1 <= sh.im - 1) r += mu(i + 1, j, k, x + P);
+    if (i - 1 >= 0) r += mu(i - 1, j, k, x - P);
+    l += g.num1 * r;
+    double t = 0.0;
+    if (j + 1 <= sh.jm - 1) t += mu(i, j + 1, k, x + ldk);
+    if (j - 1 >= 0) t += mu(i, j - 1, k, x - ldk);
+    l += g.num2[i] * t;
+    if (is_interior(sh, i, j)) {   // the phi neighbours of a non-interior cell are not interior
+        const int ke = (k == N - 1) ? 0 : k + 1, kw = (k == 0) ? N - 1 : k - 1;
+        const size_t row = x - k;
+        l += g.num3[(size_t)i * jm + j] * (mu(i, j, ke, row + ke) + mu(i, j, kw, row + kw));
+    }
+    lout[x] = l;
+}
+
+// lambda_src of the clamped source: zero where the clamp |div| <= den * p_dyn_cap was active
+__global__ void __launch_bounds__(256) k_src_mask(Shape sh, DevGeo g, StepConsts c, const double* src, double* lsrc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= sh.ldk) return;
+    const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+    if (!is_interior(sh, i, j) || k >= sh.km - 1) {
+        lsrc[x] = 0.0;
+        return;
+    }
+    const double cap = g.den[(size_t)i * sh.jm + j] * c.p_dyn_cap;
+    const double v = src[x];
+    if (!((v < cap) && (v > -cap))) lsrc[x] = 0.0;
+}
+
 // p_int += g on the interior: the held interior mean makes p_{n+1} depend on mean(p_n)
 __global__ void __launch_bounds__(256) k_add_mean(Shape sh, const double* sc, double ncell, double* lp) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -665,6 +760,24 @@ hipError_t launch_adj_src(const Shape& sh, const DevGeo& g, const StepConsts& c,
 
 hipError_t launch_adj_div(const Shape& sh, const DevGeo& g, const double* lsrc, double* laux, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_div, cells_grid(sh, sh.ldk), dim3(256), 0, s, sh, g, lsrc, laux);
+    return hipGetLastError();
+}
+
+hipError_t launch_rb_sweeps_adjoint(const Shape& sh, const DevGeo& g, const StepConsts& c, int sweeps, double omega,
+                                    const double* src, double* lp, double* t1, double* t2, double* lsrc,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(k_seam_adj, dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, lp);
+    hipLaunchKernelGGL(k_corners_adj, dim3((sh.km - 1 + 255) / 256), dim3(256), 0, s, sh, 4.0 / 3.0, 1.0 / 3.0, lp);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((sh.fs + 255) / 256)), dim3(256), 0, s, lsrc, sh.fs);
+    for (int q = 0; q < sweeps; q++) {
+        hipLaunchKernelGGL(k_pwall_adj, cells_grid(sh, sh.ldk), dim3(256), 0, s, sh, 4.0 / 3.0, 1.0 / 3.0,
+                           (const double*)lp, t1);
+        hipLaunchKernelGGL(k_rb_pass_adj, cells_grid(sh, sh.ldk), dim3(256), 0, s, sh, g, omega, 1, (const double*)t1,
+                           t2, lsrc);
+        hipLaunchKernelGGL(k_rb_pass_adj, cells_grid(sh, sh.ldk), dim3(256), 0, s, sh, g, omega, 0, (const double*)t2,
+                           lp, lsrc);
+    }
+    hipLaunchKernelGGL(k_src_mask, cells_grid(sh, sh.ldk), dim3(256), 0, s, sh, g, c, src, lsrc);
     return hipGetLastError();
 }
 

@@ -850,6 +850,7 @@ int dcopy(atu_ctx* c, double* dst, const double* src, size_t nfields) {
 }
 
 atu_solver_opts window_solver(const atu_window_opts& w) {
+    if (w.rb_sweeps > 0) return atu_solver_opts{ATU_SOLVE_REFERENCE, w.rb_sweeps, 0, 0, 1.0, 0.0};
     atu_solver_opts o{ATU_SOLVE_CONVERGE, 0, 0, 0, 1.0, 0.0};
     if (w.solve_iters > 0) {
         o.max_sweeps = w.solve_iters;
@@ -987,9 +988,27 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
     hipStream_t s = c->stream;
     auto H = [&](hipError_t e, const char* w) { return c->hip(e, w); };
     int e;
+    const bool rb = o.mode == ATU_SOLVE_REFERENCE;
+    if (rb) {
+        // red-black sweeps: their transposes in reverse order (lambda of the initial p into LP), the
+        // source clamp, div^T into lambda_aux; lambda of p_n starts from the sweeps' share
+        if ((e = H(c->timed(PC_ADJ, [&] {
+                 hipError_t r = launch_rb_sweeps_adjoint(sh, c->geo, c->consts, o.sweeps, o.omega,
+                                                         tape_field(c, slot, TP_SRC), W.LP, W.LPI, W.B, W.LSRC, s);
+                 if (r == hipSuccess) r = launch_adj_div(sh, c->geo, W.LSRC, W.LAUX, s);
+                 if (r == hipSuccess) r = launch_bc_adjoint(sh, c->params, W.LY, s);
+                 if (r == hipSuccess)
+                     r = launch_final_fold(sh, c->consts, tape_field(c, slot, TP_TNEXT), W.LY, W.LS, W.LYN, s);
+                 if (r == hipSuccess)
+                     r = hipMemcpyAsync(W.LPN, W.LP, fs * sizeof(double), hipMemcpyDeviceToDevice, s);
+                 if (r == hipSuccess) r = launch_zero(W.LPH, fs, s);
+                 return r;
+             }), "red-black adjoint")))
+            return e;
+    }
     // Poisson solve: walls folded, the held mean split off, A^T mu = lambda by the forward solver on
     // Y^-1 lambda, the consistency shift and the source clamp, then div^T into lambda_aux
-    if ((e = H(c->timed(PC_ADJ, [&] {
+    if (!rb && (e = H(c->timed(PC_ADJ, [&] {
              hipError_t r = launch_pwall_adjoint(sh, W.LP, W.LPI, s);
              if (r == hipSuccess) r = launch_dsum<double>(DSUM_P, sh, c->geo, W.LPI, nullptr, c->SROWS, W.SC + 0, s);
              if (r == hipSuccess) r = launch_adj_rhs(sh, W.LPI, W.SC + 0, c->dsol.ncell, W.B, s);
@@ -997,8 +1016,8 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
              return r;
          }), "adjoint solve prep")))
         return e;
-    if ((e = do_converge(c, &o, nullptr, W.Z, W.B))) return e;
-    if ((e = H(c->timed(PC_ADJ, [&] {
+    if (!rb && (e = do_converge(c, &o, nullptr, W.Z, W.B))) return e;
+    if (!rb && (e = H(c->timed(PC_ADJ, [&] {
              hipError_t r = launch_dsum<double>(DSUM_YSRC, sh, c->geo, W.Z, W.Z, c->SROWS, W.SC + 1, s);
              if (r == hipSuccess)
                  r = launch_adj_src(sh, c->geo, c->consts, W.Z, W.SC + 1, c->dsol.yw, tape_field(c, slot, TP_SRC),
@@ -1050,7 +1069,7 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
     if ((e = H(c->timed(PC_ADJ, [&] {
              hipError_t r = launch_hydro_adjoint(sh, c->geo, c->consts, tape_field(c, slot, TP_X0), c->PSTAT,
                                                  W.tape_level + (size_t)slot * sh.im, W.LPH, W.GREF, W.LYN, s);
-             if (r == hipSuccess) r = launch_add_mean(sh, W.SC + 0, c->dsol.ncell, W.LPN, s);
+             if (r == hipSuccess && !rb) r = launch_add_mean(sh, W.SC + 0, c->dsol.ncell, W.LPN, s);
              return r;
          }), "adjoint hydro")))
         return e;
@@ -1555,8 +1574,9 @@ int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) 
     if (!ctx || !opts || opts->nsteps < 1) return ATU_EINVAL;
     ctx->state_epoch++;
     if (!ctx->params.rigid_lid) return ctx->fail(ATU_EUNSUPPORTED, "atu_window_forward: needs rigid_lid = 1");
-    if (opts->solve_iters <= 0 && !(opts->rel_tol > 0.0))
-        return ctx->fail(ATU_EINVAL, "atu_window_forward: give solve_iters > 0 or rel_tol > 0");
+    if (opts->rb_sweeps < 0) return ctx->fail(ATU_EINVAL, "atu_window_forward: rb_sweeps < 0");
+    if (opts->rb_sweeps == 0 && opts->solve_iters <= 0 && !(opts->rel_tol > 0.0))
+        return ctx->fail(ATU_EINVAL, "atu_window_forward: give rb_sweeps > 0, solve_iters > 0 or rel_tol > 0");
     auto& W = ctx->win;
     W.held = W.ready = false;
     W.generation = 0;

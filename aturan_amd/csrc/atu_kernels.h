@@ -263,6 +263,13 @@ hipError_t launch_adj_src(const Shape& sh, const DevGeo& g, const StepConsts& c,
                           double yw, const double* src, double* lsrc, hipStream_t s);
 hipError_t launch_adj_div(const Shape& sh, const DevGeo& g, const double* lsrc, double* laux, hipStream_t s);
 hipError_t launch_add_mean(const Shape& sh, const double* sc, double ncell, double* lp, hipStream_t s);
+// The adjoint of `sweeps` red-black sweeps (ATU_SOLVE_REFERENCE at relaxation factor omega) and the
+// source clamp: lp holds lambda of the solve's output on entry and of its initial p on return;
+// lsrc receives lambda of the unclamped source (src: the clamped source the forward used); t1, t2
+// are scratch fields.
+hipError_t launch_rb_sweeps_adjoint(const Shape& sh, const DevGeo& g, const StepConsts& c, int sweeps, double omega,
+                                    const double* src, double* lp, double* t1, double* t2, double* lsrc,
+                                    hipStream_t s);
 hipError_t launch_loss(const Shape& sh, const double* w, const double* target, double scale, double* rows, double* lw,
                        double* out, hipStream_t s);
 hipError_t launch_sum(const double* v, size_t n, double* part, int nparts, double* out, hipStream_t s);

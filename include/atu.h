@@ -414,8 +414,13 @@ typedef struct atu_window_opts {
     int32_t nsteps;
     int32_t checkpoint_every;   /* <= 0: 10                                                  */
     int32_t solve_iters;        /* fixed iterations per solve (> 0), or 0: solve to rel_tol    */
-    int32_t pad_;
-    double rel_tol;             /* used when solve_iters == 0                                 */
+    int32_t rb_sweeps;          /* 0: the solve is ATU_SOLVE_CONVERGE (solve_iters / rel_tol),
+                                   differentiated as an exact solve. > 0: the solve is that many
+                                   red-black sweeps of the reference (ATU_SOLVE_REFERENCE, omega
+                                   1), differentiated exactly: the sweeps transposed in reverse
+                                   order, each its walls folded back, then its black and its red
+                                   pass with transposed coefficients                          */
+    double rel_tol;             /* used when solve_iters == 0 and rb_sweeps == 0              */
     double target_amp;          /* <= 0: 1.2                                                  */
 } atu_window_opts;
 

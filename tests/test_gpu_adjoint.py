@@ -1,4 +1,6 @@
-"""The hand-written adjoint of the window of steps against central finite differences of the forward.
+"""The hand-written adjoint of the window of steps against central finite differences of the forward,
+with the converging solve (differentiated as an exact solve) and with the reference's red-black
+sweeps (differentiated sweep by sweep).
 
 Loss: mean over the interior (r, theta) of (zonal-mean zonal wind w - 1.2 cos 3 lat)^2 after the
 window. Gradients: with respect to the initial temperature field (checked along seeded random
@@ -29,7 +31,7 @@ def _state(g, p, oracle, seed=20261203):
     return F
 
 
-def _fd_check(g, p, F, opts, ndir, seed):
+def _fd_check(g, p, F, opts, ndir, seed, eps=EPS):
     rng = np.random.default_rng(seed)
     T = A.FIELD["t"]
     nue = A.FIELD["nue"]
@@ -52,14 +54,14 @@ def _fd_check(g, p, F, opts, ndir, seed):
         gnorm = float(np.linalg.norm(gT))
         for q in range(ndir):
             d = rng.standard_normal(F[T].shape)
-            fd = (loss(F[T] + EPS * d, F[nue]) - loss(F[T] - EPS * d, F[nue])) / (2 * EPS)
+            fd = (loss(F[T] + eps * d, F[nue]) - loss(F[T] - eps * d, F[nue])) / (2 * eps)
             ad = float(np.sum(gT * d))
             e = abs(fd - ad) / max(abs(ad), gnorm, 1e-300)
             worst = max(worst, e)
             print(f"[adjoint] dir {q}: AD {ad:+.12e} FD {fd:+.12e} rel {abs(fd - ad) / max(abs(ad), 1e-300):.2e} "
                   f"(of the scale {max(abs(ad), gnorm):.3e}: {e:.2e})", flush=True)
         nu = float(F[nue].flat[0])
-        h = EPS * nu
+        h = eps * nu
         fdn = (loss(F[T], F[nue] + h) - loss(F[T], F[nue] - h)) / (2 * h)
         en = abs(fdn - gN) / max(abs(gN), 1e-300)
         print(f"[adjoint] nue: AD {gN:+.12e} FD {fdn:+.12e} rel {en:.2e}; L0 {L0:.12e}; "
@@ -117,6 +119,54 @@ def test_mixed_window_adjoint_tracks_fp64(oracle):
     eg = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
     en = abs(nmx - n64) / abs(n64)
     print(f"[adjoint] mixed vs fp64: loss {abs(lmx - l64) / l64:.2e}, grad T0 {eg:.2e}, grad nue {en:.2e}", flush=True)
+    assert abs(lmx - l64) <= 1e-6 * l64 and eg <= 1e-4 and en <= 1e-4
+
+
+def test_rb_sweep_adjoint_matches_fd(oracle):
+    """The window with the reference's red-black sweeps as its solve (rb_sweeps = 20 per step, omega 1),
+    differentiated sweep by sweep: walls folded back, the black then the red pass transposed."""
+    print("[case] red-black sweep adjoint vs FD, 12x19x37 shell, 6 steps, 20 sweeps per solve", flush=True)
+    g = A.default_grid(1)
+    g.im, g.jm, g.km = 12, 19, 37
+    p = A.default_params(g)
+    F = _state(g, p, oracle)
+    opts = dict(nsteps=6, checkpoint_every=4, rb_sweeps=20)
+    worst, en, gT, gN = _fd_check(g, p, F, opts, ndir=4, seed=13)
+    assert np.isfinite(gT).all() and np.isfinite(gN)
+    assert worst <= BAR and en <= BAR
+
+
+def test_rb_sweep_adjoint_full_shell_matches_fd(oracle):
+    """The same at 41x181x361: 3 steps of 20 sweeps. Here dL/dT0 . d is ~4e-10 against L ~ 2e-2, so the
+    central difference's rounding floor, ~1e-16 L / eps, is 5e-3 of it at eps = 1e-6: the step is
+    eps = 1e-3 (floor ~5e-6; the third-order truncation, ~eps^2, below it)."""
+    print("[case] red-black sweep adjoint vs FD, 41x181x361, 3 steps, 20 sweeps per solve", flush=True)
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F = _state(g, p, oracle)
+    opts = dict(nsteps=3, checkpoint_every=2, rb_sweeps=20)
+    worst, en, gT, gN = _fd_check(g, p, F, opts, ndir=2, seed=17, eps=1e-3)
+    assert worst <= BAR and en <= BAR
+
+
+def test_mixed_rb_window_adjoint_tracks_fp64(oracle):
+    """The red-black window on a mixed context against FP64 (same bars as the converging window)."""
+    print("[case] mixed vs FP64 red-black window adjoint, 12x19x37 shell, 6 steps", flush=True)
+    g = A.default_grid(1)
+    g.im, g.jm, g.km = 12, 19, 37
+    p = A.default_params(g)
+    F = _state(g, p, oracle)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        with A.UranusCore(g, p, prec) as core:
+            core.set_state(F)
+            loss = core.window_forward(nsteps=6, checkpoint_every=4, rb_sweeps=20)
+            res[prec] = (loss,) + core.window_backward(1.0)
+    (l64, g64, n64), (lmx, gmx, nmx) = res["fp64"], res["mixed"]
+    eg = float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))
+    en = abs(nmx - n64) / abs(n64)
+    print(f"[adjoint] red-black mixed vs fp64: loss {abs(lmx - l64) / l64:.2e}, grad T0 {eg:.2e}, grad nue {en:.2e}",
+          flush=True)
     assert abs(lmx - l64) <= 1e-6 * l64 and eg <= 1e-4 and en <= 1e-4
 
 
