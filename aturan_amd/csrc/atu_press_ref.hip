@@ -246,21 +246,63 @@ __global__ void __launch_bounds__(64 * kRows) k_ridft_add(Shape sh, DevSolve t, 
 
 // The interior sum of p in the reference's serial (i, j, k) order, the order its OpenMP reduction
 // takes on one thread (planet/PressureSolver.h:309-316). A sum whose rounding is that of one
-// sequential chain has no parallel form, so one lane runs the chain; the unrolled row loop keeps its
-// loads in flight ahead of the adds. The held mean feeds back into every cell of every sweep, and the
-// barotropic solve magnifies a one-ulp change of it to ~1e-12 of p within 20 sweeps, which is why the
-// order is the reference's rather than a tree's.
+// sequential chain has no parallel form: one wave loads the rows in coalesced chunks of 64 values,
+// kGroupS chunks at a time, into LDS (the next group's loads in flight while the current group is
+// added), and every lane adds the group's values in order from LDS (a broadcast read), so the chain
+// of dependent FP64 adds is the only serial part. The held mean feeds back into every cell of every
+// sweep, and the barotropic solve magnifies a one-ulp change of it to ~1e-12 of p within 20 sweeps,
+// which is why the order is the reference's rather than a tree's.
+constexpr int kGroupS = 8;
 __global__ void __launch_bounds__(64) k_serial_interior_sum(Shape sh, const double* p, double* out) {
-    if (threadIdx.x != 0) return;
-    const int N = sh.km - 1;
-    double s = 0.0;
-    for (int i = 1; i < sh.im - 1; i++)
-        for (int j = 1; j < sh.jm - 1; j++) {
-            const double* row = p + (size_t)i * sh.plane + (size_t)j * sh.ldk;
-#pragma unroll 16
-            for (int k = 0; k < N; k++) s += row[k];
+    __shared__ double buf[2][kGroupS * 64];
+    __shared__ int cnt[2][kGroupS];
+    const int lane = threadIdx.x;
+    const int N = sh.km - 1, nj = sh.jm - 2;
+    const int cpr = (N + 63) / 64;                       // chunks per row
+    const long nch = (long)(sh.im - 2) * nj * cpr;
+    // chunk c: row c / cpr in (i, j) order, values k0 .. k0 + n - 1 of it; lanes beyond n load nothing
+    auto load = [&](long c, int& n) -> double {
+        n = 0;
+        if (c >= nch) return 0.0;
+        const long r = c / cpr;
+        const int k0 = (int)(c - r * cpr) * 64;
+        const int i = 1 + (int)(r / nj), j = 1 + (int)(r % nj);
+        n = min(64, N - k0);
+        const double* row = p + (size_t)i * sh.plane + (size_t)j * sh.ldk;
+        return lane < n ? row[k0 + lane] : 0.0;
+    };
+    auto stash = [&](int b, const double* v, const int* n) {
+        for (int u = 0; u < kGroupS; u++) {
+            buf[b][u * 64 + lane] = v[u];
+            if (lane == 0) cnt[b][u] = n[u];
         }
-    *out = s;
+    };
+    double v[kGroupS];
+    int nv[kGroupS];
+#pragma unroll
+    for (int u = 0; u < kGroupS; u++) v[u] = load(u, nv[u]);
+    stash(0, v, nv);
+    __syncthreads();
+    double s = 0.0;
+    int cur = 0;
+    for (long c = 0; c < nch; c += kGroupS) {
+#pragma unroll
+        for (int u = 0; u < kGroupS; u++) v[u] = load(c + kGroupS + u, nv[u]);   // next group, in flight
+        for (int u = 0; u < kGroupS; u++) {
+            const double* b = &buf[cur][u * 64];
+            const int n = cnt[cur][u];
+            if (n == 64) {
+#pragma unroll 16
+                for (int q = 0; q < 64; q++) s += b[q];
+            } else {
+                for (int q = 0; q < n; q++) s += b[q];
+            }
+        }
+        stash(cur ^ 1, v, nv);
+        __syncthreads();
+        cur ^= 1;
+    }
+    if (lane == 0) *out = s;
 }
 
 }  // namespace
