@@ -92,6 +92,7 @@ struct atu_ctx {
     SolveMailbox* MBOX = nullptr;     // its host copy (pinned, mapped) and the device address of it
     SolveMailbox* MBOX_DEV = nullptr;
     hipEvent_t ev_ctl = nullptr;      // recorded after a batch of iterations; polled by the host
+    hipEvent_t ev_pre = nullptr;      // recorded before a solve's first batch: the host sleeps until it
     // PressureSolverUran: its weight tables, the last stage's rhs_u, rhs_v, rhs_w
     UranTables huran;
     UranTabs duran{};
@@ -197,6 +198,7 @@ atu_ctx::~atu_ctx() {
     if (CTL) hipFree(CTL);
     if (MBOX) hipHostFree(MBOX);
     if (ev_ctl) hipEventDestroy(ev_ctl);
+    if (ev_pre) hipEventDestroy(ev_pre);
     if (uran_blob) hipFree(uran_blob);
     if (pref_blob) hipFree(pref_blob);
     if (RHS3) hipFree(RHS3);
@@ -300,7 +302,8 @@ int upload_solve_tables(atu_ctx* c) {
         c->hip(hipHostMalloc((void**)&c->MBOX, sizeof(SolveMailbox), hipHostMallocMapped | hipHostMallocCoherent),
                "hipHostMalloc(mailbox)") ||
         c->hip(hipHostGetDevicePointer((void**)&c->MBOX_DEV, c->MBOX, 0), "hipHostGetDevicePointer") ||
-        c->hip(hipEventCreateWithFlags(&c->ev_ctl, hipEventDisableTiming), "hipEventCreate"))
+        c->hip(hipEventCreateWithFlags(&c->ev_ctl, hipEventDisableTiming), "hipEventCreate") ||
+        c->hip(hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming), "hipEventCreate"))
         return ATU_ENOMEM;
     return ATU_OK;
 }
@@ -585,21 +588,32 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     auto mark_ctl = [&]() -> int { return H(hipEventRecord(c->ev_ctl, s), "hipEventRecord"); };
     // The wait is bounded: a batch that has not finished after kWaitBase + kWaitPerIter per queued
     // iteration (far beyond any solve of the grids this library accepts) returns ATU_EHIP instead of
-    // waiting for ever. The host spins for the first 100 us of a wait and then sleeps 20 us between
-    // polls, so a long wait does not hold a host core.
+    // waiting for ever. It has two phases: until the solve has started on the device (ev_pre, recorded
+    // before the first batch, behind whatever the stream ran before: the RK stages of a step) the
+    // host sleeps 20 us between polls; from then on, for the few hundred microseconds of the
+    // iterations, it polls with a yield between polls, so that the next step is queued within
+    // microseconds of the batch's end.
     auto read_ctl = [&]() -> int {
         constexpr double kWaitBase = 10.0, kWaitPerIter = 0.05;
         const auto t0 = std::chrono::steady_clock::now();
         const double limit = kWaitBase + kWaitPerIter * max_it;
+        auto expired = [&]() {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > limit;
+        };
+        auto timeout = [&]() {
+            char m[160];
+            std::snprintf(m, sizeof m, "converging solve: a batch of iterations did not finish within %.1f s", limit);
+            return c->fail(ATU_EHIP, m);
+        };
         hipError_t q;
+        while ((q = hipEventQuery(c->ev_pre)) == hipErrorNotReady) {
+            if (expired()) return timeout();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
+        if (H(q, "hipEventQuery")) return ATU_EHIP;
         while ((q = hipEventQuery(c->ev_ctl)) == hipErrorNotReady) {
-            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (el > limit) {
-                char m[160];
-                std::snprintf(m, sizeof m, "converging solve: a batch of iterations did not finish within %.1f s", limit);
-                return c->fail(ATU_EHIP, m);
-            }
-            if (el > 1e-4) std::this_thread::sleep_for(std::chrono::microseconds(20));
+            if (expired()) return timeout();
+            std::this_thread::yield();
         }
         if (H(q, "hipEventQuery")) return ATU_EHIP;
         const volatile SolveMailbox* mb = c->MBOX;
@@ -637,6 +651,7 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         posted = !fixed && flags[CTL_DONE];
         return ATU_OK;
     };
+    if ((e = H(hipEventRecord(c->ev_pre, s), "hipEventRecord"))) return e;
     if (!c->mixed()) {
         const SolveCtl chk{F, c->SCAL + 2, o->rel_tol, 1, 0, c->MBOX_DEV};
         int launched = 0, batch = fixed ? max_it : std::max(1, c->pred_iters);

@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_col_resid_par(Shape sh, Dev
 // The DFT tables depend on (m k mod N) only: ct[m][k] = ct[1][(m k) mod N], the same bits (the host
 // builds every entry from the reduced angle). The DFT kernels stage that one row of N cosines and
 // sines in LDS with the rows of data they need, and index it with the running (m k) mod N.
-constexpr int kDftRows = 4, kDftMaxN = 1024;
+constexpr int kDftMaxN = 1024;
 
 // Copy n values src(q) -> dst[q] with a block of nt threads, kStageBatch loads in flight per thread
 // (the loads of a batch are issued before any of its stores).
@@ -299,38 +299,50 @@ __device__ __forceinline__ void stage_lds(double* dst, int n, int tid, int nt, S
     }
 }
 
-// forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin, each sum
-// over k in order. A block is 64 modes x kDftRows rows (one wave per row).
-__global__ void __launch_bounds__(64 * kDftRows) k_dft_fwd(DevSolve t, const double* R, double* Ac, double* As,
-                                                           const int* done) {
+// forward real DFT: Ac[m][jj] = sum_k R[jj][k] cos(2 pi m k / N), As likewise with sin. Each sum in
+// the order shared with the oracle (dft_sum16): 16 partials, partial l adding the terms k = l, l + 16,
+// ... in sequence from 0, then a halving tree over the 16. A group of 16 lanes forms one (m, jj);
+// the groups of a block are consecutive modes of one row, which share the row's loads; the twiddles
+// cos / sin(2 pi q / N), q = m k mod N, come from one table row in LDS.
+constexpr int kDftLanes = 16, kDftGroups = 16;   // groups of a 256-thread block
+__device__ __forceinline__ double tree16(double v) {
+    for (int h = kDftLanes / 2; h > 0; h >>= 1) v += __shfl_down(v, h, kDftLanes);
+    return v;
+}
+__global__ void __launch_bounds__(kDftLanes * kDftGroups) k_dft_fwd(DevSolve t, const double* R, double* Ac,
+                                                                     double* As, const int* done) {
     if (done && *done) return;
-    __shared__ double c1[kDftMaxN], s1[kDftMaxN], rr[kDftRows][kDftMaxN];
+    __shared__ double c1[kDftMaxN], s1[kDftMaxN], r[kDftMaxN];
     const int N = t.N, M1 = t.NM + 1, J = t.J;
-    const int tid = threadIdx.y * 64 + threadIdx.x;
-    const int jj0 = blockIdx.y * kDftRows;
-    const int nt = 64 * kDftRows;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int mblocks = (M1 + kDftGroups - 1) / kDftGroups;
+    const int jj = blockIdx.x / mblocks;
     stage_lds(c1, N, tid, nt, [&](int q) { return t.ct[(size_t)N + q]; });
     stage_lds(s1, N, tid, nt, [&](int q) { return t.st[(size_t)N + q]; });
-    stage_lds(&rr[0][0], kDftRows * kDftMaxN, tid, nt, [&](int q) {
-        const int r = q / kDftMaxN, k = q - r * kDftMaxN;
-        return (jj0 + r < J && k < N) ? R[(size_t)(jj0 + r) * N + k] : 0.0;
-    });
+    stage_lds(r, N, tid, nt, [&](int q) { return R[(size_t)jj * N + q]; });   // the block's row
     __syncthreads();
-    const int m = blockIdx.x * 64 + threadIdx.x;
-    const int r = threadIdx.y;
-    if (m >= M1 || jj0 + r >= J) return;
+    const int l = tid & (kDftLanes - 1), g = tid / kDftLanes;
+    const int m = (blockIdx.x - jj * mblocks) * kDftGroups + g;
+    const bool on = m < M1 && jj < J;
     double sc = 0.0, ss = 0.0;
-    int q = 0;   // (m k) mod N
-#pragma unroll 8
-    for (int k = 0; k < N; k++) {
-        const double rv = rr[r][k];
-        sc += rv * c1[q];
-        ss += rv * s1[q];
-        q += m;
-        if (q >= N) q -= N;
+    if (on) {
+        const int step = (m * kDftLanes) % N;
+        int q = (m * l) % N;   // (m k) mod N
+#pragma unroll 4
+        for (int k = l; k < N; k += kDftLanes) {
+            const double rv = r[k];
+            sc += rv * c1[q];
+            ss += rv * s1[q];
+            q += step;
+            if (q >= N) q -= N;
+        }
     }
-    Ac[(size_t)m * J + jj0 + r] = sc;
-    As[(size_t)m * J + jj0 + r] = ss;
+    sc = tree16(sc);
+    ss = tree16(ss);
+    if (on && l == 0) {
+        Ac[(size_t)m * J + jj] = sc;
+        As[(size_t)m * J + jj] = ss;
+    }
 }
 
 // Per mode and part: the folded theta system, mode 0 pinned at j = 1. One 64-lane block per
@@ -381,43 +393,46 @@ __global__ void __launch_bounds__(64) k_baro_tri(DevSolve t, double c13, double*
     for (int jj = threadIdx.x; jj < J; jj += blockDim.x) d[jj] = sd[jj];
 }
 
-// inverse DFT of the correction: Q[jj][k] = (Ac[0] c_0 + sum_{m=1}^{NM-1} 2 (Ac c_m + As s_m) +
-// Ac[NM] c_NM) / N, the modes in order; a block is 64 k x kDftRows rows (one wave per row), with the
-// coefficients of its rows and the one table row in LDS. Q may overwrite the column residual.
-constexpr int kDftMaxM = 1024;
-__global__ void __launch_bounds__(64 * kDftRows) k_dft_inv(DevSolve t, const double* Ac, const double* As, double* Q,
-                                                           const int* done) {
+// inverse DFT of the correction: Q[jj][k] = (sum over m = 0..NM of t_m) / N with t_0 = Ac[0] c_0,
+// t_m = 2 (Ac[m] c_m + As[m] s_m) for 0 < m < NM and t_NM = Ac[NM] c_NM (N even), summed in the
+// order shared with the oracle: 16 partials over the modes m = l, l + 16, ..., then a halving tree.
+// A group of 16 lanes forms one (jj, k); the groups of a block are consecutive k of one row. Q may
+// overwrite the column residual.
+__global__ void __launch_bounds__(kDftLanes * kDftGroups) k_dft_inv(DevSolve t, const double* Ac, const double* As,
+                                                                     double* Q, const int* done) {
     if (done && *done) return;
-    __shared__ double c1[kDftMaxN], s1[kDftMaxN], ta[kDftRows][kDftMaxM / 2 + 1], tb[kDftRows][kDftMaxM / 2 + 1];
+    __shared__ double c1[kDftMaxN], s1[kDftMaxN], ta[kDftMaxN / 2 + 1], tb[kDftMaxN / 2 + 1];
     const int N = t.N, NM = t.NM, J = t.J;
-    const int tid = threadIdx.y * 64 + threadIdx.x;
-    const int jj0 = blockIdx.y * kDftRows;
-    const int nt = 64 * kDftRows;
-    constexpr int MW = kDftMaxM / 2 + 1;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int kblocks = (N + kDftGroups - 1) / kDftGroups;
+    const int jj = blockIdx.x / kblocks;
     stage_lds(c1, N, tid, nt, [&](int q) { return t.ct[(size_t)N + q]; });
     stage_lds(s1, N, tid, nt, [&](int q) { return t.st[(size_t)N + q]; });
-    stage_lds(&ta[0][0], kDftRows * MW, tid, nt, [&](int q) {
-        const int r = q / MW, m = q - r * MW;
-        return (jj0 + r < J && m <= NM) ? Ac[(size_t)m * J + jj0 + r] : 0.0;
-    });
-    stage_lds(&tb[0][0], kDftRows * MW, tid, nt, [&](int q) {
-        const int r = q / MW, m = q - r * MW;
-        return (jj0 + r < J && m <= NM) ? As[(size_t)m * J + jj0 + r] : 0.0;
-    });
+    stage_lds(ta, NM + 1, tid, nt, [&](int m) { return Ac[(size_t)m * J + jj]; });   // the block's row's modes
+    stage_lds(tb, NM + 1, tid, nt, [&](int m) { return As[(size_t)m * J + jj]; });
     __syncthreads();
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int r = threadIdx.y;
-    if (k >= N || jj0 + r >= J) return;
-    double s = ta[r][0] * c1[0];
-    int q = k;   // (m k) mod N
+    const int l = tid & (kDftLanes - 1), g = tid / kDftLanes;
+    const int k = (blockIdx.x - jj * kblocks) * kDftGroups + g;
+    const bool on = k < N && jj < J;
+    double s = 0.0;
+    if (on) {
+        const int step = (k * kDftLanes) % N;
+        int q = (l * k) % N;   // (m k) mod N
 #pragma unroll 4
-    for (int m = 1; m < NM; m++) {
-        s += 2.0 * (ta[r][m] * c1[q] + tb[r][m] * s1[q]);
-        q += k;
-        if (q >= N) q -= N;
+        for (int m = l; m <= NM; m += kDftLanes) {
+            const double a = ta[m];
+            double tm;
+            if (m == 0 || m == NM)
+                tm = a * c1[q];
+            else
+                tm = 2.0 * (a * c1[q] + tb[m] * s1[q]);
+            s += tm;
+            q += step;
+            if (q >= N) q -= N;
+        }
     }
-    s += ta[r][NM] * c1[q];
-    Q[(size_t)(jj0 + r) * N + k] = s / N;
+    s = tree16(s);
+    if (on && l == 0) Q[(size_t)jj * N + k] = s / N;
 }
 
 // the correction added to every interior level: p += Q on the interior
@@ -668,7 +683,7 @@ template <typename P, typename B>
 hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, double* R, double* Ac,
                        double* As, hipStream_t s, const int* done) {
     if (t.J > kBaroMaxJ) return hipErrorInvalidValue;   // the theta solves stage a mode in LDS
-    if (t.N > kDftMaxN || t.NM + 1 > kDftMaxM / 2 + 1 || t.NM < 1) return hipErrorInvalidValue;   // the DFTs' LDS
+    if (t.N > kDftMaxN || t.NM < 1 || 2 * t.NM != t.N) return hipErrorInvalidValue;   // LDS tables; N even
     const int N = sh.km - 1;
     const size_t lds = (size_t)(sh.im - 2) * 64 * sizeof(double);
     if (lds <= 48 * 1024)
@@ -677,11 +692,11 @@ hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p
     else
         hipLaunchKernelGGL((k_col_resid<P, B>), dim3((N + 63) / 64, sh.jm - 2), dim3(64), 0, s, sh, g, t, (const P*)p,
                            b, R, done);
-    hipLaunchKernelGGL(k_dft_fwd, dim3((t.NM + 1 + 63) / 64, (t.J + kDftRows - 1) / kDftRows), dim3(64, kDftRows), 0,
-                       s, t, (const double*)R, Ac, As, done);
+    hipLaunchKernelGGL(k_dft_fwd, dim3(((t.NM + 1 + kDftGroups - 1) / kDftGroups) * t.J), dim3(kDftLanes * kDftGroups),
+                       0, s, t, (const double*)R, Ac, As, done);
     hipLaunchKernelGGL(k_baro_tri, dim3(2 * (t.NM + 1)), dim3(64), 0, s, t, 1.0 / 3.0, Ac, As, done);
-    hipLaunchKernelGGL(k_dft_inv, dim3((N + 63) / 64, (t.J + kDftRows - 1) / kDftRows), dim3(64, kDftRows), 0, s, t,
-                       (const double*)Ac, (const double*)As, R, done);
+    hipLaunchKernelGGL(k_dft_inv, dim3(((N + kDftGroups - 1) / kDftGroups) * t.J), dim3(kDftLanes * kDftGroups), 0, s,
+                       t, (const double*)Ac, (const double*)As, R, done);
     hipLaunchKernelGGL((k_add_q<P>), dim3((N + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0, s, sh,
                        (const double*)R, p, done);
     return hipGetLastError();

@@ -1195,6 +1195,14 @@ static void baro_init(orc_baro* B, const orc_geo* G, const orc_pgeo* q, const at
 
 static void baro_free(orc_baro* B) { free(B->ct); free(B->st); free(B->cpm); free(B->dnm); }
 
+/* The sum of 16 partials by a halving tree: p[l] += p[l + h] for h = 8, 4, 2, 1 (the order of a
+ * 16-lane shuffle-down reduction). */
+static double tree16(double* p) {
+    for (int h = 8; h > 0; h >>= 1)
+        for (int l = 0; l < h; l++) p[l] = p[l] + p[l + h];
+    return p[0];
+}
+
 /* one barotropic correction: r is a scratch field */
 static void baro_correct(const orc_baro* B, const orc_geo* G, const orc_pgeo* q, double* pd, const double* b,
                          double* r) {
@@ -1211,16 +1219,18 @@ static void baro_correct(const orc_baro* B, const orc_geo* G, const orc_pgeo* q,
             for (int i = 1; i < im - 1; i++) s += endw(i, 1, im - 2) * r[IDX(i, jj + 1, k)];
             R[(size_t)jj * N + k] = s / B->S2;
         }
+    /* the DFTs' sums in the device's order: 16 partials (partial l adds the terms l, l + 16, ... in
+       sequence from 0), then a halving tree over the 16 */
 #pragma omp parallel for collapse(2) schedule(static)
     for (int m = 0; m <= NM; m++)
         for (int jj = 0; jj < J; jj++) {
-            double sc = 0.0, ss = 0.0;
+            double pc[16] = {0.0}, ps[16] = {0.0};
             for (int k = 0; k < N; k++) {
-                sc += R[(size_t)jj * N + k] * B->ct[(size_t)m * N + k];
-                ss += R[(size_t)jj * N + k] * B->st[(size_t)m * N + k];
+                pc[k & 15] += R[(size_t)jj * N + k] * B->ct[(size_t)m * N + k];
+                ps[k & 15] += R[(size_t)jj * N + k] * B->st[(size_t)m * N + k];
             }
-            Ac[(size_t)m * J + jj] = sc;
-            As[(size_t)m * J + jj] = ss;
+            Ac[(size_t)m * J + jj] = tree16(pc);
+            As[(size_t)m * J + jj] = tree16(ps);
         }
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < 2 * (NM + 1); t++) {
@@ -1249,12 +1259,14 @@ static void baro_correct(const orc_baro* B, const orc_geo* G, const orc_pgeo* q,
 #pragma omp parallel for collapse(2) schedule(static)
     for (int jj = 0; jj < J; jj++)
         for (int k = 0; k < N; k++) {
-            double s = Ac[jj] * B->ct[k];
-            for (int m = 1; m < NM; m++)
-                s += 2.0 * (Ac[(size_t)m * J + jj] * B->ct[(size_t)m * N + k]
-                            + As[(size_t)m * J + jj] * B->st[(size_t)m * N + k]);
-            s += Ac[(size_t)NM * J + jj] * B->ct[(size_t)NM * N + k];
-            const double qv = s / N;
+            double pt[16] = {0.0};
+            for (int m = 0; m <= NM; m++) {
+                const double a = Ac[(size_t)m * J + jj], c = B->ct[(size_t)m * N + k];
+                const double tm = (m == 0 || m == NM) ? a * c
+                                                      : 2.0 * (a * c + As[(size_t)m * J + jj] * B->st[(size_t)m * N + k]);
+                pt[m & 15] += tm;
+            }
+            const double qv = tree16(pt) / N;
             for (int i = 0; i < im; i++) pd[IDX(i, jj + 1, k)] += qv;
         }
     free(R); free(Ac); free(As);
