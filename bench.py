@@ -505,13 +505,48 @@ def config1(g, p, F0, gpu_steps, cpu_steps=3):
         b = F[A.FIELD[n]]
         s = float(np.max(np.abs(b)))
         err[n] = float(np.max(np.abs(a - b))) / (s if s > 0 else 1.0)
+    ref = reference_config1(g, F0)
     return {"workload": "41x181x361: hydro prep + RK4 + BC + species clamp + 20 red-black SOR sweeps, omega 1.5",
+            "cpu_reference": ref,
             "gpu": {"steps": gpu_steps, "steps_per_s": round(gpu_steps / el, 2),
                     "ms_per_step": round(1e3 * el / gpu_steps, 4),
                     "cell_updates_per_s": round(n_int * gpu_steps / el, 1)},
             "cpu": {"steps": cpu_steps, "steps_per_s": round(cpu_steps / cel, 4), "cores": cpu_threads(),
                     "kind": "port", "cpu": cpu_model()},
             "gpu_vs_oracle_after_cpu_steps": {"max_rel": err, "worst": max(err.values())}}
+
+
+REF_HARNESS = ROOT / "oracle" / "_ref" / "ref_harness"
+
+
+def reference_config1(g, F0, steps=2):
+    """The compiled reference (oracle/_ref/ref_harness, built from the reference's own sources by
+    oracle/build_ref.sh at -O2) running config 1's step as its loop does (restoreVar, RungeKuttaUran,
+    BC, clampNegativeSpecies, PressureSolver with ATURAN_PRESS_SWEEPS=20; the reference's sweep has no
+    relaxation factor, so omega is 1) on the same state, on the host's cores; the harness times each
+    step. None when the harness is not built or its grid differs."""
+    import tempfile
+    import numpy as np
+    if not REF_HARNESS.exists():
+        return None
+    threads = cpu_threads()
+    with tempfile.TemporaryDirectory() as td:
+        fin = Path(td) / "in.bin"
+        with open(fin, "wb") as f:
+            np.array([g.im, g.jm, g.km, F0.shape[0]], dtype=np.int32).tofile(f)
+            np.ascontiguousarray(F0, dtype=np.float64).tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([str(REF_HARNESS), str(fin), "/dev/null", "step", str(steps), "20"], capture_output=True,
+                           text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-300:]}
+    secs = [float(ln.split()[2]) for ln in r.stderr.splitlines() if ln.startswith("ref_step_seconds")]
+    if not secs:
+        return None
+    best = min(secs[1:] or secs)      # after the first step, which pays the first touch of the arrays
+    return {"steps": len(secs), "seconds_per_step": [round(x, 4) for x in secs], "steps_per_s": round(1.0 / best, 4),
+            "cores": threads, "kind": "reference", "omega": 1.0, "build": "g++ -O2 -fno-fast-math -fopenmp",
+            "cpu": cpu_model()}
 
 
 def cpu_one_thread(tol):

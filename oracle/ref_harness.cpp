@@ -1,5 +1,7 @@
 // TEST INFRASTRUCTURE ONLY. Drives the reference's own hot-path member functions, compiled from
-// the reference sources where they lie (see build_ref.sh), to write golden vectors. Nothing from the
+// the reference sources where they lie (see build_ref.sh), to write golden vectors and, for the
+// benchmark's CPU baseline of its config-1 step, to time them (stderr: ref_step_seconds per step in
+// mode step, ref_mode_seconds for the mode's work; no file I/O inside either). Nothing from the
 // reference is copied: this file only constructs the model, fills its arrays from an input file,
 // calls the reference functions in the benchmark's order and dumps the arrays.
 //
@@ -153,6 +155,8 @@ int main(int argc, char** argv) {
     };
     // the model sets the wiggle filter's seam convention in Run() (planet/cUranusModel.cpp:459)
     AtomUtils::set_phi_seam_periodic(ATPhys::phi_periodic<cUranusModel>());
+    // the wall time of the mode's work alone (no file I/O), and of each step in mode step
+    const auto t_mode = std::chrono::steady_clock::now();
     if (mode == "bc") {
         bc();
     } else if (mode == "bc_clamp") {
@@ -165,11 +169,14 @@ int main(int argc, char** argv) {
         PressureSolver<cUranusModel>(m).run();
     } else if (mode == "step") {
         for (int s = 0; s < steps; s++) {
+            const auto t0 = std::chrono::steady_clock::now();
             m.restoreVar(1.0);
             m.RungeKuttaUran();
             bc();
             m.clampNegativeSpecies();
             PressureSolver<cUranusModel>(m).run();
+            std::fprintf(stderr, "ref_step_seconds %d %.6f\n", s,
+                         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         }
     } else if (mode == "rk4_uran") {
         m.restoreVar(1.0);
@@ -227,6 +234,9 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
         return 2;
     }
+
+    std::fprintf(stderr, "ref_mode_seconds %.6f\n",
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t_mode).count());
 
     std::ofstream out(argv[2], std::ios::binary);
     out.write((const char*)hdr, sizeof hdr);
