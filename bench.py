@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--warm", type=int, choices=[0, 1, 2], default=2,
                     help="solve start: 0 p_n, 1 2 p_n - p_(n-1), 2 3 p_n - 3 p_(n-1) + p_(n-2) (default)")
     ap.add_argument("--cpu-one-thread", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--config1-one-thread", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-events", action="store_true",
                     help="no HIP events in the timed region and no per-class pass (for runs under a profiler)")
     ap.add_argument("--adjoint-steps", type=int, default=50, help="window of the adjoint timing (0: skip)")
@@ -473,11 +474,13 @@ def fp64_vs_oracle(g, p, F0, Fo, n, opts):
     return {"steps": n, "max_rel": out, "worst": max(out.values()), "bar": 1e-12}
 
 
-def config1(g, p, F0, gpu_steps, cpu_steps=3):
+def config1(g, p, F0, gpu_steps, cpu_steps=5):
     """The reference's CPU configuration: one RK4 step (hydro prep, four stages, BC and species clamp),
     then 20 red-black SOR sweeps at omega = 1.5. GPU: gpu_steps such steps in one call after 3 warm-up
-    steps; CPU: the oracle, cpu_steps steps (one call) after one warm-up step on a copy; the GPU's
-    state after cpu_steps steps from the initial state against the oracle's."""
+    steps; CPU: the oracle on the host's cores, one warm-up step on a copy, then cpu_steps steps from
+    the initial state, each timed alone (the median is reported), and the same on one thread in a child
+    process (one warm-up step, the median of 3); the GPU's state after cpu_steps steps from the initial
+    state against the oracle's; and the compiled reference's own step."""
     import numpy as np
     import aturan_amd as A
     from oracle import binding as O
@@ -497,9 +500,20 @@ def config1(g, p, F0, gpu_steps, cpu_steps=3):
     O.step(g, p, Fw, 1, 20, 1.5)
     del Fw
     F = F0.copy()
-    t0 = time.perf_counter()
-    O.step(g, p, F, cpu_steps, 20, 1.5)
-    cel = time.perf_counter() - t0
+    times = []
+    for _ in range(cpu_steps):
+        t0 = time.perf_counter()
+        O.step(g, p, F, 1, 20, 1.5)
+        times.append(time.perf_counter() - t0)
+    cel = sorted(times)[len(times) // 2]
+    one = None
+    try:
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--config1-one-thread"], env=env,
+                           capture_output=True, text=True, timeout=240)
+        one = json.loads(r.stdout.strip().splitlines()[-1])["steps_per_s"] if r.returncode == 0 else None
+    except (subprocess.TimeoutExpired, ValueError, IndexError, KeyError):
+        one = None
     err = {}
     for n, a in G.items():
         b = F[A.FIELD[n]]
@@ -511,8 +525,9 @@ def config1(g, p, F0, gpu_steps, cpu_steps=3):
             "gpu": {"steps": gpu_steps, "steps_per_s": round(gpu_steps / el, 2),
                     "ms_per_step": round(1e3 * el / gpu_steps, 4),
                     "cell_updates_per_s": round(n_int * gpu_steps / el, 1)},
-            "cpu": {"steps": cpu_steps, "steps_per_s": round(cpu_steps / cel, 4), "cores": cpu_threads(),
-                    "kind": "port", "cpu": cpu_model()},
+            "cpu": {"steps": cpu_steps, "steps_per_s": round(1.0 / cel, 4), "seconds_per_step": [round(x, 4) for x in times],
+                    "statistic": "median of single-step calls after one warm-up step", "cores": cpu_threads(),
+                    "one_thread_steps_per_s": one, "kind": "port", "cpu": cpu_model()},
             "gpu_vs_oracle_after_cpu_steps": {"max_rel": err, "worst": max(err.values())}}
 
 
@@ -549,6 +564,24 @@ def reference_config1(g, F0, steps=2):
             "cpu": cpu_model()}
 
 
+def config1_one_thread():
+    """Config 1's step in the oracle on this process's OpenMP threads (run with OMP_NUM_THREADS=1):
+    one warm-up step, then the median of 3 single-step calls."""
+    import aturan_amd as A
+    from oracle import binding as O
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F = A.synthetic_state(g, p, 20261203)
+    O.bc(g, p, F)
+    O.step(g, p, F, 1, 20, 1.5)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.step(g, p, F, 1, 20, 1.5)
+        ts.append(time.perf_counter() - t0)
+    print(json.dumps({"steps_per_s": round(1.0 / sorted(ts)[1], 4)}), flush=True)
+
+
 def cpu_one_thread(tol):
     import aturan_amd as A
     from oracle import binding as O
@@ -568,6 +601,9 @@ def progress(msg):
 
 def main():
     args = parse()
+    if args.config1_one_thread:
+        config1_one_thread()
+        return
     if args.cpu_one_thread:
         cpu_one_thread(args.tol)
         return
