@@ -382,6 +382,43 @@ __global__ void __launch_bounds__(64 * TJ) k_rk_stage(const StageArgs<S> a) {
 
 // blocks of 64 k x 4 rows (taller blocks, 64 x 8 and 64 x 12, and the XCD-aware block orders that
 // were measured against it ran slower)
+// The same stage with the arrays that are never written through another pointer passed as restrict
+// kernel arguments (the stage input, the next stage input, the accumulator, the frozen inputs, the
+// pressures, aux), so that the compiler may issue a field's loads ahead of the previous field's
+// stores. Measured: the FP32 stage 0.335 -> 0.325 ms, the FP64 stage 0.517 -> 0.530 ms (the FP64
+// kernel's schedule gets longer), so only the mixed path launches it. y_n is left out: the last stage
+// writes y_{n+1} over it.
+template <typename S, int STAGE, int TJ>
+__global__ void __launch_bounds__(64 * TJ) k_rk_stage_noalias(const StageArgs<S> a0, const S* __restrict__ xs,
+                                                         S* __restrict__ xnext, double* __restrict__ acc,
+                                                         const S* __restrict__ frozen,
+                                                         const double* __restrict__ pdyn,
+                                                         const double* __restrict__ phydro, S* __restrict__ aux) {
+    using R = S;
+    StageArgs<S> a = a0;
+    a.xs = xs;
+    a.xnext = xnext;
+    a.acc = acc;
+    a.frozen = frozen;
+    a.pdyn = pdyn;
+    a.phydro = phydro;
+    a.aux = aux;
+    const int nk = a.sh.km - 1;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = 1 + blockIdx.y * TJ + threadIdx.y;
+    const int i = 1 + blockIdx.z;
+    if (k >= nk || j > a.sh.jm - 2) return;
+    const int ldk = a.sh.ldk;
+    const size_t row = ((size_t)i * a.sh.jm + j) * ldk;
+    const size_t c = row + k;
+    const size_t cw = row + (k == 0 ? nk - 1 : k - 1);
+    const size_t ce = row + (k == nk - 1 ? 0 : k + 1);
+    const bool pole = a.geo.pole_row[j] != 0.0;
+    const NbGlobal<R, S> nb{a.xs, a.pdyn, a.phydro, c, cw, ce, a.sh.plane, a.sh.fs, ldk, inv_of<R>(a)};
+    tend_dyn<R, S, STAGE>(a, i, j, c, nb, pole);
+    tend_sca<R, S, STAGE>(a, i, j, c, nb, pole);
+}
+
 constexpr int kTJ = 4;
 
 template <typename S>
@@ -404,7 +441,20 @@ hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_
     return launch_stage<double>(stage, a, s);
 }
 hipError_t launch_rk_stage_mixed(int stage, const StageArgs<float>& a, hipStream_t s) {
-    return launch_stage<float>(stage, a, s);
+    const int nk = a.sh.km - 1;
+    const dim3 block(64, kTJ, 1);
+    const dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + kTJ - 1) / kTJ, a.sh.im - 2);
+#define ATU_NOALIAS(ST)                                                                                          \
+    hipLaunchKernelGGL((k_rk_stage_noalias<float, ST, kTJ>), grid, block, 0, s, a, a.xs, a.xnext, a.acc, a.frozen, \
+                       a.pdyn, a.phydro, a.aux)
+    switch (stage) {
+        case 0: ATU_NOALIAS(0); break;
+        case 1: ATU_NOALIAS(1); break;
+        case 2: ATU_NOALIAS(2); break;
+        default: ATU_NOALIAS(3); break;
+    }
+#undef ATU_NOALIAS
+    return hipGetLastError();
 }
 
 }  // namespace atu
