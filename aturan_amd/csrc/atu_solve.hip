@@ -121,13 +121,25 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_line_par(Shape sh, DevGeo g
                                                               int colour, const int* done) {
     if (done && *done) return;
     extern __shared__ unsigned char line_lds[];
-    R* sv = reinterpret_cast<R*>(line_lds);   // [ni][64]
+    const int ni = t.ni;
+    R* sv = reinterpret_cast<R*>(line_lds);                                        // [ni][64]
+    double* scp = reinterpret_cast<double*>(line_lds + (size_t)ni * 64 * sizeof(R));   // the row's factors
+    double* sdn = scp + ni;
     const int N = sh.km - 1;
     const int j = 1 + blockIdx.y;
     const int lane = threadIdx.x, w = threadIdx.y;
     const int k = 2 * (int)(blockIdx.x * 64 + lane) + ((colour + j) & 1);
     const bool on = k < N;
-    const int ldk = sh.ldk, jm = sh.jm, ni = t.ni;
+    const int ldk = sh.ldk, jm = sh.jm;
+    {   // the row's Thomas factors into LDS, off the recurrences' chain
+        constexpr bool kDivF = sizeof(R) == sizeof(double);
+        const double* cpg = t.line_cp + (size_t)j * ni;
+        const double* dng = (kDivF ? t.line_dn : t.line_inv_dn) + (size_t)j * ni;
+        for (int q = w * 64 + lane; q < ni; q += 64 * kLineWaves) {
+            scp[q] = cpg[q];
+            sdn[q] = dng[q];
+        }
+    }
     const R c13 = R(1.0) / R(3.0);
     const int kw = (k == 0) ? N - 1 : k - 1, ke = (k == N - 1) ? 0 : k + 1;
     const size_t stride = sh.plane;
@@ -151,8 +163,8 @@ __global__ void __launch_bounds__(64 * kLineWaves) k_line_par(Shape sh, DevGeo g
     __syncthreads();
     if (w == 0 && on) {
         constexpr bool kDiv = sizeof(R) == sizeof(double);
-        const double* cp = t.line_cp + (size_t)j * ni;
-        const double* dn = (kDiv ? t.line_dn : t.line_inv_dn) + (size_t)j * ni;
+        const double* cp = scp;
+        const double* dn = sdn;
         const R num1 = (R)g.num1;
         R dprev = R(0.0);
 #pragma unroll 4
@@ -656,7 +668,7 @@ template <typename P, typename B, typename R>
 hipError_t launch_line(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p, const B* b, int colour,
                        hipStream_t s, const int* done) {
     const int half = (sh.km - 1 + 1) / 2;
-    const size_t lds = (size_t)t.ni * 64 * sizeof(R);
+    const size_t lds = (size_t)t.ni * 64 * sizeof(R) + 2 * (size_t)t.ni * sizeof(double);
     if (lds <= 48 * 1024)
         hipLaunchKernelGGL((k_line_par<P, B, R>), dim3((half + 63) / 64, sh.jm - 2), dim3(64, kLineWaves), lds, s, sh,
                            g, t, p, b, colour, done);
