@@ -91,18 +91,26 @@ __device__ __forceinline__ void fold(const StageArgs<S>& a, NB& nb, int f, size_
 }
 
 // cp_lapse(T) / cp_mix ratio input: CpH2::Lapse::cp (planet/CpH2.h:88-93,128-134) on the
-// host-built H2 table.
+// host-built H2 table: the table interval of T, then the interpolation on it.
+__device__ __forceinline__ double cp_x(double T) {
+    const double Tc = (T < 5.0 ? 5.0 : T);
+    return ((Tc > 1500.0 ? 1500.0 : Tc) - 5.0) / 0.5;
+}
+__device__ __forceinline__ int cp_i(double x, int n) {
+    const int ix = (int)x;
+    return (n - 2 < ix) ? n - 2 : ix;
+}
+__device__ __forceinline__ double cp_value(const StepConsts& c, double x, int i, double t0, double t1) {
+    const double f = x - i;
+    const double ch = t0 + f * (t1 - t0);
+    return (c.r_h2 * ch + c.cp_a + c.cp_b) / c.cp_den;
+}
 template <typename S>
 __device__ __forceinline__ double cp_lapse_dev(const StageArgs<S>& a, double T) {
     const double* t = a.geo.cp_tab;
-    const int n = a.geo.cp_n;
-    const double Tc = (T < 5.0 ? 5.0 : T);
-    const double x = ((Tc > 1500.0 ? 1500.0 : Tc) - 5.0) / 0.5;
-    const int ix = (int)x;
-    const int i = (n - 2 < ix) ? n - 2 : ix;
-    const double f = x - i;
-    const double ch = t[i] + f * (t[i + 1] - t[i]);
-    return (a.c.r_h2 * ch + a.c.cp_a + a.c.cp_b) / a.c.cp_den;
+    const double x = cp_x(T);
+    const int i = cp_i(x, a.geo.cp_n);
+    return cp_value(a.c, x, i, t[i], t[i + 1]);
 }
 
 // A metric factor that carries 1/sin(theta) (1/(rm sin), 1/(rm^2 sin), 1/(rm^2 sin^2),
@@ -208,9 +216,9 @@ __device__ __forceinline__ void tend_dyn(const StageArgs<S>& a, const CellGeo<R>
     {
         nb.template item<4>();
         const D7<R> T = nb.st(ATU_T);
-        R pressure_t = (R)C.coeff_energy_p * scale_p * (u0 * dpdr + v_invrm * dpdthe + w_invrs * dpdphi);
-        if ((C.cp_mode != 0 || C.cp_gas != 0) && T.c > R(0.0))
-            pressure_t *= (R)(C.cp_mix / cp_lapse_dev(a, (double)T.c * C.t_ref));
+        // cp(T) of the pressure work: the source may start its table reads here and finish them below
+        const bool cp_on = (C.cp_mode != 0 || C.cp_gas != 0) && T.c > R(0.0);
+        if (cp_on) nb.cp_request(a, (double)T.c * C.t_ref);
         const R transport_t = u0 * T.d1r + v_invrm * T.d1t + w_invrs * T.d1p;
         const R diffusion_t = T.d2r + T.d1r * two_inv_rm + T.d2t * inv_rm2 + mf(T.d1t, cirs, pole)
                               + mf(T.d2p, ir2s2, pole);
@@ -218,8 +226,10 @@ __device__ __forceinline__ void tend_dyn(const StageArgs<S>& a, const CellGeo<R>
         if (!isfinite(precip_t)) precip_t = R(0.0);
         else if (precip_t > R(0.5)) precip_t = R(0.5);
         else if (precip_t < R(-0.5)) precip_t = R(-0.5);
-        const R rhs_t = +pressure_t - transport_t + diffusion_t / (R)C.re_pr + diffusion_t * nue_t_s + precip_t;
         nb.template item<5>();
+        R pressure_t = (R)C.coeff_energy_p * scale_p * (u0 * dpdr + v_invrm * dpdthe + w_invrs * dpdphi);
+        if (cp_on) pressure_t *= (R)(C.cp_mix / nb.cp_lapse(a, (double)T.c * C.t_ref));
+        const R rhs_t = +pressure_t - transport_t + diffusion_t / (R)C.re_pr + diffusion_t * nue_t_s + precip_t;
         fold<STAGE, CL_T>(a, nb, ATU_T, c, nb.template y0<STAGE>(a, ATU_T, c, T.c), (double)rhs_t);
     }
 
@@ -394,6 +404,8 @@ struct NbGlobal {
     __device__ __forceinline__ double acc(const StageArgs<S>& a, int, size_t q) const { return nt_ld(a.acc + q); }
     template <typename T>
     __device__ __forceinline__ T* out(T* p) const { return p; }
+    __device__ __forceinline__ void cp_request(const StageArgs<S>&, double) const {}
+    __device__ __forceinline__ double cp_lapse(const StageArgs<S>& a, double T) const { return cp_lapse_dev(a, T); }
 };
 
 // ---- the LDS-ring source (FP64) ----
@@ -590,6 +602,27 @@ struct NbRing {
     }
     template <typename T>
     __device__ __forceinline__ T* out(T* p) const { return valid ? p : (T*)(g_ring_sink + lane); }
+    // The cp table pair (t[i], t[i + 1]) of the cell, requested during group 4 as four 4-byte loads per
+    // lane into segment P1 of group 5's slot (free in every stage: group 5 fills R1 R2 C P0 at most),
+    // and read when group 5 is current (the loads precede group 6's, so item<5>'s wait covers them).
+    // Only lanes with cp_on request; only they read.
+    __device__ __forceinline__ void cp_request(const StageArgs<double>& aa, double T) const {
+        const int i = cp_i(cp_x(T), aa.geo.cp_n);
+        const char* src = (const char*)(aa.geo.cp_tab + i);
+        unsigned* d = (unsigned*)(slot(5) + SEG_P1);
+        __builtin_amdgcn_global_load_lds((const void*)(src + 0), (lds_void_ptr)(d + 0), 4, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(src + 4), (lds_void_ptr)(d + 64), 4, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(src + 8), (lds_void_ptr)(d + 128), 4, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(src + 12), (lds_void_ptr)(d + 192), 4, 0, 0);
+    }
+    __device__ __forceinline__ double cp_lapse(const StageArgs<double>& aa, double T) const {
+        const double x = cp_x(T);
+        const int i = cp_i(x, aa.geo.cp_n);
+        const unsigned* d = (const unsigned*)(cur + SEG_P1);
+        const double t0 = __hiloint2double((int)d[64 + lane], (int)d[lane]);
+        const double t1 = __hiloint2double((int)d[192 + lane], (int)d[128 + lane]);
+        return cp_value(aa.c, x, i, t0, t1);
+    }
 };
 
 template <typename R, typename S>
