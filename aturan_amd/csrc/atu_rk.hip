@@ -13,8 +13,6 @@
 // read and written with non-temporal hints, so that they do not push the stencil rows out of the
 // caches.
 
-#include <cstdlib>
-
 #include "atu_kernels.h"
 
 namespace atu {
@@ -222,11 +220,11 @@ __device__ __forceinline__ void tend_dyn(const StageArgs<S>& a, const CellGeo<R>
         const R transport_t = u0 * T.d1r + v_invrm * T.d1t + w_invrs * T.d1p;
         const R diffusion_t = T.d2r + T.d1r * two_inv_rm + T.d2t * inv_rm2 + mf(T.d1t, cirs, pole)
                               + mf(T.d2p, ir2s2, pole);
+        nb.template item<5>();
         R precip_t = (R)C.precip_coupling * nb.fz(a, ATU_Q_PRECIP, c) * (R)C.precip_num / (R)C.precip_den;
         if (!isfinite(precip_t)) precip_t = R(0.0);
         else if (precip_t > R(0.5)) precip_t = R(0.5);
         else if (precip_t < R(-0.5)) precip_t = R(-0.5);
-        nb.template item<5>();
         R pressure_t = (R)C.coeff_energy_p * scale_p * (u0 * dpdr + v_invrm * dpdthe + w_invrs * dpdphi);
         if (cp_on) pressure_t *= (R)(C.cp_mix / nb.cp_lapse(a, (double)T.c * C.t_ref));
         const R rhs_t = +pressure_t - transport_t + diffusion_t / (R)C.re_pr + diffusion_t * nue_t_s + precip_t;
@@ -422,14 +420,14 @@ struct NbGlobal {
 // Groups, in the order the tendencies read them (stages 1-3; stage 0 reads neither y_n nor the
 // accumulator):
 //   0 U: R1 R2 C, P0 (nue | nue)   1 V: R1 R2 C   2 W: R1 R2 C
-//   3 pressures: p_dyn R1 R2 C, p_hydro R2 C   4 T: R1 R2 C, P0 (q_precip | q_precip)
-//   5 (y_n | acc) of t, u, v, w in R1 R2 C P0
+//   3 pressures: p_dyn R1 R2 C, p_hydro R2 C   4 T: R1 R2 C (+ P0 (q_precip | q_precip) at stages 1-3)
+//   5 (y_n | acc) of t, u, v, w in R1 R2 C P0 (stage 0: P0 (q_precip | q_precip)); P1 the cp table pairs
 //   6-17 the twelve species: R1 R2 C, P0 (sp | mf of H2S and NH3, mf_nh4sh at 17, else sp), P1 (y_n | acc)
 //   18 NH4SH: R1 R2 C, P0 (rho_mix | fluxlim), P1 (y_n | acc)
 //   19 TKE: R1 R2 C, P0 (tke_source | tke_source), P1 (y_n | acc)
 //   20 DIS: R1 R2 C, P0 (y_n | acc), P1 (dis_source | dis_source)
-// Values a later group needs (nue, mf_nh4sh, the p_hydro differences, u, v, w at the cell) are read
-// into registers while their group is current.
+// Values a later group needs (nue, mf_nh4sh, q_precip, the p_hydro differences, u, v, w at the cell)
+// are read into registers while their group is current.
 // The compiler does not order LDS reads after these loads, so the ring waits explicitly: entering
 // group N, the wave issues group N+1 into the other slot (after s_waitcnt lgkmcnt(0), so that the
 // slot's earlier LDS reads have completed) and then waits with s_waitcnt vmcnt(n), n = the number of
@@ -445,7 +443,7 @@ enum { SEG_R1 = 0, SEG_R2 = kRingSeg, SEG_C = 2 * kRingSeg, SEG_P0 = 3 * kRingSe
 // Load instructions of group g at stage STAGE.
 template <int STAGE>
 __host__ __device__ constexpr int ring_ops(int g) {
-    return g == 0 ? 4 : g <= 2 ? 3 : g == 3 ? 5 : g == 4 ? 4 : g == 5 ? (STAGE > 0 ? 4 : 0)
+    return g == 0 ? 4 : g <= 2 ? 3 : g == 3 ? 5 : g == 4 ? (STAGE > 0 ? 4 : 3) : g == 5 ? (STAGE > 0 ? 4 : 1)
          : g <= 20 ? (STAGE > 0 ? 5 : 4) : 0;
 }
 
@@ -478,7 +476,7 @@ struct NbRing {
     int lane, iw, ie;        // the lane and its LDS indices of the zonal neighbours in segment C
     bool hi, valid;
     Inv<double> v;
-    double nue_v, mf4_v, u0_v, v0_v, w0_v, dpht_v, dphp_v;
+    double nue_v, mf4_v, qp_v, u0_v, v0_v, w0_v, dpht_v, dphp_v;
 
     __device__ __forceinline__ double* slot(int g) const { return ring + (g & 1) * kRingSlot; }
     __device__ __forceinline__ const double* fld(const double* base, int f) const { return base + (size_t)f * a.sh.fs; }
@@ -511,13 +509,15 @@ struct NbRing {
             ring_load(a.phydro + o_c, s + SEG_P1);
         } else if constexpr (G == 4) {
             stencil(fld(a.xs, ATU_T), s);
-            pair(fzp(ATU_Q_PRECIP), fzp(ATU_Q_PRECIP), s + SEG_P0);
+            if constexpr (Y) pair(fzp(ATU_Q_PRECIP), fzp(ATU_Q_PRECIP), s + SEG_P0);
         } else if constexpr (G == 5) {
             if constexpr (Y) {
                 ya(ATU_T, s + SEG_R1);
                 ya(ATU_U, s + SEG_R2);
                 ya(ATU_V, s + SEG_C);
                 ya(ATU_W, s + SEG_P0);
+            } else {
+                pair(fzp(ATU_Q_PRECIP), fzp(ATU_Q_PRECIP), s + SEG_P0);
             }
         } else if constexpr (G <= 17) {
             constexpr int f = ATU_CH4 + (G - 6);
@@ -561,6 +561,10 @@ struct NbRing {
         } else if constexpr (G == 3) {
             dpht_v = cur[SEG_P0 + lane] - cur[SEG_P0 + 64 + lane];
             dphp_v = cur[SEG_P1 + ie] - cur[SEG_P1 + iw];
+        } else if constexpr (G == 4) {
+            if constexpr (STAGE > 0) qp_v = cur[SEG_P0 + lane];
+        } else if constexpr (G == 5) {
+            if constexpr (STAGE == 0) qp_v = cur[SEG_P0 + lane];
         } else if constexpr (G == 17) {
             mf4_v = cur[SEG_P0 + 64 + lane];
         }
@@ -592,6 +596,7 @@ struct NbRing {
     __device__ __forceinline__ double fz(const StageArgs<double>&, int f, size_t) const {
         if (f == ATU_NUE) return nue_v;
         if (f == ATU_MF_NH4SH) return mf4_v;
+        if (f == ATU_Q_PRECIP) return qp_v;
         if (f == ATU_DIS_SOURCE) return cur[SEG_P1 + lane];
         // the low row of the current group's frozen pair, or the high one (mf of H2S / NH3, the flux limiter)
         const bool high = f == ATU_MF_H2S || f == ATU_MF_NH3 || f == ATU_FLUXLIM_NH4SH;
@@ -753,12 +758,9 @@ hipError_t launch_stage(int stage, const StageArgs<S>& a, hipStream_t s) {
 }  // namespace
 
 hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s) {
-    static const int ring_env = [] {
-        const char* e = std::getenv("ATU_STAGE_RING");
-        return e ? std::atoi(e) : 1;
-    }();
     const int nk = a.sh.km - 1;
-    if (!ring_env || (nk & 1) || (a.sh.ldk & 1)) return launch_stage<double>(stage, a, s);
+    // the ring loads 16-byte pairs: an odd ring of meridians takes the direct kernel
+    if ((nk & 1) || (a.sh.ldk & 1)) return launch_stage<double>(stage, a, s);
     const dim3 block(64, 4, 1);
     const dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + 3) / 4, a.sh.im - 2);
     switch (stage) {
