@@ -11,7 +11,7 @@
 //
 // The arrays a stage touches once per cell (the frozen inputs, y_n, the accumulator, the outputs) are
 // read and written with non-temporal hints, so that they do not push the stencil rows out of the
-// caches.
+// caches (FP64 stage 0.4885 -> 0.4735 ms for the ring's loads).
 
 #include "atu_kernels.h"
 
@@ -365,11 +365,11 @@ __device__ __forceinline__ void tend_sca(const StageArgs<S>& a, const CellGeo<R>
     }
 }
 
-// ================================ the stage kernel ================================
-// One thread per interior cell; a block is 64 consecutive k (one wavefront per row) by TJ rows j at one
-// level i, so every field load is a coalesced row segment (512 B in FP64, 256 B in FP32) and the
-// per-(i, j) geometry is wave-uniform. The stencil values come straight from global memory: the
-// neighbouring rows and levels that other blocks also read are served by the caches.
+// ================================ the stage kernels ================================
+// One thread per interior cell; a block is 64 consecutive k (one wavefront per row) by 4 rows j at one
+// level i, so every row segment is coalesced and the per-(i, j) geometry is wave-uniform. Two sources
+// of the values: NbGlobal reads them straight from global memory (the mixed path), NbRing through a
+// per-wave LDS ring (the FP64 path).
 template <typename R, typename S>
 struct NbGlobal {
     const S* xs;
@@ -459,8 +459,9 @@ __device__ __forceinline__ void wait_vm() {
 }
 
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
+template <int CPOL = 0>
 __device__ __forceinline__ void ring_load(const double* g, double* l) {
-    __builtin_amdgcn_global_load_lds((const void*)g, (lds_void_ptr)l, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((const void*)g, (lds_void_ptr)l, 16, 0, CPOL);
 }
 
 // Invalid lanes (k >= km-1 in a row's last segment) form values from whatever the ring holds and store
@@ -487,7 +488,7 @@ struct NbRing {
     }
     // (lo | hi) pair of rows at the cells
     __device__ __forceinline__ void pair(const double* lo, const double* hi_p, double* s) const {
-        ring_load((hi ? hi_p : lo) + o_p, s);
+        ring_load<2>((hi ? hi_p : lo) + o_p, s);
     }
     __device__ __forceinline__ void ya(int f, double* s) const { pair(fld(a.yn, f), fld(a.acc, f), s); }
     __device__ __forceinline__ const double* fzp(int f) const { return fld(a.frozen, f - ATU_NUE); }
@@ -636,26 +637,6 @@ __device__ __forceinline__ Inv<R> inv_of(const StageArgs<S>& a) {
     return Inv<R>{(R)G.inv_2dr, (R)G.inv_dr2, (R)G.inv_2dthe, (R)G.inv_dthe2, (R)G.inv_2dphi, (R)G.inv_dphi2};
 }
 
-template <typename S, int STAGE, int TJ>
-__global__ void __launch_bounds__(64 * TJ) k_rk_stage(const StageArgs<S> a) {
-    using R = S;   // FP64 storage: FP64 arithmetic; FP32 storage: FP32 arithmetic
-    const int nk = a.sh.km - 1;
-    const int k = blockIdx.x * 64 + threadIdx.x;
-    const int j = 1 + blockIdx.y * TJ + threadIdx.y;
-    const int i = 1 + blockIdx.z;
-    if (k >= nk || j > a.sh.jm - 2) return;
-    const int ldk = a.sh.ldk;
-    const size_t row = ((size_t)i * a.sh.jm + j) * ldk;
-    const size_t c = row + k;
-    const size_t cw = row + (k == 0 ? nk - 1 : k - 1);   // phi ring over km-1 meridians
-    const size_t ce = row + (k == nk - 1 ? 0 : k + 1);
-    const bool pole = a.geo.pole_row[j] != 0.0;
-    NbGlobal<R, S> nb{a.xs, a.pdyn, a.phydro, c, cw, ce, a.sh.plane, a.sh.fs, ldk, inv_of<R>(a)};
-    const CellGeo<R> g = cell_geo<R>(a, i, j);
-    tend_dyn<R, S, STAGE>(a, g, c, nb, pole);
-    tend_sca<R, S, STAGE>(a, g, c, nb, pole);
-}
-
 // The FP64 stage with the ring source: a block is 64 k x 4 rows at one level (one wave per row), each
 // wave with its own ring (2 slots x 5 KiB), no barriers. Needs km-1 and ldk even (the halo and row
 // segments are loaded as 16-byte pairs); launch_rk_stage_f64 checks that.
@@ -699,14 +680,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     tend_sca<double, double, STAGE>(a, g, c, nb, pole);
 }
 
-// blocks of 64 k x 4 rows (taller blocks, 64 x 8 and 64 x 12, and the XCD-aware block orders that
-// were measured against it ran slower)
-// The same stage with the arrays that are never written through another pointer passed as restrict
+// The mixed (FP32 storage) stage with the direct source: a block is 64 k x 4 rows at one level, one
+// thread per cell. The arrays that are never written through another pointer are passed as restrict
 // kernel arguments (the stage input, the next stage input, the accumulator, the frozen inputs, the
 // pressures, aux), so that the compiler may issue a field's loads ahead of the previous field's
-// stores. Measured: the FP32 stage 0.335 -> 0.325 ms, the FP64 stage 0.517 -> 0.530 ms (the FP64
-// kernel's schedule gets longer), so only the mixed path launches it. y_n is left out: the last stage
-// writes y_{n+1} over it.
+// stores (0.335 -> 0.325 ms per stage). y_n is left out: the last stage writes y_{n+1} over it.
 template <typename S, int STAGE, int TJ>
 __global__ void __launch_bounds__(64 * TJ) k_rk_stage_noalias(const StageArgs<S> a0, const S* __restrict__ xs,
                                                          S* __restrict__ xnext, double* __restrict__ acc,
@@ -741,26 +719,13 @@ __global__ void __launch_bounds__(64 * TJ) k_rk_stage_noalias(const StageArgs<S>
 
 constexpr int kTJ = 4;
 
-template <typename S>
-hipError_t launch_stage(int stage, const StageArgs<S>& a, hipStream_t s) {
-    const int nk = a.sh.km - 1;
-    const dim3 block(64, kTJ, 1);
-    const dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + kTJ - 1) / kTJ, a.sh.im - 2);
-    switch (stage) {
-        case 0: hipLaunchKernelGGL((k_rk_stage<S, 0, kTJ>), grid, block, 0, s, a); break;
-        case 1: hipLaunchKernelGGL((k_rk_stage<S, 1, kTJ>), grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_rk_stage<S, 2, kTJ>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_rk_stage<S, 3, kTJ>), grid, block, 0, s, a); break;
-    }
-    return hipGetLastError();
-}
-
 }  // namespace
 
 hipError_t launch_rk_stage_f64(int stage, const StageArgs<double>& a, hipStream_t s) {
     const int nk = a.sh.km - 1;
-    // the ring loads 16-byte pairs: an odd ring of meridians takes the direct kernel
-    if ((nk & 1) || (a.sh.ldk & 1)) return launch_stage<double>(stage, a, s);
+    // the ring loads 16-byte pairs; atu_create accepts only an even ring of meridians, ldk is a
+    // multiple of 32
+    if ((nk & 1) || (a.sh.ldk & 1)) return hipErrorInvalidValue;
     const dim3 block(64, 4, 1);
     const dim3 grid((nk + 63) / 64, (a.sh.jm - 2 + 3) / 4, a.sh.im - 2);
     switch (stage) {
