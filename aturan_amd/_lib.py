@@ -159,6 +159,7 @@ _SIGS = {
     "atu_profile_begin_classes": (C.c_int, [C.c_void_p, C.c_char_p]),
     "atu_profile_query": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.POINTER(C.c_int)]),
     "atu_profile_end": (C.c_int, [C.c_void_p]),
+    "atu_profile_sample": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 _lib = None

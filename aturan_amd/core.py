@@ -351,8 +351,10 @@ class UranusCore:
         _check(self._lib.atu_window_rewind(self._ctx), self._ctx, "atu_window_rewind")
 
     # ---- per-kernel timing ----
-    def profile_begin(self, classes: str | None = None) -> None:
-        """Bracket kernel launches with HIP events: every class, or the comma-separated `classes`."""
+    def profile_begin(self, classes: str | None = None, every: int = 1) -> None:
+        """Bracket kernel launches with HIP events: every class, or the comma-separated `classes`; in
+        step(), every `every`-th step only."""
+        _check(self._lib.atu_profile_sample(self._ctx, every), self._ctx, "atu_profile_sample")
         if classes is None:
             _check(self._lib.atu_profile_begin(self._ctx), self._ctx, "atu_profile_begin")
         else:

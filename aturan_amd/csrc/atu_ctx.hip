@@ -139,6 +139,8 @@ struct atu_ctx {
     // profiling
     bool prof = false;
     unsigned prof_mask = ~0u;   // the kernel classes bracketed while profiling
+    int prof_every = 1;         // while profiling, atu_step brackets every prof_every-th step only
+    long prof_step = 0;         // atu_step's steps since atu_profile_begin
     std::vector<ProfEvent> events;
     std::vector<hipEvent_t> pool;
 
@@ -1562,8 +1564,15 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
     // The hydrostatic prep of step n+1 reads only t, v, w of step n's result, and the BC and the
     // Poisson solve of step n do not read its output: the prep runs on the side stream, its level
     // means once t's BC is done (the step does t's BC first), its column scan after the rest.
+    // a step left out of the profiler's sample runs with the events off; restored on every exit
+    struct ProfGate {
+        atu_ctx* c;
+        bool was;
+        ~ProfGate() { c->prof = was; }
+    } gate{ctx, ctx->prof};
     for (int n = 0; n < nsteps; n++) {
         int e;
+        if (gate.was) ctx->prof = (ctx->prof_step++ % ctx->prof_every) == 0;
         const bool step_ev = ctx->prof && (ctx->prof_mask & (1u << PC_STEP));
         ProfEvent pe{PC_STEP, nullptr, nullptr};
         if (step_ev) {
@@ -1753,6 +1762,13 @@ int atu_profile_begin_classes(atu_ctx* ctx, const char* classes) {
     ctx->events.clear();
     ctx->prof_mask = mask;
     ctx->prof = true;
+    ctx->prof_step = 0;
+    return ATU_OK;
+}
+
+int atu_profile_sample(atu_ctx* ctx, int every) {
+    if (!ctx || every < 1) return ATU_EINVAL;
+    ctx->prof_every = every;
     return ATU_OK;
 }
 

@@ -103,6 +103,7 @@ def hbm_copy():
 
 
 TRAFFIC_FILE = "profiles/traffic_rk_stage.json"
+EVENT_EVERY = 5   # the timed region brackets the kernels of one step in five with events
 
 
 def traffic_from_profiles():
@@ -150,14 +151,15 @@ def initial_state(refine=1):
 PROFILE_CLASSES = ("rk_stage", "line", "baro", "residual", "hydro", "bc", "proj_src")
 
 
-def timed_run(core, steps, warmup, opts, events=True):
-    """The timed region: only the stage kernels and the whole steps are bracketed by HIP events (for
-    the roofline and the stream time per step)."""
+def timed_run(core, steps, warmup, opts, events=True, every=EVENT_EVERY):
+    """The timed region: the stage kernels and the whole steps of every `every`-th step are bracketed
+    by HIP events (for the roofline and the stream time per step); the other steps run without the
+    events, whose markers add ~11 us between two kernels."""
     for _ in range(warmup):
         core.step(1, **opts)
     core.synchronize()
     if events:
-        core.profile_begin("rk_stage,step")
+        core.profile_begin("rk_stage,step", every=every)
     t0 = time.perf_counter()
     st = core.step(steps, **opts)
     core.synchronize()
@@ -165,9 +167,9 @@ def timed_run(core, steps, warmup, opts, events=True):
     if not events:
         return el, st, (0.0, 0), 0.0
     rk = core.profile_query("rk_stage")
-    gpu_step_ms = core.profile_query("step")[0] / steps
+    step_ms, step_n = core.profile_query("step")
     core.profile_end()
-    return el, st, rk, gpu_step_ms
+    return el, st, rk, (step_ms / step_n if step_n else 0.0)
 
 
 def kernel_breakdown(core, steps, opts):
@@ -641,8 +643,8 @@ def main():
             breakdown = None if args.no_events else kernel_breakdown(core, min(args.steps, 20), opts64)
         sps = args.steps / el
         progress(f"fp64 {sps:.1f} steps/s")
-        # every timed step runs the four stages once (the first and the last as two launches each)
-        achieved = sum(RK_PASSES_FP64) * 8 * N_INT * args.steps / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
+        # the sampled steps' four stage launches: the mean algorithmic bytes of a stage over its mean duration
+        achieved = sum(RK_PASSES_FP64) / 4 * 8 * N_INT / (rk_ms / rk_n * 1e-3) / 1e9 if rk_ms > 0 else 0.0
         line.update(value=round(sps, 3), ms_per_step=round(1e3 * el / args.steps, 4))
         cfg.update(cell_updates_per_s=round(sps * N_INT, 1), solve_iterations_last_step=st.sweeps_done,
                    solve_rel_resid_last_step=st.rel_resid)
@@ -653,8 +655,9 @@ def main():
                             "traffic_source": (f"{TRAFFIC_FILE}: rocprofv3 FETCH_SIZE and WRITE_SIZE of "
                                                f"{tr.get('kernel')}, scaled by the copy calibration; committed "
                                                f"summary, not measured in this run") if tr else None,
-                            "kernel": "k_rk_stage<double> (fused RHS + RK fold)",
-                            "avg_stage_ms": round(rk_ms / (4 * args.steps), 5), "launches": rk_n,
+                            "kernel": "k_rk_stage_ring<STAGE> (fused RHS + RK fold, FP64, LDS ring)",
+                            "avg_stage_ms": round(rk_ms / rk_n, 5) if rk_n else None, "launches": rk_n,
+                            "events": f"stage kernels of every {EVENT_EVERY}th timed step",
                             "bytes_per_stage": sum(RK_PASSES_FP64) * 8 * N_INT / 4,
                             "copy_GBps": copy["GBps"], "frac_of_copy": round(achieved / copy["GBps"], 4)}
         line["kernels_ms_per_step"] = breakdown
@@ -667,7 +670,7 @@ def main():
             breakdown = None if args.no_events else kernel_breakdown(core, min(args.steps, 20), optsmx)
         sps = args.steps / el
         progress(f"mixed {sps:.1f} steps/s")
-        achieved = sum(RK_BYTES_MIXED) * N_INT * args.steps / (rk_ms * 1e-3) / 1e9 if rk_ms > 0 else 0.0
+        achieved = sum(RK_BYTES_MIXED) / 4 * N_INT / (rk_ms / rk_n * 1e-3) / 1e9 if rk_ms > 0 else 0.0
         mixed = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 * el / args.steps, 4),
                  "speedup_vs_fp64": round(sps / line["value"], 4) if line["value"] else None,
                  "rk_stage_GBps": round(achieved, 1), "rk_stage_frac": round(achieved / PEAK_HBM_GBS, 4),

@@ -454,6 +454,9 @@ int atu_profile_begin(atu_ctx* ctx);
 int atu_profile_begin_classes(atu_ctx* ctx, const char* classes);
 int atu_profile_query(atu_ctx* ctx, const char* kernel_class, double* total_ms, int* launches);
 int atu_profile_end(atu_ctx* ctx);
+/* Bracket the launches of every `every`-th step of atu_step only (counted from atu_profile_begin;
+ * default 1): a sample of the steps carries the events, the others run without their gaps. */
+int atu_profile_sample(atu_ctx* ctx, int every);
 
 #ifdef __cplusplus
 }
