@@ -338,6 +338,8 @@ def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s, events=True):
             "adjoint_kernels_ms": round(prof_ms, 2) if prof_ms is not None else None,
             "fwd_plus_adjoint_steps_per_s": round(nsteps / (fwd + bwd), 3),
             "grad_t0_norm": float((g_t * g_t).sum() ** 0.5), "grad_nue": g_nue,
+            "checked_by": "tests/test_gpu_adjoint.py::test_adjoint_full_shell_tolerance_solves (this window, "
+                          "checkpoint every 10, solves to 1e-8: central FD along 4 directions and in nue, bar 1e-4)",
             "memory": f"checkpoints {(nsteps + K - 1) // K + 1} x 20 fields + tape {K} x 19 fields of "
                       f"{g.im}x{g.jm}x384 FP64"}
 

@@ -171,6 +171,20 @@ def test_mixed_rb_window_adjoint_tracks_fp64(oracle):
 
 
 @pytest.mark.slow
+def test_adjoint_full_shell_tolerance_solves(oracle):
+    """The configuration bench.py times: 41x181x361, 50 steps, checkpoint every 10, every solve to a
+    relative residual of 1e-8 (the adjoint differentiates the converged solve)."""
+    print("[case] adjoint vs FD, 41x181x361, 50 steps, checkpoint every 10, solves to 1e-8, 4 directions",
+          flush=True)
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F = _state(g, p, oracle)
+    opts = dict(nsteps=50, checkpoint_every=10, rel_tol=1e-8)
+    worst, en, gT, gN = _fd_check(g, p, F, opts, ndir=4, seed=12)
+    assert worst <= BAR and en <= BAR
+
+
+@pytest.mark.slow
 def test_adjoint_full_shell_50_steps(oracle):
     print("[case] adjoint vs FD, 41x181x361, 50 steps, checkpoint every 10, 16 directions", flush=True)
     g = A.default_grid(1)
