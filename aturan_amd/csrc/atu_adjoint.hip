@@ -703,6 +703,11 @@ __global__ void __launch_bounds__(256) k_zero(double* v, size_t n) {
     if (q < n) v[q] = 0.0;
 }
 
+__global__ void __launch_bounds__(256) k_extrapolate(double* z, const double* z1, const double* z2, size_t n) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) z[q] = z2 ? 2.0 * z1[q] - z2[q] : z1[q];
+}
+
 inline dim3 cells_grid(const Shape& sh, int kext) { return dim3((kext + 255) / 256, sh.jm, sh.im); }
 
 }  // namespace
@@ -796,6 +801,11 @@ hipError_t launch_loss(const Shape& sh, const double* w, const double* target, d
 hipError_t launch_sum(const double* v, size_t n, double* part, int nparts, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_partial_sum, dim3(nparts), dim3(256), 0, s, v, n, part);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, s, (const double*)part, (size_t)nparts, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_extrapolate(double* z, const double* z1, const double* z2, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z, z1, z2, n);
     return hipGetLastError();
 }
 

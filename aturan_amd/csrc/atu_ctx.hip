@@ -132,6 +132,10 @@ struct atu_ctx {
         double *LAUX = nullptr;                                                            // 3 fields
         double *LP = nullptr, *LPN = nullptr, *LPI = nullptr, *LPH = nullptr, *LNUE = nullptr, *Z = nullptr,
                *B = nullptr, *LSRC = nullptr, *GREF = nullptr;                               // 1 field each
+        // the transposed solves' last two solutions (the next reverse step starts from their
+        // extrapolation) and how many are held since the backward began
+        double *ZP1 = nullptr, *ZP2 = nullptr;
+        int zheld = 0;
         double *TGT = nullptr, *ROWS = nullptr, *PART = nullptr, *SC = nullptr;
         double loss = 0.0;
     } win;
@@ -205,7 +209,7 @@ atu_ctx::~atu_ctx() {
     if (pref_blob) hipFree(pref_blob);
     if (RHS3) hipFree(RHS3);
     for (double* p : {win.ckpt, win.tape, win.tape_level, win.LY, win.LYN, win.LS, win.LK, win.LK2, win.LAUX, win.LP,
-                      win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.B, win.LSRC, win.GREF, win.TGT, win.ROWS,
+                      win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.ZP1, win.ZP2, win.B, win.LSRC, win.GREF, win.TGT, win.ROWS,
                       win.PART, win.SC})
         if (p) hipFree(p);
     for (float* p : {Y32, SA32, SB32, FROZEN32, AUX32, E32, R32})
@@ -905,7 +909,7 @@ int window_alloc(atu_ctx* c, int K, int slots) {
     for (double** p : {&W.LY, &W.LYN, &W.LS, &W.LK, &W.LK2})
         if ((e = grab(p, 4 * fs))) return e;
     if ((e = grab(&W.LAUX, 3 * fs))) return e;
-    for (double** p : {&W.LP, &W.LPN, &W.LPI, &W.LPH, &W.LNUE, &W.Z, &W.B, &W.LSRC, &W.GREF})
+    for (double** p : {&W.LP, &W.LPN, &W.LPI, &W.LPH, &W.LNUE, &W.Z, &W.ZP1, &W.ZP2, &W.B, &W.LSRC, &W.GREF})
         if ((e = grab(p, fs))) return e;
     if ((e = grab(&W.TGT, c->sh.jm)) || (e = grab(&W.ROWS, (size_t)(c->sh.im - 2) * (c->sh.jm - 2))) ||
         (e = grab(&W.PART, kSumParts)) || (e = grab(&W.SC, 8)))
@@ -1024,12 +1028,18 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
             return e;
     }
     // Poisson solve: walls folded, the held mean split off, A^T mu = lambda by the forward solver on
-    // Y^-1 lambda, the consistency shift and the source clamp, then div^T into lambda_aux
+    // Y^-1 lambda, the consistency shift and the source clamp, then div^T into lambda_aux. The solve
+    // starts from the extrapolation 2 z_(n+1) - z_(n+2) of the previous reverse steps' solutions (z_(n+1)
+    // after one, zero at the first): it converges to the same tolerance, and the start's constant
+    // component (the solve's null space, which the held mean keeps) is removed by the consistency
+    // shift's transpose, since mu = Y z and (1 - y 1^T / y.1) annihilates y.
     if (!rb && (e = H(c->timed(PC_ADJ, [&] {
              hipError_t r = launch_pwall_adjoint(sh, W.LP, W.LPI, s);
              if (r == hipSuccess) r = launch_dsum<double>(DSUM_P, sh, c->geo, W.LPI, nullptr, c->SROWS, W.SC + 0, s);
              if (r == hipSuccess) r = launch_adj_rhs(sh, W.LPI, W.SC + 0, c->dsol.ncell, W.B, s);
-             if (r == hipSuccess) r = launch_zero(W.Z, fs, s);
+             if (r == hipSuccess)
+                 r = W.zheld == 0 ? launch_zero(W.Z, fs, s)
+                                  : launch_extrapolate(W.Z, W.ZP1, W.zheld >= 2 ? W.ZP2 : nullptr, fs, s);
              return r;
          }), "adjoint solve prep")))
         return e;
@@ -1092,6 +1102,11 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
         return e;
     std::swap(W.LY, W.LYN);
     std::swap(W.LP, W.LPN);
+    if (!rb) {   // this step's solution becomes z_(n+1) of the next one
+        std::swap(W.ZP2, W.ZP1);
+        std::swap(W.ZP1, W.Z);
+        W.zheld = std::min(W.zheld + 1, 2);
+    }
     return ATU_OK;
 }
 
@@ -1644,6 +1659,7 @@ static int window_backward(atu_ctx* ctx, double dloss, double* grad_t0, int ld, 
     const atu_solver_opts o = window_solver(W.opts);
     const int N = W.opts.nsteps;
     int e;
+    W.zheld = 0;
     // seed: dloss * dL/dy_N from the final state
     if ((e = ck_restore(ctx, W.nck)) || (e = window_loss(ctx, dloss, nullptr)) ||
         (e = ctx->hip(launch_zero(W.LP, sh.fs, ctx->stream), "zero")) ||
