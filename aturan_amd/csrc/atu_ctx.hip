@@ -125,7 +125,8 @@ struct atu_ctx {
         uint64_t forwards = 0;
         int K = 0, nck = 0;          // checkpoint interval, checkpoints held
         size_t tape_steps = 0, ck_slots = 0;
-        double* ckpt = nullptr;      // ck_slots x (19 prognostic + p_dyn) fields; the last slot is y_N
+        double* ckpt = nullptr;      // ck_slots x kCkFields fields; the last slot is y_N
+        std::vector<int> ck_hist;    // per slot: the warm-start solutions it holds (0, 1 or 2)
         double* tape = nullptr;      // K x kTapeFields fields
         double* tape_level = nullptr;// K x im level means
         double *LY = nullptr, *LYN = nullptr, *LS = nullptr, *LK = nullptr, *LK2 = nullptr;   // 4 fields each
@@ -858,7 +859,9 @@ int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
 // clamped Poisson source it formed and the temperature of y_{n+1} (for the clamp mask).
 constexpr int kTapeFields = 19;
 enum { TP_X0 = 0, TP_X1 = 4, TP_X2 = 8, TP_X3 = 12, TP_P = 16, TP_SRC = 17, TP_TNEXT = 18 };
-constexpr int kCkFields = ATU_NPROG + 1;   // a checkpoint: the prognostic fields and p_dyn
+// a checkpoint: the prognostic fields, p_dyn, and the two earlier solutions the next solve's warm
+// start extrapolates from (so that the recompute from a checkpoint repeats the forward bit for bit)
+constexpr int kCkFields = ATU_NPROG + 3;
 constexpr int kSumParts = 1024;
 
 double* tape_field(atu_ctx* c, int slot, int f) {
@@ -876,9 +879,10 @@ atu_solver_opts window_solver(const atu_window_opts& w) {
     if (w.solve_iters > 0) {
         o.max_sweeps = w.solve_iters;
         o.rel_tol = 0.0;
-    } else {
+    } else {   // solves to a tolerance start from the quadratic extrapolation, as atu_step's do
         o.max_sweeps = 200;
         o.rel_tol = w.rel_tol;
+        o.warm_start = 2;
     }
     return o;
 }
@@ -919,8 +923,14 @@ int window_alloc(atu_ctx* c, int K, int slots) {
 
 int ck_save(atu_ctx* c, int slot) {
     double* d = c->win.ckpt + (size_t)slot * kCkFields * c->sh.fs;
+    const size_t fs = c->sh.fs;
     int e = dcopy(c, d, c->Y, ATU_NPROG);
-    if (!e) e = dcopy(c, d + (size_t)ATU_NPROG * c->sh.fs, c->PDYN, 1);
+    if (!e) e = dcopy(c, d + (size_t)ATU_NPROG * fs, c->PDYN, 1);
+    const int hist = c->warm_epoch == c->state_epoch ? c->warm_hist : 0;
+    if (!e && hist >= 1) e = dcopy(c, d + (size_t)(ATU_NPROG + 1) * fs, c->PPREV, 1);
+    if (!e && hist >= 2) e = dcopy(c, d + (size_t)(ATU_NPROG + 2) * fs, c->PPREV2, 1);
+    if ((size_t)slot >= c->win.ck_hist.size()) c->win.ck_hist.resize(slot + 1, 0);
+    c->win.ck_hist[slot] = hist;
     return e;
 }
 
@@ -938,6 +948,12 @@ int ck_restore(atu_ctx* c, int slot) {
         if (!e) e = c->hip(launch_convert(c->Y, c->SB32, n, c->stream), "convert");
     }
     if (!e) e = dcopy(c, c->PDYN, d + n, 1);
+    // the warm-start history as the checkpoint found it
+    const int hist = (size_t)slot < c->win.ck_hist.size() ? c->win.ck_hist[slot] : 0;
+    if (!e && hist >= 1) e = dcopy(c, c->PPREV, d + n + c->sh.fs, 1);
+    if (!e && hist >= 2) e = dcopy(c, c->PPREV2, d + n + 2 * c->sh.fs, 1);
+    c->warm_hist = hist;
+    c->warm_epoch = hist > 0 ? c->state_epoch : ~0ul;
     return e;
 }
 
@@ -1638,6 +1654,7 @@ int atu_window_forward(atu_ctx* ctx, const atu_window_opts* opts, double* loss) 
         if ((e = window_step(ctx, o, -1, n % W.K == 0, n == N - 1 || (n + 1) % W.K == 0))) return e;
     }
     if ((e = ck_save(ctx, W.nck)) || (e = window_loss(ctx, 1.0, &W.loss))) return e;
+    ctx->warm_epoch = ~0ul;   // the window calls leave no warm-start history for atu_step
     if (loss) *loss = W.loss;
     W.held = W.ready = true;
     W.generation = ++W.forwards;
@@ -1682,6 +1699,7 @@ static int window_backward(atu_ctx* ctx, double dloss, double* grad_t0, int ld, 
     const double* gsrc = W.LY + (size_t)ATU_T * sh.fs;
     if (grad_t0 && (e = (ld < 0 ? get_field(ctx, gsrc, grad_t0) : get_field_dev(ctx, gsrc, grad_t0, ld)))) return e;
     if ((e = ck_restore(ctx, W.nck))) return e;
+    ctx->warm_epoch = ~0ul;
     return ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
 }
 
@@ -1709,6 +1727,7 @@ int atu_window_rewind(atu_ctx* ctx) {
     ctx->state_epoch++;
     if (!ctx->win.held) return ctx->fail(ATU_EINVAL, "atu_window_rewind: no atu_window_forward to rewind to");
     int e = ck_restore(ctx, 0);
+    ctx->warm_epoch = ~0ul;
     if (!e) e = ctx->hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return e;
 }
