@@ -271,7 +271,10 @@ __global__ void __launch_bounds__(256) k_rhs_adjoint(AdjStageArgs a) {
 // ---- hydrostatic prep adjoint ----
 // Column part: lambda_f(i) = dr/2 (S(i) + S(i+1)) with S the suffix sums of lambda_p_hydro
 // (lambda_f(0) = dr/2 S(1)); then the local derivatives of f = buoy + (v^2 + w^2)/rm + cor w, and
-// into g the derivative with respect to the level mean, reduced per level afterwards.
+// into g the derivative with respect to the level mean, reduced per level afterwards. One thread per
+// (j, k) column; the levels go in chunks of kHydChunk whose loads are all issued before the chunk's
+// sequential suffix sum, so a column waits for ceil(im / kHydChunk) memory round trips, not im.
+constexpr int kHydChunk = 8;
 __global__ void __launch_bounds__(256) k_hydro_adj_col(Shape sh, DevGeo g, StepConsts c, const double* t,
                                                        const double* v, const double* w, const double* pstat,
                                                        const double* level, const double* lph, double* ly,
@@ -280,72 +283,113 @@ __global__ void __launch_bounds__(256) k_hydro_adj_col(Shape sh, DevGeo g, StepC
     const int j = blockIdx.y;
     if (k >= sh.km) return;
     const size_t fs = sh.fs;
+    const size_t col = (size_t)j * sh.ldk + k;
     const double cor = g.cor_hydro[j];
     double Sp = 0.0;   // S(i + 1)
-    for (int i = sh.im - 1; i >= 0; i--) {
-        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-        double lf;
-        if (i >= 1) {
-            const double S = Sp + lph[x];
-            lf = 0.5 * (S + Sp) * c.dr;
-            Sp = S;
-        } else {
-            lf = 0.5 * Sp * c.dr;
+    for (int top = sh.im - 1; top >= 0; top -= kHydChunk) {
+        double vl[kHydChunk], vv[kHydChunk], ww[kHydChunk], tt[kHydChunk], ps[kHydChunk];
+        double l0[kHydChunk], l2[kHydChunk], l3[kHydChunk];
+#pragma unroll
+        for (int q = 0; q < kHydChunk; q++) {
+            const int i = top - q;
+            if (i < 0) break;
+            const size_t x = (size_t)i * sh.plane + col;
+            vl[q] = lph[x];
+            vv[q] = v[x];
+            ww[q] = w[x];
+            tt[q] = t[x];
+            ps[q] = pstat[x];
+            l0[q] = ly[x];
+            l2[q] = ly[2 * fs + x];
+            l3[q] = ly[3 * fs + x];
         }
-        const double vv = v[x], ww = w[x];
-        ly[2 * fs + x] += lf * (2.0 * vv / g.rm[i]);
-        ly[3 * fs + x] += lf * (2.0 * ww / g.rm[i] + cor);
-        double gr = 0.0;
-        const double T = t[x];
-        if (T > 0.0) {
-            const double ref = level[i];
-            const double bb = c.g * pstat[x] / (c.hyd_bnum * T * c.t_ref);
-            const double f = -c.nd_rel * 1.0 * 1.0 * (bb - ref) / ref;
-            if (isfinite(f)) {
-                ly[x] += lf * (c.nd_rel * bb / (ref * T));
-                gr = lf * (c.nd_rel * bb / (ref * ref));
+#pragma unroll
+        for (int q = 0; q < kHydChunk; q++) {
+            const int i = top - q;
+            if (i < 0) break;
+            const size_t x = (size_t)i * sh.plane + col;
+            double lf;
+            if (i >= 1) {
+                const double S = Sp + vl[q];
+                lf = 0.5 * (S + Sp) * c.dr;
+                Sp = S;
+            } else {
+                lf = 0.5 * Sp * c.dr;
             }
+            ly[2 * fs + x] = l2[q] + lf * (2.0 * vv[q] / g.rm[i]);
+            ly[3 * fs + x] = l3[q] + lf * (2.0 * ww[q] / g.rm[i] + cor);
+            double gr = 0.0;
+            const double T = tt[q];
+            if (T > 0.0) {
+                const double ref = level[i];
+                const double bb = c.g * ps[q] / (c.hyd_bnum * T * c.t_ref);
+                const double f = -c.nd_rel * 1.0 * 1.0 * (bb - ref) / ref;
+                if (isfinite(f)) {
+                    ly[x] = l0[q] + lf * (c.nd_rel * bb / (ref * T));
+                    gr = lf * (c.nd_rel * bb / (ref * ref));
+                }
+            }
+            gref[x] = gr;
         }
-        gref[x] = gr;
     }
 }
 
 // Level part: lambda_ref(i) = sum over the level of g; then
 // d ref / d T(j, k) = -sin_weight(j) bb / (T W), W the level's weight sum over the cells it counts.
-__global__ void __launch_bounds__(1024) k_hydro_adj_level(Shape sh, DevGeo g, StepConsts c, const double* t,
-                                                          const double* pstat, const double* gref, double* ly) {
-    const int i = blockIdx.x;
+// Each level is cut into kHydParts parts of consecutive cells: the first kernel sums g and the
+// weights of one part (256 strided partials, then a halving tree), the second sums a level's parts in
+// part order (the same order in every block) and updates one part's cells.
+constexpr int kHydParts = kHydAdjParts;
+__device__ __forceinline__ double hyd_bb(const StepConsts& c, double p, double T) {
+    return c.g * p / (c.hyd_bnum * T * c.t_ref);
+}
+__global__ void __launch_bounds__(256) k_hydro_adj_level_sums(Shape sh, DevGeo g, StepConsts c, const double* t,
+                                                              const double* pstat, const double* gref,
+                                                              double* parts) {
+    const int i = blockIdx.y, part = blockIdx.x;
     const int n = sh.jm * sh.km;
-    __shared__ double red[2][1024];
+    const int q0 = (int)((long)n * part / kHydParts), q1 = (int)((long)n * (part + 1) / kHydParts);
+    __shared__ double red[2][256];
     double sg = 0.0, sw = 0.0;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += 256) {
         const int j = q / sh.km, k = q - j * sh.km;
         const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
         sg += gref[x];
         const double T = t[x];
-        if (T > 0.0) {
-            const double bb = c.g * pstat[x] / (c.hyd_bnum * T * c.t_ref);
-            if (isfinite(bb)) sw += g.sin_weight[j];
-        }
+        if (T > 0.0 && isfinite(hyd_bb(c, pstat[x], T))) sw += g.sin_weight[j];
     }
     red[0][threadIdx.x] = sg;
     red[1][threadIdx.x] = sw;
     __syncthreads();
-    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    for (int h = 128; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h) {
             red[0][threadIdx.x] += red[0][threadIdx.x + h];
             red[1][threadIdx.x] += red[1][threadIdx.x + h];
         }
         __syncthreads();
     }
-    const double lref = red[0][0], W = red[1][0];
+    if (threadIdx.x == 0) {
+        parts[2 * ((size_t)i * kHydParts + part)] = red[0][0];
+        parts[2 * ((size_t)i * kHydParts + part) + 1] = red[1][0];
+    }
+}
+__global__ void __launch_bounds__(256) k_hydro_adj_level(Shape sh, DevGeo g, StepConsts c, const double* t,
+                                                         const double* pstat, const double* parts, double* ly) {
+    const int i = blockIdx.y, part = blockIdx.x;
+    const int n = sh.jm * sh.km;
+    const int q0 = (int)((long)n * part / kHydParts), q1 = (int)((long)n * (part + 1) / kHydParts);
+    double lref = 0.0, W = 0.0;
+    for (int r = 0; r < kHydParts; r++) {
+        lref += parts[2 * ((size_t)i * kHydParts + r)];
+        W += parts[2 * ((size_t)i * kHydParts + r) + 1];
+    }
     if (!(W > 0.0)) return;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += 256) {
         const int j = q / sh.km, k = q - j * sh.km;
         const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
         const double T = t[x];
         if (T > 0.0) {
-            const double bb = c.g * pstat[x] / (c.hyd_bnum * T * c.t_ref);
+            const double bb = hyd_bb(c, pstat[x], T);
             if (isfinite(bb)) ly[x] += lref * (-g.sin_weight[j] * bb / (T * W));
         }
     }
@@ -724,11 +768,14 @@ hipError_t launch_rhs_adjoint(const AdjStageArgs& a, hipStream_t s) {
 
 hipError_t launch_hydro_adjoint(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* x4,
                                 const double* pstat, const double* level, const double* lph, double* gref, double* ly,
-                                hipStream_t s) {
+                                double* parts, hipStream_t s) {
     const size_t fs = sh.fs;
     hipLaunchKernelGGL(k_hydro_adj_col, dim3((sh.km + 255) / 256, sh.jm), dim3(256), 0, s, sh, g, c, x4,
                        x4 + 2 * fs, x4 + 3 * fs, pstat, level, lph, ly, gref);
-    hipLaunchKernelGGL(k_hydro_adj_level, dim3(sh.im), dim3(1024), 0, s, sh, g, c, x4, pstat, (const double*)gref, ly);
+    hipLaunchKernelGGL(k_hydro_adj_level_sums, dim3(kHydParts, sh.im), dim3(256), 0, s, sh, g, c, x4, pstat,
+                       (const double*)gref, parts);
+    hipLaunchKernelGGL(k_hydro_adj_level, dim3(kHydParts, sh.im), dim3(256), 0, s, sh, g, c, x4, pstat,
+                       (const double*)parts, ly);
     return hipGetLastError();
 }
 

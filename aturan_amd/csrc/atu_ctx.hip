@@ -137,7 +137,7 @@ struct atu_ctx {
         // extrapolation) and how many are held since the backward began
         double *ZP1 = nullptr, *ZP2 = nullptr;
         int zheld = 0;
-        double *TGT = nullptr, *ROWS = nullptr, *PART = nullptr, *SC = nullptr;
+        double *TGT = nullptr, *ROWS = nullptr, *PART = nullptr, *SC = nullptr, *HPARTS = nullptr;
         double loss = 0.0;
     } win;
 
@@ -210,7 +210,7 @@ atu_ctx::~atu_ctx() {
     if (pref_blob) hipFree(pref_blob);
     if (RHS3) hipFree(RHS3);
     for (double* p : {win.ckpt, win.tape, win.tape_level, win.LY, win.LYN, win.LS, win.LK, win.LK2, win.LAUX, win.LP,
-                      win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.ZP1, win.ZP2, win.B, win.LSRC, win.GREF, win.TGT, win.ROWS,
+                      win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.ZP1, win.ZP2, win.B, win.LSRC, win.GREF, win.HPARTS, win.TGT, win.ROWS,
                       win.PART, win.SC})
         if (p) hipFree(p);
     for (float* p : {Y32, SA32, SB32, FROZEN32, AUX32, E32, R32})
@@ -916,7 +916,7 @@ int window_alloc(atu_ctx* c, int K, int slots) {
     for (double** p : {&W.LP, &W.LPN, &W.LPI, &W.LPH, &W.LNUE, &W.Z, &W.ZP1, &W.ZP2, &W.B, &W.LSRC, &W.GREF})
         if ((e = grab(p, fs))) return e;
     if ((e = grab(&W.TGT, c->sh.jm)) || (e = grab(&W.ROWS, (size_t)(c->sh.im - 2) * (c->sh.jm - 2))) ||
-        (e = grab(&W.PART, kSumParts)) || (e = grab(&W.SC, 8)))
+        (e = grab(&W.PART, kSumParts)) || (e = grab(&W.SC, 8)) || (e = grab(&W.HPARTS, 2 * kHydAdjParts * c->sh.im)))
         return e;
     return ATU_OK;
 }
@@ -1111,7 +1111,7 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
     // hydrostatic prep, then the held mean's share of p_n
     if ((e = H(c->timed(PC_ADJ, [&] {
              hipError_t r = launch_hydro_adjoint(sh, c->geo, c->consts, tape_field(c, slot, TP_X0), c->PSTAT,
-                                                 W.tape_level + (size_t)slot * sh.im, W.LPH, W.GREF, W.LYN, s);
+                                                 W.tape_level + (size_t)slot * sh.im, W.LPH, W.GREF, W.LYN, W.HPARTS, s);
              if (r == hipSuccess && !rb) r = launch_add_mean(sh, W.SC + 0, c->dsol.ncell, W.LPN, s);
              return r;
          }), "adjoint hydro")))

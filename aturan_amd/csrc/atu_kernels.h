@@ -250,8 +250,11 @@ struct AdjStageArgs {
     double* lnue;          // += lambda of the eddy viscosity
 };
 hipError_t launch_rhs_adjoint(const AdjStageArgs& a, hipStream_t s);
+// `parts`: 2 * kHydAdjParts * im doubles of scratch (the per-level partial sums)
+constexpr int kHydAdjParts = 16;
 hipError_t launch_hydro_adjoint(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* x4,
                                 const double* pstat, const double* level, const double* lph, double* gref, double* ly,
+                                double* parts,
                                 hipStream_t s);
 hipError_t launch_bc_adjoint(const Shape& sh, const atu_params& p, double* ly, hipStream_t s);
 hipError_t launch_final_fold(const Shape& sh, const StepConsts& c, const double* tnext, const double* ly, double* ls,
