@@ -505,7 +505,7 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
 }
 
 // The four stages. With `hydro_pending` this step's hydrostatic prep is still running on the side
-// stream (side_hydro_after_bc): the first stage, which reads p_hydro, waits for it.
+// stream (side_bc_hydro, side_hydro_after_bc): the first stage, which reads p_hydro, waits for it.
 int do_rk4(atu_ctx* c, bool hydro_pending = false) {
     c->rhs_valid = c->keep_rhs && !c->mixed();
     if (hydro_pending && c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent")) return ATU_EHIP;
@@ -528,6 +528,25 @@ int side_hydro_after_bc(atu_ctx* c) {
 int do_bc(atu_ctx* c) {
     const BcTargets t = bc_targets(c);
     return c->hip(c->timed(PC_BC, [&] { return launch_bc(c->sh, c->params, t, c->stream); }), "bc");
+}
+
+// The BC of a step's RK4 result and, with `hydro`, the next step's hydrostatic prep, on the side
+// stream behind the last stage: the Poisson solve on the main stream reads aux and p_dyn only, which
+// neither writes, so they run beside it. The species clamp, which the prep does not read, goes last,
+// so that its streaming pass meets the solve's latency-bound iterations rather than its
+// bandwidth-bound set-up. ev_hydro marks their end; the next step's first stage, or (for the last
+// step) the main stream after the solve, waits for it.
+int side_bc_hydro(atu_ctx* c, bool hydro) {
+    int e;
+    const BcTargets t = bc_targets(c);
+    if ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
+        (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")) ||
+        (e = c->hip(c->timed_on(PC_BC, c->side, [&] { return launch_bc(c->sh, c->params, t, c->side, false); }), "bc")) ||
+        (hydro && (e = do_hydro(c, c->side))) ||
+        (e = c->hip(c->timed_on(PC_BC, c->side, [&] { return launch_species_clamp(c->sh, c->params, t, c->side); }),
+                    "species clamp")))
+        return e;
+    return c->hip(hipEventRecord(c->ev_hydro, c->side), "hipEventRecord");
 }
 
 int do_residual(atu_ctx* c, double* rel, double* srcn) {
@@ -1592,9 +1611,9 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
     }
     if (opts && !(opts->omega > 0.0 && opts->omega < 2.0))
         return ctx->fail(ATU_EINVAL, "atu_step: omega must lie in (0, 2)");
-    // The hydrostatic prep of step n+1 reads only t, v, w of step n's result, and the BC and the
-    // Poisson solve of step n do not read its output: the prep runs on the side stream, its level
-    // means once t's BC is done (the step does t's BC first), its column scan after the rest.
+    // The BC of step n and the hydrostatic prep of step n+1 (which reads t, v, w of step n's result
+    // after the BC) run on the side stream beside step n's Poisson solve, which reads neither's
+    // inputs nor outputs.
     // a step left out of the profiler's sample runs with the events off; restored on every exit
     struct ProfGate {
         atu_ctx* c;
@@ -1611,12 +1630,13 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
             pe.b = ctx->take_event();
             hipEventRecord(pe.a, ctx->stream);
         }
-        // after the first step the prep is still running on the side stream: do_rk4 waits for it
-        // after the part of the first stage that does not read p_hydro
+        // after the first step the BC and the prep are still running on the side stream: the first
+        // stage waits for them
         if (n == 0 && (e = do_hydro(ctx, ctx->stream))) return e;
         const bool more = n + 1 < nsteps;
-        if ((e = do_rk4(ctx, n > 0)) || (e = do_bc(ctx)) || (more && (e = side_hydro_after_bc(ctx)))) return e;
+        if ((e = do_rk4(ctx, n > 0)) || (e = side_bc_hydro(ctx, more))) return e;
         if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;
+        if (!more && (e = ctx->hip(hipStreamWaitEvent(ctx->stream, ctx->ev_hydro, 0), "hipStreamWaitEvent"))) return e;
         if (step_ev) {
             hipEventRecord(pe.b, ctx->stream);
             ctx->events.push_back(pe);

@@ -313,7 +313,7 @@ hipError_t launch_fill(float* dst, float v, size_t n, hipStream_t s, const int* 
 }
 
 hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets& t, int f0, int f1, bool rest,
-                          hipStream_t s) {
+                          hipStream_t s, bool clamp) {
     BcCtx b;
     b.sh = sh;
     b.t = t;
@@ -330,14 +330,24 @@ hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets&
     hipLaunchKernelGGL(k_bc_radius, dim3((sh.km + 255) / 256, sh.jm, nz), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_bc_theta, dim3((sh.km + 255) / 256, sh.im, nz), dim3(256), 0, s, b);
     hipLaunchKernelGGL(k_bc_phi, dim3((sh.jm + 255) / 256, sh.im, nz), dim3(256), 0, s, b);
-    if (rest && p.species_clamp)   // fs is a multiple of ldk, itself a multiple of 32: even
+    if (rest && clamp && p.species_clamp)   // fs is a multiple of ldk, itself a multiple of 32: even
         hipLaunchKernelGGL(k_species_clamp, dim3((unsigned)((sh.fs / 2 + 255) / 256), ATU_NH4SH - ATU_CH4 + 1),
                            dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
-hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s) {
-    return launch_bc_part(sh, p, t, 0, kNumBcFields, true, s);
+hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s, bool clamp) {
+    return launch_bc_part(sh, p, t, 0, kNumBcFields, true, s, clamp);
+}
+
+hipError_t launch_species_clamp(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s) {
+    if (!p.species_clamp) return hipSuccess;
+    BcCtx b;
+    b.sh = sh;
+    b.t = t;
+    hipLaunchKernelGGL(k_species_clamp, dim3((unsigned)((sh.fs / 2 + 255) / 256), ATU_NH4SH - ATU_CH4 + 1), dim3(256),
+                       0, s, b);
+    return hipGetLastError();
 }
 
 }  // namespace atu

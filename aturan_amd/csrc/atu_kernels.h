@@ -84,12 +84,14 @@ struct BcTargets {
     double* frozen;
     float* frozen32;
 };
-hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s);
+hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s, bool clamp = true);
+// clampNegativeSpecies alone (launch_bc with clamp = false and this, in that order, is launch_bc)
+hipError_t launch_species_clamp(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s);
 // The same passes on the prognostic fields [f0, f1) only, and with `rest` the frozen fields and the
 // field-specific forms (rigid lid, species floor, lid form of v and w, species clamp). The fields do
 // not interact in the BC, so a pass over [0, 1) followed by one over [1, 17) with `rest` is the BC.
 hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets& t, int f0, int f1, bool rest,
-                          hipStream_t s);
+                          hipStream_t s, bool clamp = true);
 
 // Projection prep on aux (prepareProjectionBoundaries) and the clamped divergence source.
 template <typename S>
