@@ -577,7 +577,8 @@ int do_residual(atu_ctx* c, double* rel, double* srcn) {
 // Mixed context (iterative refinement): each outer iteration forms the residual r = b - A p in
 // FP64 (stored in FP32), solves A e = r approximately with `inner` FP32 line passes plus barotropic
 // corrections on an FP32 correction e, and accumulates p += e in FP64.
-int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, double* P, const double* SRCP) {
+int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, double* P, const double* SRCP,
+                const void* aux_src = nullptr) {
     const Shape& sh = c->sh;
     hipStream_t s = c->stream;
     auto H = [&](hipError_t e, const char* w) { return c->hip(e, w); };
@@ -593,11 +594,14 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
         return ATU_ENOMEM;
     if (quad && !c->PPREV2 && H(hipMalloc((void**)&c->PPREV2, sh.fs * sizeof(double)), "hipMalloc(p_prev)"))
         return ATU_ENOMEM;
+    // with aux_src the same first pass forms SRCP from aux (the step's projection source)
     if ((e = H(c->timed(PC_RES, [&] { return launch_solve_setup(sh, P, SRCP, c->SROWS, c->SROWS2, c->SCAL,
                                                                  c->dsol.yw, c->RES, c->CTL, record ? c->PPREV : nullptr,
-                                                                 quad ? c->PPREV2 : nullptr, extrap, s); }),
+                                                                 quad ? c->PPREV2 : nullptr, extrap, s, aux_src,
+                                                                 c->mixed(), &c->geo, &c->consts); }),
                "solve setup")))
         return e;
+    if (quad) std::swap(c->PPREV, c->PPREV2);   // the pass rotated the history through PPREV2's buffer
     c->warm_epoch = ~0ul;
     const int max_it = o->max_sweeps > 0 ? o->max_sweeps : 1;
     // Convergence is decided on the device (SolveCtl): the iterations' kernels return at once after
@@ -830,7 +834,8 @@ int do_pressure_fp64(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) 
     const int mode = o->mode & ATU_SOLVE_MODE_MASK, flags = o->mode & ~ATU_SOLVE_MODE_MASK;
     if (flags && (mode != ATU_SOLVE_REFERENCE || (flags & ~(ATU_SOLVE_PRESS_LINE | ATU_SOLVE_PRESS_BAROTROPIC))))
         return c->fail(ATU_EINVAL, "PRESS_LINE / PRESS_BAROTROPIC are options of ATU_SOLVE_REFERENCE only");
-    if (mode == ATU_SOLVE_CONVERGE) return do_converge(c, o, st, c->PDYN, c->SRC);
+    if (mode == ATU_SOLVE_CONVERGE)
+        return do_converge(c, o, st, c->PDYN, c->SRC, c->mixed() ? (const void*)c->AUX32 : (const void*)c->AUX);
     if (mode == ATU_SOLVE_URAN) return do_uran(c, o, st);
     if (mode != ATU_SOLVE_REFERENCE) return c->fail(ATU_EINVAL, "unknown solve mode");
     const int sweeps = o->sweeps > 0 ? o->sweeps : 1;   // PressureSolver.h:174-175
@@ -866,9 +871,13 @@ int do_pressure(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st) {
     atu_solver_opts def{ATU_SOLVE_REFERENCE, 1, 0, 0, 1.0, 0.0};
     if (!o) o = &def;
     if (o->mode == ATU_SOLVE_URAN) return do_uran(c, o, st);   // its own aux walls and source
+    // the converging solve forms the source in its first pass (do_converge's aux_src)
+    const bool src_now = (o->mode & ATU_SOLVE_MODE_MASK) != ATU_SOLVE_CONVERGE;
     const hipError_t pe = c->mixed()
-        ? c->timed(PC_PROJ, [&] { return launch_proj_src<float>(c->sh, c->geo, c->consts, c->AUX32, c->SRC, c->stream); })
-        : c->timed(PC_PROJ, [&] { return launch_proj_src<double>(c->sh, c->geo, c->consts, c->AUX, c->SRC, c->stream); });
+        ? c->timed(PC_PROJ, [&] { return launch_proj_src<float>(c->sh, c->geo, c->consts, c->AUX32, c->SRC, c->stream,
+                                                                src_now); })
+        : c->timed(PC_PROJ, [&] { return launch_proj_src<double>(c->sh, c->geo, c->consts, c->AUX, c->SRC, c->stream,
+                                                                 src_now); });
     if (c->hip(pe, "proj_src")) return ATU_EHIP;
     return do_pressure_fp64(c, o, st);
 }

@@ -93,10 +93,31 @@ hipError_t launch_species_clamp(const Shape& sh, const atu_params& p, const BcTa
 hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets& t, int f0, int f1, bool rest,
                           hipStream_t s, bool clamp = true);
 
+// The clamped divergence source of the Poisson solve at interior cell (i, j, k) (x its offset, nk
+// the ring of meridians): div(aux) with the central differences of the projection
+// (planet/Pressure_Uran.cpp:30-146), non-finite -> 0, clamped to +-den * p_dyn_cap.
+template <typename S>
+__device__ __forceinline__ double poisson_src_at(const Shape& sh, const DevGeo& g, const StepConsts& c, const S* aux,
+                                                 int i, int j, int k, size_t x) {
+    const int nk = sh.km - 1;
+    const size_t row = x - k;
+    const size_t xw = row + (k == 0 ? nk - 1 : k - 1), xe = row + (k == nk - 1 ? 0 : k + 1);
+    const size_t ij = (size_t)i * sh.jm + j;
+    const double du_dr = ((double)aux[x + sh.plane] - (double)aux[x - sh.plane]) * g.inv_2dr;
+    const double dv_dthe = ((double)aux[sh.fs + x + sh.ldk] - (double)aux[sh.fs + x - sh.ldk]) * g.inv_2dthe;
+    const double dw_dphi = ((double)aux[2 * sh.fs + xe] - (double)aux[2 * sh.fs + xw]) * g.inv_2dphi;
+    double div_src = du_dr * 1.0 + dv_dthe * g.inv_rm[i] + dw_dphi * g.inv_rmsinthe_s[ij];
+    const double src_max = g.den[ij] * c.p_dyn_cap;
+    if (!isfinite(div_src)) div_src = 0.0;
+    else if (div_src > src_max) div_src = src_max;
+    else if (div_src < -src_max) div_src = -src_max;
+    return div_src;
+}
+
 // Projection prep on aux (prepareProjectionBoundaries) and the clamped divergence source.
 template <typename S>
 hipError_t launch_proj_src(const Shape& sh, const DevGeo& g, const StepConsts& c, S* aux, double* src,
-                           hipStream_t s);
+                           hipStream_t s, bool form_src = true);
 
 // One colour pass of the red-black relaxation on p (FP64 or FP32 storage, FP64 source). The black
 // pass (colour 1) also writes the radial and polar wall values and the seam copy that the
@@ -160,7 +181,9 @@ hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, d
 // 2 p - pprev (extrap 1) or 3 p - 3 pprev + pprev2 (extrap 2): the warm start.
 hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, double* rows, double* rows2,
                               double* scal, double yw, double* b, int* flags, double* pprev, double* pprev2,
-                              int extrap, hipStream_t s);
+                              int extrap, hipStream_t s,
+                              const void* aux = nullptr, bool aux_f32 = false, const DevGeo* g = nullptr,
+                              const StepConsts* c = nullptr);
 hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done = nullptr);
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,

@@ -78,19 +78,8 @@ __global__ void __launch_bounds__(256) k_src(Shape sh, DevGeo g, StepConsts c, c
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
     const int i = 1 + blockIdx.z;
     if (k >= nk || j > sh.jm - 2) return;
-    const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
-    const size_t x = row + k;
-    const size_t xw = row + (k == 0 ? nk - 1 : k - 1), xe = row + (k == nk - 1 ? 0 : k + 1);
-    const size_t ij = (size_t)i * sh.jm + j;
-    const double du_dr = ((double)aux[x + sh.plane] - (double)aux[x - sh.plane]) * g.inv_2dr;
-    const double dv_dthe = ((double)aux[sh.fs + x + sh.ldk] - (double)aux[sh.fs + x - sh.ldk]) * g.inv_2dthe;
-    const double dw_dphi = ((double)aux[2 * sh.fs + xe] - (double)aux[2 * sh.fs + xw]) * g.inv_2dphi;
-    double div_src = du_dr * 1.0 + dv_dthe * g.inv_rm[i] + dw_dphi * g.inv_rmsinthe_s[ij];
-    const double src_max = g.den[ij] * c.p_dyn_cap;
-    if (!isfinite(div_src)) div_src = 0.0;
-    else if (div_src > src_max) div_src = src_max;
-    else if (div_src < -src_max) div_src = -src_max;
-    src[x] = div_src;
+    const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+    src[x] = poisson_src_at<S>(sh, g, c, aux, i, j, k, x);
 }
 
 // ---- one colour pass; colour 1 also forms the wall values ----
@@ -248,13 +237,14 @@ int residual_partials_count(const Shape& sh) {
 
 template <typename S>
 hipError_t launch_proj_src(const Shape& sh, const DevGeo& g, const StepConsts& c, S* aux, double* src,
-                           hipStream_t s) {
+                           hipStream_t s, bool form_src) {
     const int nk = sh.km - 1;
     const int n = (sh.jm - 2) * nk + (sh.im - 2) * nk;
     hipLaunchKernelGGL((k_proj_walls<S>), dim3((n + 255) / 256), dim3(256), 0, s, sh, 1, aux);
     hipLaunchKernelGGL((k_proj_seam<S>), dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, aux);
-    hipLaunchKernelGGL((k_src<S>), dim3((nk + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0, s, sh, g, c,
-                       (const S*)aux, src);
+    if (form_src)
+        hipLaunchKernelGGL((k_src<S>), dim3((nk + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0, s, sh, g,
+                           c, (const S*)aux, src);
     return hipGetLastError();
 }
 
@@ -284,9 +274,9 @@ hipError_t launch_residual(const Shape& sh, const DevGeo& g, const P* p, const d
 }
 
 template hipError_t launch_proj_src<double>(const Shape&, const DevGeo&, const StepConsts&, double*, double*,
-                                            hipStream_t);
+                                            hipStream_t, bool);
 template hipError_t launch_proj_src<float>(const Shape&, const DevGeo&, const StepConsts&, float*, double*,
-                                           hipStream_t);
+                                           hipStream_t, bool);
 template hipError_t launch_rb_pass<double>(const Shape&, const DevGeo&, const StepConsts&, double*, const double*, int,
                                            double, hipStream_t, int);
 template hipError_t launch_rb_pass<float>(const Shape&, const DevGeo&, const StepConsts&, float*, const double*, int,

@@ -503,9 +503,29 @@ __device__ __forceinline__ void row_sum_store(double term, double* chunk, double
     }
 }
 
-// The two sums the solve starts with, in one pass: y . src and the interior sum of p. With pprev,
-// the pass also keeps p in pprev (and the previous pprev in pprev2, when given) and starts the
-// iterations from 2 p - pprev (extrap 1) or 3 p - 3 pprev + pprev2 (extrap 2).
+// The p part of the solve's first pass at one cell: the value whose interior sum the solve holds,
+// and with pprev the warm start. With pprev2 (quadratic start) the history rotates by buffer: p goes
+// into pprev2's buffer (its p_(n-2) read first) and the caller swaps the two pointers, so pprev
+// holds p_n and pprev2 p_(n-1) after the pass.
+__device__ __forceinline__ double setup_p(double* p, double* pprev, double* pprev2, int extrap, size_t x) {
+    const double tp = p[x];
+    if (pprev) {
+        const double p1 = pprev[x];
+        if (pprev2) {
+            const double p2 = pprev2[x];
+            if (extrap == 2) p[x] = 3.0 * tp - 3.0 * p1 + p2;
+            else if (extrap) p[x] = 2.0 * tp - p1;
+            pprev2[x] = tp;
+        } else {
+            if (extrap) p[x] = 2.0 * tp - p1;
+            pprev[x] = tp;
+        }
+    }
+    return tp;
+}
+
+// The two sums the solve starts with, in one pass: y . src and the interior sum of p, with the warm
+// start of setup_p.
 __global__ void __launch_bounds__(1024) k_dsum_rows_ysrc_p(Shape sh, double* p, const double* src, double* rows_y,
                                                            double* rows_p, double* pprev, double* pprev2, int extrap) {
     const int N = sh.km - 1;
@@ -515,14 +535,30 @@ __global__ void __launch_bounds__(1024) k_dsum_rows_ysrc_p(Shape sh, double* p, 
     if (k < N) {
         const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
         ty = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * src[x];
-        tp = p[x];
-        if (pprev) {
-            const double p1 = pprev[x];
-            if (extrap == 2) p[x] = 3.0 * tp - 3.0 * p1 + pprev2[x];
-            else if (extrap) p[x] = 2.0 * tp - p1;
-            if (pprev2) pprev2[x] = p1;
-            pprev[x] = tp;
-        }
+        tp = setup_p(p, pprev, pprev2, extrap, x);
+    }
+    __shared__ double chunk_y[16], chunk_p[16];
+    const size_t r = (size_t)(i - 1) * (sh.jm - 2) + (j - 1);
+    row_sum_store(ty, chunk_y, rows_y + r);
+    row_sum_store(tp, chunk_p, rows_p + r);
+}
+
+// The same pass forming the source from aux first (the projection's clamped divergence, written to
+// src): k_src and k_dsum_rows_ysrc_p in one launch, the same values.
+template <typename S>
+__global__ void __launch_bounds__(1024) k_src_dsum_rows(Shape sh, DevGeo g, StepConsts c, const S* aux, double* src,
+                                                        double* p, double* rows_y, double* rows_p, double* pprev,
+                                                        double* pprev2, int extrap) {
+    const int N = sh.km - 1;
+    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
+    const int k = threadIdx.x;
+    double ty = 0.0, tp = 0.0;
+    if (k < N) {
+        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
+        const double sv = poisson_src_at<S>(sh, g, c, aux, i, j, k, x);
+        src[x] = sv;
+        ty = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * sv;
+        tp = setup_p(p, pprev, pprev2, extrap, x);
     }
     __shared__ double chunk_y[16], chunk_p[16];
     const size_t r = (size_t)(i - 1) * (sh.jm - 2) + (j - 1);
@@ -742,13 +778,21 @@ hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, c
 
 hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, double* rows, double* rows2,
                               double* scal, double yw, double* b, int* flags, double* pprev, double* pprev2,
-                              int extrap, hipStream_t s) {
+                              int extrap, hipStream_t s, const void* aux, bool aux_f32, const DevGeo* g,
+                              const StepConsts* c) {
     const int N = sh.km - 1;
     const int threads = ((N + 63) / 64) * 64;
     if (threads > 1024) return hipErrorInvalidValue;
     const dim3 grid(sh.jm - 2, sh.im - 2);
     const int nrows = (sh.im - 2) * (sh.jm - 2);
-    hipLaunchKernelGGL(k_dsum_rows_ysrc_p, grid, dim3(threads), 0, s, sh, p, src, rows, rows2, pprev, pprev2, extrap);
+    if (!aux)
+        hipLaunchKernelGGL(k_dsum_rows_ysrc_p, grid, dim3(threads), 0, s, sh, p, src, rows, rows2, pprev, pprev2, extrap);
+    else if (aux_f32)
+        hipLaunchKernelGGL((k_src_dsum_rows<float>), grid, dim3(threads), 0, s, sh, *g, *c, (const float*)aux,
+                           (double*)src, p, rows, rows2, pprev, pprev2, extrap);
+    else
+        hipLaunchKernelGGL((k_src_dsum_rows<double>), grid, dim3(threads), 0, s, sh, *g, *c, (const double*)aux,
+                           (double*)src, p, rows, rows2, pprev, pprev2, extrap);
     hipLaunchKernelGGL(k_dsum_final2, dim3(1), dim3(kDsumParts), 0, s, nrows, (const double*)rows,
                        (const double*)rows2, scal + 0, scal + 1, flags);
     hipLaunchKernelGGL(k_make_b_sq_rows, grid, dim3(threads), 0, s, sh, src, (const double*)scal, yw, b, rows);
