@@ -505,24 +505,13 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
 }
 
 // The four stages. With `hydro_pending` this step's hydrostatic prep is still running on the side
-// stream (side_bc_hydro, side_hydro_after_bc): the first stage, which reads p_hydro, waits for it.
+// stream (side_bc_hydro): the first stage, which reads p_hydro, waits for it.
 int do_rk4(atu_ctx* c, bool hydro_pending = false) {
     c->rhs_valid = c->keep_rhs && !c->mixed();
     if (hydro_pending && c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent")) return ATU_EHIP;
     for (int s = 0; s < 4; s++)
         if (c->hip(c->timed(PC_RK, [&] { return launch_stage(c, s); }), "rk_stage")) return ATU_EHIP;
     return ATU_OK;
-}
-
-// The next step's hydrostatic prep on the side stream, once this step's BC is done: it reads t, v and
-// w of y_{n+1}, which the Poisson solve running meanwhile on the main stream does not touch, and
-// writes the level means and p_hydro, which the solve does not read. ev_hydro marks its end.
-int side_hydro_after_bc(atu_ctx* c) {
-    int e;
-    if ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
-        (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")) || (e = do_hydro(c, c->side)))
-        return e;
-    return c->hip(hipEventRecord(c->ev_hydro, c->side), "hipEventRecord");
 }
 
 int do_bc(atu_ctx* c) {
@@ -532,16 +521,20 @@ int do_bc(atu_ctx* c) {
 
 // The BC of a step's RK4 result and, with `hydro`, the next step's hydrostatic prep, on the side
 // stream behind the last stage: the Poisson solve on the main stream reads aux and p_dyn only, which
-// neither writes, so they run beside it. The species clamp, which the prep does not read, goes last,
+// neither writes (the prep reads t, v, w of y_{n+1} after the BC and writes the level means and
+// p_hydro, which the solve does not read), so they run beside it. The species clamp, which the prep does not read, goes last,
 // so that its streaming pass meets the solve's latency-bound iterations rather than its
 // bandwidth-bound set-up. ev_hydro marks their end; the next step's first stage, or (for the last
 // step) the main stream after the solve, waits for it.
-int side_bc_hydro(atu_ctx* c, bool hydro) {
+// With `tnext`, t after the BC is also copied there (the window's tape) before the prep.
+int side_bc_hydro(atu_ctx* c, bool hydro, double* tnext = nullptr) {
     int e;
     const BcTargets t = bc_targets(c);
     if ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
         (e = c->hip(hipStreamWaitEvent(c->side, c->ev_bc, 0), "hipStreamWaitEvent")) ||
         (e = c->hip(c->timed_on(PC_BC, c->side, [&] { return launch_bc(c->sh, c->params, t, c->side, false); }), "bc")) ||
+        (tnext && (e = c->hip(hipMemcpyAsync(tnext, c->prog(c->Y, ATU_T), c->sh.fs * sizeof(double),
+                                             hipMemcpyDeviceToDevice, c->side), "hipMemcpyAsync(D2D)"))) ||
         (hydro && (e = do_hydro(c, c->side))) ||
         (e = c->hip(c->timed_on(PC_BC, c->side, [&] { return launch_species_clamp(c->sh, c->params, t, c->side); }),
                     "species clamp")))
@@ -986,15 +979,15 @@ int ck_restore(atu_ctx* c, int slot) {
 }
 
 // One step of the window in atu_step's order. With slot >= 0 the step's inputs and stage states go
-// to that tape slot. As in atu_step, the hydrostatic prep of the next step runs on the side stream
-// during this step's Poisson solve: `first` (the step after a state change) prepares its own on the
-// main stream, `last` launches none and leaves no side-stream work behind.
+// to that tape slot. As in atu_step, the BC and the hydrostatic prep of the next step run on the side
+// stream during this step's Poisson solve: `first` (the step after a state change) prepares its own on
+// the main stream, `last` launches no prep and leaves the main stream waiting for the side's BC.
 int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool last) {
     int e;
     if (slot >= 0 && ((e = dcopy(c, tape_field(c, slot, TP_X0), c->Y, 4)) ||
                       (e = dcopy(c, tape_field(c, slot, TP_P), c->PDYN, 1))))
         return e;
-    if (first) {   // else the previous step's side_hydro_after_bc has queued this step's prep
+    if (first) {   // else the previous step's side_bc_hydro has queued this step's prep
         if ((e = do_hydro(c, c->stream))) return e;
     } else if ((e = c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent"))) {
         return e;
@@ -1019,11 +1012,11 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
                 return e;
         }
     }
-    if ((e = do_bc(c))) return e;
-    if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_TNEXT), c->prog(c->Y, ATU_T), 1))) return e;
-    if (!last && (e = side_hydro_after_bc(c))) return e;
+    // the BC (and the tape of t after it) and the next step's prep beside the solve, as in atu_step
+    if ((e = side_bc_hydro(c, !last, slot >= 0 ? tape_field(c, slot, TP_TNEXT) : nullptr))) return e;
     if ((e = do_pressure(c, &o, nullptr))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_SRC), c->SRC, 1))) return e;
+    if (last && (e = c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent"))) return e;
     return ATU_OK;
 }
 
