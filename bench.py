@@ -337,6 +337,7 @@ def adjoint_bench(g, p, F, nsteps, K, tol, fwd_step_s, events=True):
             "backward_over_plain_steps": round(bwd / (nsteps * fwd_step_s), 3) if fwd_step_s else None,
             "adjoint_kernels_ms": round(prof_ms, 2) if prof_ms is not None else None,
             "fwd_plus_adjoint_steps_per_s": round(nsteps / (fwd + bwd), 3),
+            "fwd_plus_adjoint_cell_updates_per_s": round(nsteps / (fwd + bwd) * (g.im - 2) * (g.jm - 2) * (g.km - 1), 1),
             "grad_t0_norm": float((g_t * g_t).sum() ** 0.5), "grad_nue": g_nue,
             "checked_by": "tests/test_gpu_adjoint.py::test_adjoint_full_shell_tolerance_solves (this window, "
                           "checkpoint every 10, solves to 1e-8: central FD along 4 directions and in nue, bar 1e-4)",
@@ -674,6 +675,7 @@ def main():
         progress(f"mixed {sps:.1f} steps/s")
         achieved = sum(RK_BYTES_MIXED) / 4 * N_INT / (rk_ms / rk_n * 1e-3) / 1e9 if rk_ms > 0 else 0.0
         mixed = {"steps_per_s": round(sps, 3), "ms_per_step": round(1e3 * el / args.steps, 4),
+                 "cell_updates_per_s": round(sps * N_INT, 1),
                  "speedup_vs_fp64": round(sps / line["value"], 4) if line["value"] else None,
                  "rk_stage_GBps": round(achieved, 1), "rk_stage_frac": round(achieved / PEAK_HBM_GBS, 4),
                  "solve_iterations_last_step": st.sweeps_done, "solve_refinements_last_step": st.outer_iters,
