@@ -97,6 +97,39 @@ def test_converge_100_steps_match_oracle(oracle):
     assert st.rel_resid <= 1e-8 and rel <= 1e-8
 
 
+def test_profile_sample_brackets_every_nth_step(oracle):
+    """atu_profile_sample: with every = 3, only steps 0, 3, 6 of a 7-step call carry the stage and step
+    events (4 stage launches each); the state equals that of an unprofiled run."""
+    print("[case] profiler sample: every 3rd step of 7", flush=True)
+    g = A.default_grid(1)
+    g.im, g.jm, g.km = 12, 19, 37
+    p = A.default_params(g)
+    F = A.synthetic_state(g, p, SEED)
+    oracle.bc(g, p, F)
+    opts = dict(rel_tol=1e-8, max_sweeps=60, warm_start=2)
+    with A.UranusCore(g, p) as core:
+        core.set_state(F)
+        core.profile_begin("rk_stage,step", every=3)
+        core.step(7, "converge", **opts)
+        ms, n_rk = core.profile_query("rk_stage")
+        _, n_step = core.profile_query("step")
+        core.profile_end()
+        G = core.get_state()
+        core.set_state(F)
+        core.profile_begin("rk_stage", every=1)
+        core.step(2, "converge", **opts)
+        _, n_all = core.profile_query("rk_stage")
+        core.profile_end()
+    with A.UranusCore(g, p) as core:
+        core.set_state(F)
+        core.step(7, "converge", **opts)
+        H = core.get_state()
+    print(f"[profile] sampled stage launches {n_rk}, steps {n_step}, unsampled 2 steps {n_all}, {ms:.3f} ms",
+          flush=True)
+    assert (n_rk, n_step, n_all) == (12, 3, 8) and ms > 0.0
+    assert np.array_equal(G, H)
+
+
 def test_window_backward_refuses_changed_inputs(oracle):
     """A frozen input written between the forward and the backward (here nue) makes the backward
     refuse; rewinding still works, and a new forward re-arms the backward with a new generation."""
