@@ -747,9 +747,10 @@ __global__ void __launch_bounds__(256) k_zero(double* v, size_t n) {
     if (q < n) v[q] = 0.0;
 }
 
-__global__ void __launch_bounds__(256) k_extrapolate(double* z, const double* z1, const double* z2, size_t n) {
+__global__ void __launch_bounds__(256) k_extrapolate(double* z, const double* z1, const double* z2, const double* z3,
+                                                     size_t n) {
     const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) z[q] = z2 ? 2.0 * z1[q] - z2[q] : z1[q];
+    if (q < n) z[q] = z3 ? 3.0 * z1[q] - 3.0 * z2[q] + z3[q] : (z2 ? 2.0 * z1[q] - z2[q] : z1[q]);
 }
 
 inline dim3 cells_grid(const Shape& sh, int kext) { return dim3((kext + 255) / 256, sh.jm, sh.im); }
@@ -851,8 +852,9 @@ hipError_t launch_sum(const double* v, size_t n, double* part, int nparts, doubl
     return hipGetLastError();
 }
 
-hipError_t launch_extrapolate(double* z, const double* z1, const double* z2, size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z, z1, z2, n);
+hipError_t launch_extrapolate(double* z, const double* z1, const double* z2, const double* z3, size_t n,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_extrapolate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z, z1, z2, z3, n);
     return hipGetLastError();
 }
 

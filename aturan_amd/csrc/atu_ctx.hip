@@ -133,9 +133,9 @@ struct atu_ctx {
         double *LAUX = nullptr;                                                            // 3 fields
         double *LP = nullptr, *LPN = nullptr, *LPI = nullptr, *LPH = nullptr, *LNUE = nullptr, *Z = nullptr,
                *B = nullptr, *LSRC = nullptr, *GREF = nullptr;                               // 1 field each
-        // the transposed solves' last two solutions (the next reverse step starts from their
+        // the transposed solves' last three solutions (the next reverse step starts from their
         // extrapolation) and how many are held since the backward began
-        double *ZP1 = nullptr, *ZP2 = nullptr;
+        double *ZP1 = nullptr, *ZP2 = nullptr, *ZP3 = nullptr;
         int zheld = 0;
         double *TGT = nullptr, *ROWS = nullptr, *PART = nullptr, *SC = nullptr, *HPARTS = nullptr;
         double loss = 0.0;
@@ -210,7 +210,7 @@ atu_ctx::~atu_ctx() {
     if (pref_blob) hipFree(pref_blob);
     if (RHS3) hipFree(RHS3);
     for (double* p : {win.ckpt, win.tape, win.tape_level, win.LY, win.LYN, win.LS, win.LK, win.LK2, win.LAUX, win.LP,
-                      win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.ZP1, win.ZP2, win.B, win.LSRC, win.GREF, win.HPARTS, win.TGT, win.ROWS,
+                      win.LPN, win.LPI, win.LPH, win.LNUE, win.Z, win.ZP1, win.ZP2, win.ZP3, win.B, win.LSRC, win.GREF, win.HPARTS, win.TGT, win.ROWS,
                       win.PART, win.SC})
         if (p) hipFree(p);
     for (float* p : {Y32, SA32, SB32, FROZEN32, AUX32, E32, R32})
@@ -934,7 +934,7 @@ int window_alloc(atu_ctx* c, int K, int slots) {
     for (double** p : {&W.LY, &W.LYN, &W.LS, &W.LK, &W.LK2})
         if ((e = grab(p, 4 * fs))) return e;
     if ((e = grab(&W.LAUX, 3 * fs))) return e;
-    for (double** p : {&W.LP, &W.LPN, &W.LPI, &W.LPH, &W.LNUE, &W.Z, &W.ZP1, &W.ZP2, &W.B, &W.LSRC, &W.GREF})
+    for (double** p : {&W.LP, &W.LPN, &W.LPI, &W.LPH, &W.LNUE, &W.Z, &W.ZP1, &W.ZP2, &W.ZP3, &W.B, &W.LSRC, &W.GREF})
         if ((e = grab(p, fs))) return e;
     if ((e = grab(&W.TGT, c->sh.jm)) || (e = grab(&W.ROWS, (size_t)(c->sh.im - 2) * (c->sh.jm - 2))) ||
         (e = grab(&W.PART, kSumParts)) || (e = grab(&W.SC, 8)) || (e = grab(&W.HPARTS, 2 * kHydAdjParts * c->sh.im)))
@@ -1066,17 +1066,19 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
     }
     // Poisson solve: walls folded, the held mean split off, A^T mu = lambda by the forward solver on
     // Y^-1 lambda, the consistency shift and the source clamp, then div^T into lambda_aux. The solve
-    // starts from the extrapolation 2 z_(n+1) - z_(n+2) of the previous reverse steps' solutions (z_(n+1)
-    // after one, zero at the first): it converges to the same tolerance, and the start's constant
-    // component (the solve's null space, which the held mean keeps) is removed by the consistency
-    // shift's transpose, since mu = Y z and (1 - y 1^T / y.1) annihilates y.
+    // starts from the extrapolation 3 z_(n+1) - 3 z_(n+2) + z_(n+3) of the previous reverse steps'
+    // solutions (2 z_(n+1) - z_(n+2) after two, z_(n+1) after one, zero at the first): it converges
+    // to the same tolerance, and the start's constant component (the solve's null space, which the
+    // held mean keeps) is removed by the consistency shift's transpose, since mu = Y z and
+    // (1 - y 1^T / y.1) annihilates y.
     if (!rb && (e = H(c->timed(PC_ADJ, [&] {
              hipError_t r = launch_pwall_adjoint(sh, W.LP, W.LPI, s);
              if (r == hipSuccess) r = launch_dsum<double>(DSUM_P, sh, c->geo, W.LPI, nullptr, c->SROWS, W.SC + 0, s);
              if (r == hipSuccess) r = launch_adj_rhs(sh, W.LPI, W.SC + 0, c->dsol.ncell, W.B, s);
              if (r == hipSuccess)
                  r = W.zheld == 0 ? launch_zero(W.Z, fs, s)
-                                  : launch_extrapolate(W.Z, W.ZP1, W.zheld >= 2 ? W.ZP2 : nullptr, fs, s);
+                                  : launch_extrapolate(W.Z, W.ZP1, W.zheld >= 2 ? W.ZP2 : nullptr,
+                                                       W.zheld >= 3 ? W.ZP3 : nullptr, fs, s);
              return r;
          }), "adjoint solve prep")))
         return e;
@@ -1140,9 +1142,10 @@ int window_reverse_step(atu_ctx* c, const atu_solver_opts& o, int slot) {
     std::swap(W.LY, W.LYN);
     std::swap(W.LP, W.LPN);
     if (!rb) {   // this step's solution becomes z_(n+1) of the next one
+        std::swap(W.ZP3, W.ZP2);
         std::swap(W.ZP2, W.ZP1);
         std::swap(W.ZP1, W.Z);
-        W.zheld = std::min(W.zheld + 1, 2);
+        W.zheld = std::min(W.zheld + 1, 3);
     }
     return ATU_OK;
 }

@@ -302,8 +302,9 @@ hipError_t launch_loss(const Shape& sh, const double* w, const double* target, d
                        double* out, hipStream_t s);
 hipError_t launch_sum(const double* v, size_t n, double* part, int nparts, double* out, hipStream_t s);
 hipError_t launch_zero(double* v, size_t n, hipStream_t s);
-// z = 2 z1 - z2 over n values (z = z1 when z2 is null)
-hipError_t launch_extrapolate(double* z, const double* z1, const double* z2, size_t n, hipStream_t s);
+// z = 3 z1 - 3 z2 + z3 over n values (2 z1 - z2 when z3 is null, z1 when z2 is null too)
+hipError_t launch_extrapolate(double* z, const double* z1, const double* z2, const double* z3, size_t n,
+                              hipStream_t s);
 
 // Residual r = src - Lc p over the interior: writes per-block partial sums of r^2 and src^2 into
 // `partials` (2 * nblocks doubles) and the fixed-order total into out[0] (r^2), out[1] (src^2).
