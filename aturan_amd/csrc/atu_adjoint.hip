@@ -120,7 +120,9 @@ __global__ void __launch_bounds__(256) k_rhs_adjoint(AdjStageArgs a) {
     double aT = 0.0, aU = 0.0, aV = 0.0, aW = 0.0, aP = 0.0, aH = 0.0;
 
     // ---- the transposed difference stencils of the 6 neighbours (and of the cell itself) ----
-#pragma unroll 1
+    // unrolled, so that the neighbours' loads go out together (bit-identical: the accumulations keep
+    // their order)
+#pragma unroll
     for (int q = 0; q < 7; q++) {
         int axis, sgn;
         int ni = i, nj = j, nk = k;
