@@ -437,8 +437,11 @@ int check_finite_grid(const atu_grid* g) {
     return 1;
 }
 
-StageArgs<double> stage_args(atu_ctx* c, int stage) {
+// With `floor`, the last stage also takes the species clamp at the rows the BC does not touch, and
+// side_bc_hydro(shell) clamps the rest after the BC.
+StageArgs<double> stage_args(atu_ctx* c, int stage, bool floor = false) {
     StageArgs<double> a{};
+    a.sp_floor = (floor && stage == 3 && c->params.species_clamp) ? 1 : 0;
     a.sh = c->sh;
     a.geo = c->geo;
     a.c = c->consts;
@@ -461,8 +464,9 @@ StageArgs<double> stage_args(atu_ctx* c, int stage) {
 
 // Mixed path: FP32 stencil states Y32 -> SA32 -> SB32 -> SA32, FP64 y_n and accumulator; stage 3
 // writes y_{n+1} into Y and its FP32 mirror Y32.
-StageArgs<float> stage_args32(atu_ctx* c, int stage) {
+StageArgs<float> stage_args32(atu_ctx* c, int stage, bool floor = false) {
     StageArgs<float> a{};
+    a.sp_floor = (floor && stage == 3 && c->params.species_clamp) ? 1 : 0;
     a.sh = c->sh;
     a.geo = c->geo;
     a.c = c->consts;
@@ -489,9 +493,9 @@ BcTargets bc_targets(atu_ctx* c) {
                       : BcTargets{c->Y, c->SA, c->SB, nullptr, nullptr, nullptr, c->FROZEN, nullptr};
 }
 
-hipError_t launch_stage(atu_ctx* c, int s) {
-    return c->mixed() ? launch_rk_stage_mixed(s, stage_args32(c, s), c->stream)
-                      : launch_rk_stage_f64(s, stage_args(c, s), c->stream);
+hipError_t launch_stage(atu_ctx* c, int s, bool floor) {
+    return c->mixed() ? launch_rk_stage_mixed(s, stage_args32(c, s, floor), c->stream)
+                      : launch_rk_stage_f64(s, stage_args(c, s, floor), c->stream);
 }
 
 int do_hydro(atu_ctx* c, hipStream_t s) {
@@ -505,12 +509,13 @@ int do_hydro(atu_ctx* c, hipStream_t s) {
 }
 
 // The four stages. With `hydro_pending` this step's hydrostatic prep is still running on the side
-// stream (side_bc_hydro): the first stage, which reads p_hydro, waits for it.
-int do_rk4(atu_ctx* c, bool hydro_pending = false) {
+// stream (side_bc_hydro): the first stage, which reads p_hydro, waits for it. `floor`: the species
+// clamp split between the last stage and side_bc_hydro(shell) (a step's RK4, not atu_rk4's).
+int do_rk4(atu_ctx* c, bool hydro_pending = false, bool floor = false) {
     c->rhs_valid = c->keep_rhs && !c->mixed();
     if (hydro_pending && c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent")) return ATU_EHIP;
     for (int s = 0; s < 4; s++)
-        if (c->hip(c->timed(PC_RK, [&] { return launch_stage(c, s); }), "rk_stage")) return ATU_EHIP;
+        if (c->hip(c->timed(PC_RK, [&] { return launch_stage(c, s, floor); }), "rk_stage")) return ATU_EHIP;
     return ATU_OK;
 }
 
@@ -526,8 +531,10 @@ int do_bc(atu_ctx* c) {
 // so that its streaming pass meets the solve's latency-bound iterations rather than its
 // bandwidth-bound set-up. ev_hydro marks their end; the next step's first stage, or (for the last
 // step) the main stream after the solve, waits for it.
-// With `tnext`, t after the BC is also copied there (the window's tape) before the prep.
-int side_bc_hydro(atu_ctx* c, bool hydro, double* tnext = nullptr) {
+// With `tnext`, t after the BC is also copied there (the window's tape) before the prep. With `shell`
+// the last stage has clamped the species at the rows the BC does not touch, and the clamp here covers
+// the rest (the same values as the clamp of every cell).
+int side_bc_hydro(atu_ctx* c, bool hydro, double* tnext = nullptr, bool shell = false) {
     int e;
     const BcTargets t = bc_targets(c);
     if ((e = c->hip(hipEventRecord(c->ev_bc, c->stream), "hipEventRecord")) ||
@@ -536,7 +543,11 @@ int side_bc_hydro(atu_ctx* c, bool hydro, double* tnext = nullptr) {
         (tnext && (e = c->hip(hipMemcpyAsync(tnext, c->prog(c->Y, ATU_T), c->sh.fs * sizeof(double),
                                              hipMemcpyDeviceToDevice, c->side), "hipMemcpyAsync(D2D)"))) ||
         (hydro && (e = do_hydro(c, c->side))) ||
-        (e = c->hip(c->timed_on(PC_BC, c->side, [&] { return launch_species_clamp(c->sh, c->params, t, c->side); }),
+        (e = c->hip(c->timed_on(PC_BC, c->side,
+                                [&] {
+                                    return shell ? launch_species_clamp_shell(c->sh, c->params, t, c->side)
+                                                 : launch_species_clamp(c->sh, c->params, t, c->side);
+                                }),
                     "species clamp")))
         return e;
     return c->hip(hipEventRecord(c->ev_hydro, c->side), "hipEventRecord");
@@ -998,12 +1009,12 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
         return e;
     for (int s = 0; s < 4; s++) {
         if (!c->mixed()) {
-            const StageArgs<double> a = stage_args(c, s);
+            const StageArgs<double> a = stage_args(c, s, true);
             if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_f64(s, a, c->stream); }), "rk_stage")))
                 return e;
             if (slot >= 0 && s < 3 && (e = dcopy(c, tape_field(c, slot, TP_X1 + 4 * s), a.xnext, 4))) return e;
         } else {   // mixed: the FP32 stage states, taped widened to FP64
-            const StageArgs<float> a = stage_args32(c, s);
+            const StageArgs<float> a = stage_args32(c, s, true);
             if ((e = c->hip(c->timed(PC_RK, [&] { return launch_rk_stage_mixed(s, a, c->stream); }), "rk_stage")))
                 return e;
             if (slot >= 0 && s < 3 &&
@@ -1013,7 +1024,7 @@ int window_step(atu_ctx* c, const atu_solver_opts& o, int slot, bool first, bool
         }
     }
     // the BC (and the tape of t after it) and the next step's prep beside the solve, as in atu_step
-    if ((e = side_bc_hydro(c, !last, slot >= 0 ? tape_field(c, slot, TP_TNEXT) : nullptr))) return e;
+    if ((e = side_bc_hydro(c, !last, slot >= 0 ? tape_field(c, slot, TP_TNEXT) : nullptr, true))) return e;
     if ((e = do_pressure(c, &o, nullptr))) return e;
     if (slot >= 0 && (e = dcopy(c, tape_field(c, slot, TP_SRC), c->SRC, 1))) return e;
     if (last && (e = c->hip(hipStreamWaitEvent(c->stream, c->ev_hydro, 0), "hipStreamWaitEvent"))) return e;
@@ -1639,7 +1650,7 @@ int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_st
         // stage waits for them
         if (n == 0 && (e = do_hydro(ctx, ctx->stream))) return e;
         const bool more = n + 1 < nsteps;
-        if ((e = do_rk4(ctx, n > 0)) || (e = side_bc_hydro(ctx, more))) return e;
+        if ((e = do_rk4(ctx, n > 0, true)) || (e = side_bc_hydro(ctx, more, nullptr, true))) return e;
         if ((e = do_pressure(ctx, opts, (n == nsteps - 1) ? stats : nullptr))) return e;
         if (!more && (e = ctx->hip(hipStreamWaitEvent(ctx->stream, ctx->ev_hydro, 0), "hipStreamWaitEvent"))) return e;
         if (step_ev) {

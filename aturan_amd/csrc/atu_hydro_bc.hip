@@ -272,6 +272,29 @@ __global__ void __launch_bounds__(256) k_species_clamp(BcCtx b) {
     if (!(v.y >= 0.0)) bc_put(b, q + 1, 0.0);
 }
 
+// The clamp on the shell of rows the BC writes or reads: the planes i = 0, 1, 2, im-3, im-2, im-1 (every
+// j), then at the other levels the rows j = 0, 1, 2, jm-3, jm-2, jm-1; blockIdx.x = row of that list,
+// blockIdx.y = species, one thread per value of the row's storage (the padding holds zeros).
+__global__ void __launch_bounds__(256) k_species_clamp_shell(BcCtx b) {
+    const Shape& sh = b.sh;
+    const int r = blockIdx.x;
+    int i, j;
+    if (r < 6 * sh.jm) {
+        const int pi = r / sh.jm;
+        i = pi < 3 ? pi : sh.im - 6 + pi;
+        j = r - pi * sh.jm;
+    } else {
+        const int t = r - 6 * sh.jm, pj = t % 6;
+        i = 3 + t / 6;
+        j = pj < 3 ? pj : sh.jm - 6 + pj;
+    }
+    const size_t q0 = (size_t)(ATU_CH4 + blockIdx.y) * sh.fs + ((size_t)i * sh.jm + j) * sh.ldk;
+    for (int k = threadIdx.x; k < sh.ldk; k += blockDim.x) {
+        const double v = b.t.y[q0 + k];
+        if (!(v >= 0.0)) bc_put(b, q0 + k, 0.0);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_level(const Shape& sh, const DevGeo& g, const StepConsts& c, const double* t, const double* pstat,
@@ -347,6 +370,17 @@ hipError_t launch_species_clamp(const Shape& sh, const atu_params& p, const BcTa
     b.t = t;
     hipLaunchKernelGGL(k_species_clamp, dim3((unsigned)((sh.fs / 2 + 255) / 256), ATU_NH4SH - ATU_CH4 + 1), dim3(256),
                        0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_species_clamp_shell(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s) {
+    if (!p.species_clamp) return hipSuccess;
+    if (sh.im < 6 || sh.jm < 6) return hipErrorInvalidValue;
+    BcCtx b;
+    b.sh = sh;
+    b.t = t;
+    const int rows = 6 * sh.jm + 6 * (sh.im - 6);
+    hipLaunchKernelGGL(k_species_clamp_shell, dim3((unsigned)rows, ATU_NH4SH - ATU_CH4 + 1), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

@@ -44,6 +44,10 @@ struct StageArgs {
     // FP32 storage only: y_n rounded to FP32 (its mirror), from which stages 1-2 form the next
     // stage input (stage 0 reads it as its own stencil centre); the final update reads FP64 y_n
     const S* yn_lo;
+    // stage 3 only: clampNegativeSpecies on the species it writes at the rows the BC does not read
+    // (3 <= i <= im-4, 3 <= j <= jm-4), for a step whose BC is followed by the clamp of the other rows
+    // (launch_species_clamp_shell): the same values as the clamp of every cell after the BC
+    int sp_floor;
     Shape sh;
     DevGeo geo;
     StepConsts c;
@@ -87,6 +91,9 @@ struct BcTargets {
 hipError_t launch_bc(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s, bool clamp = true);
 // clampNegativeSpecies alone (launch_bc with clamp = false and this, in that order, is launch_bc)
 hipError_t launch_species_clamp(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s);
+// The same clamp on the rows the BC writes or reads only (i < 3 or i > im-4, or j < 3 or j > jm-4): the
+// rest of a step's species were clamped by its last stage (StageArgs::sp_floor).
+hipError_t launch_species_clamp_shell(const Shape& sh, const atu_params& p, const BcTargets& t, hipStream_t s);
 // The same passes on the prognostic fields [f0, f1) only, and with `rest` the frozen fields and the
 // field-specific forms (rigid lid, species floor, lid form of v and w, species clamp). The fields do
 // not interact in the BC, so a pass over [0, 1) followed by one over [1, 17) with `rest` is the BC.

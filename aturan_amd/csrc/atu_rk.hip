@@ -18,7 +18,7 @@
 namespace atu {
 namespace {
 
-enum { CL_NONE = 0, CL_T = 1, CL_TKE = 2, CL_DIS = 3 };
+enum { CL_NONE = 0, CL_T = 1, CL_TKE = 2, CL_DIS = 3, CL_SP = 4 };
 
 template <typename R>
 struct D7 {
@@ -60,9 +60,12 @@ template <typename T>
 __device__ __forceinline__ void nt_st(T* p, T v) { __builtin_nontemporal_store(v, p); }
 
 // Fold one tendency into the accumulator and form the next input (pass B) or the step (final). The
-// source `nb` supplies the accumulator's value and the store targets.
+// source `nb` supplies the accumulator's value and the store targets. A species (CL_SP) at the final
+// update with `floor` set takes clampNegativeSpecies' value (planet/BC_Uran.h:110-128): not >= 0, NaN
+// included, becomes 0.
 template <int STAGE, int CLAMP, typename S, typename NB>
-__device__ __forceinline__ void fold(const StageArgs<S>& a, NB& nb, int f, size_t c, double y0, double r) {
+__device__ __forceinline__ void fold(const StageArgs<S>& a, NB& nb, int f, size_t c, double y0, double r,
+                                     bool floor = false) {
     const size_t q = (size_t)f * a.sh.fs + c;
     if (STAGE < 3) {
         if (STAGE == 0)
@@ -83,6 +86,7 @@ __device__ __forceinline__ void fold(const StageArgs<S>& a, NB& nb, int f, size_
             y = y < 0.0 ? 0.0 : (y > a.c.tke_max ? a.c.tke_max : y);
         }
         if (CLAMP == CL_DIS) y = (y < a.c.dis_min) ? a.c.dis_min : y;
+        if (CLAMP == CL_SP && floor) y = (y >= 0.0) ? y : 0.0;
         nt_st(nb.out(a.yout + q), y);
         if (a.ymirror) nt_st(nb.out(a.ymirror + q), (S)y);
     }
@@ -279,7 +283,8 @@ __device__ __forceinline__ void tend_dyn(const StageArgs<S>& a, const CellGeo<R>
 }
 
 template <typename R, typename S, int STAGE, typename NB>
-__device__ __forceinline__ void tend_sca(const StageArgs<S>& a, const CellGeo<R>& g, size_t c, NB& nb, bool pole) {
+__device__ __forceinline__ void tend_sca(const StageArgs<S>& a, const CellGeo<R>& g, size_t c, NB& nb, bool pole,
+                                         bool floor) {
     const R inv_rm = g.inv_rm, inv_rm2 = g.inv_rm2, two_inv_rm = g.two_inv_rm;
     const MF<R> irs = g.irs, ir2s2 = g.ir2s2, cirs = g.cirs;
     const StepConsts& C = a.c;
@@ -305,7 +310,7 @@ __device__ __forceinline__ void tend_sca(const StageArgs<S>& a, const CellGeo<R>
         R rhs = -transport + diffusion / sc_re + diffusion * nue_t_s;
         if (mfl >= 0) rhs = rhs + cr * nb.fz(a, mfl, c);
         rhs = rhs + pc * nb.fz(a, sp, c);
-        fold<STAGE, CL_NONE>(a, nb, f, c, nb.template y0<STAGE>(a, f, c, X.c), (double)rhs);
+        fold<STAGE, CL_SP>(a, nb, f, c, nb.template y0<STAGE>(a, f, c, X.c), (double)rhs, floor);
         };
     nb.template item<6>();
     species(ATU_CH4, (R)C.sc_re_ch4, ATU_SP_CH4, -1);
@@ -340,7 +345,7 @@ __device__ __forceinline__ void tend_sca(const StageArgs<S>& a, const CellGeo<R>
         const R v_stokes = (R)C.stokes_c * ((R)C.rho_cond_nh4sh - nb.fz(a, ATU_RHO_MIX, c)) * (R)C.g / (R)C.mue_mix;
         const R rhs = -transport + nb.fz(a, ATU_FLUXLIM_NH4SH, c) + diffusion / (R)C.sc_re_nh4sh + diffusion * nue_t_s
                       + cr * nb.fz(a, ATU_MF_NH4SH, c) + (v_stokes / (R)C.u_0) * X.d1r;
-        fold<STAGE, CL_NONE>(a, nb, ATU_NH4SH, c, nb.template y0<STAGE>(a, ATU_NH4SH, c, X.c), (double)rhs);
+        fold<STAGE, CL_SP>(a, nb, ATU_NH4SH, c, nb.template y0<STAGE>(a, ATU_NH4SH, c, X.c), (double)rhs, floor);
     }
 
     // ---- k* and dis* transport (:989-1018); untouched with the closure off ----
@@ -637,6 +642,14 @@ __device__ __forceinline__ Inv<R> inv_of(const StageArgs<S>& a) {
     return Inv<R>{(R)G.inv_2dr, (R)G.inv_dr2, (R)G.inv_2dthe, (R)G.inv_dthe2, (R)G.inv_2dphi, (R)G.inv_dphi2};
 }
 
+// The rows at which the final update takes the species clamp (StageArgs::sp_floor): those the BC
+// neither writes nor reads (its NEUMANN forms read i = 1, 2, im-3, im-2 and j = 1, 2, jm-3, jm-2;
+// the seam copy takes k = 0 of the row, which this clamp has then already covered).
+template <int STAGE, typename S>
+__device__ __forceinline__ bool sp_floor_row(const StageArgs<S>& a, int i, int j) {
+    return STAGE == 3 && a.sp_floor && i >= 3 && i <= a.sh.im - 4 && j >= 3 && j <= a.sh.jm - 4;
+}
+
 // The FP64 stage with the ring source: a block is 64 k x 4 rows at one level (one wave per row), each
 // wave with its own ring (2 slots x 5 KiB), no barriers. Needs km-1 and ldk even (the halo and row
 // segments are loaded as 16-byte pairs); launch_rk_stage_f64 checks that.
@@ -675,9 +688,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))
     const size_t c = row + (valid ? k : 0);
     const bool pole = a.geo.pole_row[j] != 0.0;
     const CellGeo<double> g = cell_geo<double>(a, i, j);
+    const bool floor = sp_floor_row<STAGE>(a, i, j);
     nb.template issue<0>();
     tend_dyn<double, double, STAGE>(a, g, c, nb, pole);
-    tend_sca<double, double, STAGE>(a, g, c, nb, pole);
+    tend_sca<double, double, STAGE>(a, g, c, nb, pole, floor);
 }
 
 // The mixed (FP32 storage) stage with the direct source: a block is 64 k x 4 rows at one level, one
@@ -714,7 +728,7 @@ __global__ void __launch_bounds__(64 * TJ) k_rk_stage_noalias(const StageArgs<S>
     NbGlobal<R, S> nb{a.xs, a.pdyn, a.phydro, c, cw, ce, a.sh.plane, a.sh.fs, ldk, inv_of<R>(a)};
     const CellGeo<R> g = cell_geo<R>(a, i, j);
     tend_dyn<R, S, STAGE>(a, g, c, nb, pole);
-    tend_sca<R, S, STAGE>(a, g, c, nb, pole);
+    tend_sca<R, S, STAGE>(a, g, c, nb, pole, sp_floor_row<STAGE>(a, i, j));
 }
 
 constexpr int kTJ = 4;

@@ -57,9 +57,19 @@ def test_bc_species_clamp_matches_oracle(oracle, prec):
     print(f"[parity] bc + clamp {prec}: {bad} negative or NaN species values before, none after", flush=True)
 
 
+def shell_mask(g):
+    """The (i, j) rows the BC writes or reads (i < 3 or i > im-4, or j < 3 or j > jm-4)."""
+    far = np.zeros((g.im, g.jm), bool)
+    far[3:g.im - 3, 3:g.jm - 3] = True
+    return ~far
+
+
 def test_bc_species_clamp_then_step_matches_oracle(oracle):
     """The clamp's writes reach the stage buffers the next step's stencils read (a mixed context's FP32
-    mirrors, an FP64 context's stage states): one converging step after the clamp equals the oracle."""
+    mirrors, an FP64 context's stage states): one converging step after the clamp equals the oracle.
+    The step's own clamp is split (the last stage clamps the rows the BC does not touch, a pass after
+    the BC the shell of rows it does): the same step without the clamp leaves negative species in both
+    parts, and the split clamp gives the oracle's clamp-after-BC values bit for bit."""
     print("[case] bc + clamp, then one converging step", flush=True)
     g = A.default_grid(1)
     p = A.default_params(g)
@@ -70,10 +80,47 @@ def test_bc_species_clamp_then_step_matches_oracle(oracle):
         core.step(1, "converge", rel_tol=1e-8, max_sweeps=50)
         G = core.get_state()
     oracle.bc(g, p, F)
+    p0 = A.default_params(g)
+    p0.species_clamp = 0
+    H = F.copy()
+    oracle.step_converge(g, p0, H, 1, 50, 1e-8, 0)
+    shell = shell_mask(g)
+    neg_far = neg_shell = 0
+    for n in SPECIES:
+        bad = ~(H[A.FIELD[n]].reshape(g.im, g.jm, -1) >= 0.0)
+        neg_far += int(bad[~shell].sum())
+        neg_shell += int(bad[shell].sum())
+    print(f"[parity] unclamped step: {neg_far} negative species values at the rows the BC does not touch, "
+          f"{neg_shell} in the shell", flush=True)
+    assert neg_far > 0 and neg_shell > 0
     oracle.step_converge(g, p, F, 1, 50, 1e-8, 0)
     errs = {n: rel_err(G[A.FIELD[n]], F[A.FIELD[n]]) for n in PROG + ["p_dyn", "p_hydro"]}
     print(f"[parity] clamp then step: worst {max(errs.values()):.3e}", flush=True)
     assert max(errs.values()) == 0.0
+
+
+def test_mixed_step_species_clamp():
+    """The split clamp on a mixed context: after two steps from the clamped input no species value is
+    negative or NaN, and the FP32 mirror the next stencils read holds the clamped values (a third step
+    stays within 1e-5 of the FP64 context's)."""
+    print("[case] mixed context: bc + clamp, then steps with the split clamp", flush=True)
+    g = A.default_grid(1)
+    p = A.default_params(g)
+    F = MG.case_input("bc_clamp", g, p)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        with A.UranusCore(g, p, prec) as core:
+            core.set_state(F)
+            core.boundary_conditions()
+            core.step(2, "converge", rel_tol=1e-8, max_sweeps=50)
+            G2 = core.get_state()
+            core.step(1, "converge", rel_tol=1e-8, max_sweeps=50)
+            out[prec] = (G2, core.get_state())
+    for n in SPECIES:
+        assert np.all(out["mixed"][0][A.FIELD[n]] >= 0.0), n
+    errs = {n: rel_err(out["mixed"][1][A.FIELD[n]], out["fp64"][1][A.FIELD[n]]) for n in PROG}
+    print(f"[parity] mixed vs fp64 after 3 steps: worst {max(errs.values()):.3e}", flush=True)
+    assert max(errs.values()) <= 1e-5
 
 
 def test_converge_100_steps_match_oracle(oracle):
