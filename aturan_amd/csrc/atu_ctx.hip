@@ -666,13 +666,8 @@ int do_converge(atu_ctx* c, const atu_solver_opts* o, atu_solve_stats* st, doubl
     bool posted = false;
     auto post = [&](const int* skip) -> int {
         int r;
-        if ((r = H(c->timed(PC_LINE, [&] { return launch_walls<double>(sh, P, s, skip); }), "walls")) ||
-            (r = H(launch_p_corners<double>(sh, c->consts, P, s, skip), "corners")) ||
-            (r = H(launch_seam<double>(sh, P, s, skip), "seam")))
-            return r;
-        SolveCtl k;
-        k.skip = skip;
-        if ((r = H(launch_dsum<double>(DSUM_P, sh, c->geo, P, nullptr, c->SROWS, c->SCAL + 4, s, nullptr, k), "dsum")) ||
+        if ((r = H(c->timed(PC_LINE, [&] { return launch_post_dsum<double>(sh, P, c->SROWS, c->SCAL + 4, s, skip); }),
+                   "walls, corners, seam and interior sum")) ||
             (r = H(launch_shift_all<double>(sh, c->SCAL, 1, 4, c->dsol.ncell, P, s, skip), "shift")))
             return r;
         return ATU_OK;

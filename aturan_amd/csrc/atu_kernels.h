@@ -103,22 +103,29 @@ hipError_t launch_bc_part(const Shape& sh, const atu_params& p, const BcTargets&
 // The clamped divergence source of the Poisson solve at interior cell (i, j, k) (x its offset, nk
 // the ring of meridians): div(aux) with the central differences of the projection
 // (planet/Pressure_Uran.cpp:30-146), non-finite -> 0, clamped to +-den * p_dyn_cap.
-template <typename S>
-__device__ __forceinline__ double poisson_src_at(const Shape& sh, const DevGeo& g, const StepConsts& c, const S* aux,
-                                                 int i, int j, int k, size_t x) {
-    const int nk = sh.km - 1;
-    const size_t row = x - k;
-    const size_t xw = row + (k == 0 ? nk - 1 : k - 1), xe = row + (k == nk - 1 ? 0 : k + 1);
-    const size_t ij = (size_t)i * sh.jm + j;
-    const double du_dr = ((double)aux[x + sh.plane] - (double)aux[x - sh.plane]) * g.inv_2dr;
-    const double dv_dthe = ((double)aux[sh.fs + x + sh.ldk] - (double)aux[sh.fs + x - sh.ldk]) * g.inv_2dthe;
-    const double dw_dphi = ((double)aux[2 * sh.fs + xe] - (double)aux[2 * sh.fs + xw]) * g.inv_2dphi;
+// The source at one cell from its six aux neighbours (u at i+-1, v at j+-1, w at k+-1 across the
+// seam), formed as poisson_src_at forms it.
+__device__ __forceinline__ double poisson_src_of(const DevGeo& g, const StepConsts& c, int i, size_t ij, double up,
+                                                 double un, double vp, double vn, double we, double ww) {
+    const double du_dr = (up - un) * g.inv_2dr;
+    const double dv_dthe = (vp - vn) * g.inv_2dthe;
+    const double dw_dphi = (we - ww) * g.inv_2dphi;
     double div_src = du_dr * 1.0 + dv_dthe * g.inv_rm[i] + dw_dphi * g.inv_rmsinthe_s[ij];
     const double src_max = g.den[ij] * c.p_dyn_cap;
     if (!isfinite(div_src)) div_src = 0.0;
     else if (div_src > src_max) div_src = src_max;
     else if (div_src < -src_max) div_src = -src_max;
     return div_src;
+}
+template <typename S>
+__device__ __forceinline__ double poisson_src_at(const Shape& sh, const DevGeo& g, const StepConsts& c, const S* aux,
+                                                 int i, int j, int k, size_t x) {
+    const int nk = sh.km - 1;
+    const size_t row = x - k;
+    const size_t xw = row + (k == 0 ? nk - 1 : k - 1), xe = row + (k == nk - 1 ? 0 : k + 1);
+    return poisson_src_of(g, c, i, (size_t)i * sh.jm + j, (double)aux[x + sh.plane], (double)aux[x - sh.plane],
+                          (double)aux[sh.fs + x + sh.ldk], (double)aux[sh.fs + x - sh.ldk], (double)aux[2 * sh.fs + xe],
+                          (double)aux[2 * sh.fs + xw]);
 }
 
 // Projection prep on aux (prepareProjectionBoundaries) and the clamped divergence source.
@@ -192,6 +199,10 @@ hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, dou
                               const void* aux = nullptr, bool aux_f32 = false, const DevGeo* g = nullptr,
                               const StepConsts* c = nullptr);
 hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done = nullptr);
+// The end of a converging solve: launch_walls, launch_p_corners, launch_seam and
+// launch_dsum(DSUM_P) in one pass plus the total (the same values); `rows` as launch_dsum's.
+template <typename P>
+hipError_t launch_post_dsum(const Shape& sh, P* p, double* rows, double* out, hipStream_t s, const int* skip = nullptr);
 template <typename P>
 hipError_t launch_shift_all(const Shape& sh, const double* sc, int slot_before, int slot_now, double ncell, P* p,
                             hipStream_t s, const int* skip = nullptr);

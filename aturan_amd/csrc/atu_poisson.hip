@@ -19,10 +19,13 @@ namespace {
 // ---- prepareProjectionBoundaries on aux (rigid lid; rhs_* are not kept on this path) ----
 template <typename S>
 __global__ void __launch_bounds__(256) k_proj_walls(Shape sh, int rigid_lid, S* aux) {
-    // index set 1: radial planes, j = 1..jm-2, k = 0..km-2; index set 2: polar rows, i = 1..im-2
+    // index set 1: radial planes, j = 1..jm-2, k = 0..km-2; index set 2: polar rows, i = 1..im-2;
+    // index set 3: the seam copy (k = km-1 <- k = 0) of the rows neither wall set writes (the interior
+    // rows and the four corner rows); the wall sets write the seam copy of the values they form at k = 0
     const int nk = sh.km - 1;
     const int n1 = (sh.jm - 2) * nk;
     const int n2 = (sh.im - 2) * nk;
+    const int n3 = (sh.im - 2) * (sh.jm - 2) + 4;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const size_t fs = sh.fs, P = sh.plane;
     const int ldk = sh.ldk;
@@ -33,17 +36,30 @@ __global__ void __launch_bounds__(256) k_proj_walls(Shape sh, int rigid_lid, S* 
         S* au = aux;
         S* av = aux + fs;
         S* aw = aux + 2 * fs;
+        S ulo, uhi;
         if (rigid_lid) {
-            au[col] = S(0.0);
-            au[col + L * P] = S(0.0);
+            ulo = S(0.0);
+            uhi = S(0.0);
         } else {
-            au[col] = au[col + 3 * P] - S(3.0) * au[col + 2 * P] + S(3.0) * au[col + P];
-            au[col + L * P] = au[col + (L - 3) * P] - S(3.0) * au[col + (L - 2) * P] + S(3.0) * au[col + (L - 1) * P];
+            ulo = au[col + 3 * P] - S(3.0) * au[col + 2 * P] + S(3.0) * au[col + P];
+            uhi = au[col + (L - 3) * P] - S(3.0) * au[col + (L - 2) * P] + S(3.0) * au[col + (L - 1) * P];
+        }
+        au[col] = ulo;
+        au[col + L * P] = uhi;
+        if (k == 0) {
+            au[col + nk] = ulo;
+            au[col + L * P + nk] = uhi;
         }
         for (int f = 0; f < 2; f++) {
             S* a = (f == 0) ? av : aw;
-            a[col] = a[col + 3 * P] - S(3.0) * a[col + 2 * P] + S(3.0) * a[col + P];
-            a[col + L * P] = a[col + (L - 3) * P] - S(3.0) * a[col + (L - 2) * P] + S(3.0) * a[col + (L - 1) * P];
+            const S lo = a[col + 3 * P] - S(3.0) * a[col + 2 * P] + S(3.0) * a[col + P];
+            const S hi = a[col + (L - 3) * P] - S(3.0) * a[col + (L - 2) * P] + S(3.0) * a[col + (L - 1) * P];
+            a[col] = lo;
+            a[col + L * P] = hi;
+            if (k == 0) {
+                a[col + nk] = lo;
+                a[col + L * P + nk] = hi;
+            }
         }
     } else if (t < n1 + n2) {
         const int q = t - n1;
@@ -51,23 +67,39 @@ __global__ void __launch_bounds__(256) k_proj_walls(Shape sh, int rigid_lid, S* 
         const size_t b = (size_t)i * P + k;
         const int J = sh.jm - 1;
         S* au = aux;
-        au[b] = au[b + 3 * ldk] - S(3.0) * au[b + 2 * ldk] + S(3.0) * au[b + ldk];
-        au[b + (size_t)J * ldk] = au[b + (size_t)(J - 3) * ldk] - S(3.0) * au[b + (size_t)(J - 2) * ldk]
-                                  + S(3.0) * au[b + (size_t)(J - 1) * ldk];
+        const S lo = au[b + 3 * ldk] - S(3.0) * au[b + 2 * ldk] + S(3.0) * au[b + ldk];
+        const S hi = au[b + (size_t)(J - 3) * ldk] - S(3.0) * au[b + (size_t)(J - 2) * ldk]
+                     + S(3.0) * au[b + (size_t)(J - 1) * ldk];
+        au[b] = lo;
+        au[b + (size_t)J * ldk] = hi;
         aux[fs + b] = S(0.0);
         aux[2 * fs + b] = S(0.0);
         aux[fs + b + (size_t)J * ldk] = S(0.0);
         aux[2 * fs + b + (size_t)J * ldk] = S(0.0);
+        if (k == 0) {
+            au[b + nk] = lo;
+            au[b + (size_t)J * ldk + nk] = hi;
+            aux[fs + b + nk] = S(0.0);
+            aux[2 * fs + b + nk] = S(0.0);
+            aux[fs + b + (size_t)J * ldk + nk] = S(0.0);
+            aux[2 * fs + b + (size_t)J * ldk + nk] = S(0.0);
+        }
+    } else if (t < n1 + n2 + n3) {
+        const int q = t - n1 - n2, nin = (sh.im - 2) * (sh.jm - 2);
+        int i, j;
+        if (q < nin) {
+            i = 1 + q / (sh.jm - 2);
+            j = 1 + q % (sh.jm - 2);
+        } else {
+            i = (q - nin) & 1 ? sh.im - 1 : 0;
+            j = (q - nin) & 2 ? sh.jm - 1 : 0;
+        }
+        const size_t row = ((size_t)i * sh.jm + j) * ldk;
+        const S a0 = aux[row], a1 = aux[fs + row], a2 = aux[2 * fs + row];
+        aux[row + nk] = a0;
+        aux[fs + row + nk] = a1;
+        aux[2 * fs + row + nk] = a2;
     }
-}
-
-template <typename S>
-__global__ void __launch_bounds__(256) k_proj_seam(Shape sh, S* aux) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (j >= sh.jm) return;
-    const size_t row = ((size_t)i * sh.jm + j) * sh.ldk;
-    for (int f = 0; f < 3; f++) aux[f * sh.fs + row + sh.km - 1] = aux[f * sh.fs + row];
 }
 
 // ---- the clamped divergence source div(aux) (planet/PressureSolver.h:448-487) ----
@@ -239,9 +271,8 @@ template <typename S>
 hipError_t launch_proj_src(const Shape& sh, const DevGeo& g, const StepConsts& c, S* aux, double* src,
                            hipStream_t s, bool form_src) {
     const int nk = sh.km - 1;
-    const int n = (sh.jm - 2) * nk + (sh.im - 2) * nk;
+    const int n = (sh.jm - 2) * nk + (sh.im - 2) * nk + (sh.im - 2) * (sh.jm - 2) + 4;
     hipLaunchKernelGGL((k_proj_walls<S>), dim3((n + 255) / 256), dim3(256), 0, s, sh, 1, aux);
-    hipLaunchKernelGGL((k_proj_seam<S>), dim3((sh.jm + 255) / 256, sh.im), dim3(256), 0, s, sh, aux);
     if (form_src)
         hipLaunchKernelGGL((k_src<S>), dim3((nk + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0, s, sh, g,
                            c, (const S*)aux, src);

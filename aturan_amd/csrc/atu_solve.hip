@@ -461,58 +461,129 @@ __global__ void __launch_bounds__(256) k_add_q(Shape sh, const double* Q, P* p, 
 }
 
 // ---- canonical interior sums: per row, 64-wide trees, chunks, rows and levels in sequence ----
+// One wave per interior (i, j) row (kRowWaves rows per block, no barriers): the row's terms in
+// chunks of 64 k (lane = k mod 64, zero past km-2), each chunk reduced by the 64-wide halving tree,
+// the chunk sums added in order from 0. A wave forms kRowBatch chunks' terms before it reduces them,
+// so that their loads go out together. The row index r = (i-1)(jm-2) + (j-1).
+constexpr int kRowWaves = 4;
+constexpr int kRowBatch = 3;
+
+__device__ __forceinline__ double tree64(double t) {
+    for (int h = 32; h > 0; h >>= 1) t += __shfl_down(t, h, 64);
+    return t;   // the chunk's sum in lane 0
+}
+
+struct RowWave {
+    int r, i, j, lane;
+};
+__device__ __forceinline__ bool row_wave(const Shape& sh, RowWave& w) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    w.r = blockIdx.x * kRowWaves + wave;
+    if (w.r >= (sh.im - 2) * (sh.jm - 2)) return false;
+    w.i = 1 + w.r / (sh.jm - 2);
+    w.j = 1 + w.r % (sh.jm - 2);
+    w.lane = threadIdx.x & 63;
+    return true;
+}
+
 template <int MODE, typename P>
-__global__ void __launch_bounds__(1024) k_dsum_rows(Shape sh, DevGeo g, const P* p, const double* v, double* rows,
-                                                    float* r_out, const int* done) {
+__global__ void __launch_bounds__(64 * kRowWaves) k_dsum_rows(Shape sh, DevGeo g, const P* __restrict__ p,
+                                                               const double* __restrict__ v, double* __restrict__ rows,
+                                                               float* __restrict__ r_out, const int* done) {
     if (done && *done) return;
+    RowWave w;
+    if (!row_wave(sh, w)) return;
     const int N = sh.km - 1;
-    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
-    const int k = threadIdx.x;
-    double term = 0.0;
-    if (k < N) {
-        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-        if (MODE == DSUM_P) term = (double)p[x];
-        if (MODE == DSUM_YSRC) term = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * v[x];
-        if (MODE == DSUM_SQ) term = v[x] * v[x];
-        if (MODE == DSUM_RESID2) {
-            const double r = resid_at(sh, g, p, v, i, j, k);
-            if (r_out) r_out[x] = (float)r;
-            term = r * r;
+    const size_t row = ((size_t)w.i * sh.jm + w.j) * sh.ldk;
+    const double wy = endw(w.i, 1, sh.im - 2) * endw(w.j, 1, sh.jm - 2);
+    double sum = 0.0;
+    for (int c0 = 0; c0 < N; c0 += 64 * kRowBatch) {
+        double t[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++) {
+            const int k = c0 + 64 * u + w.lane;
+            t[u] = 0.0;
+            if (k < N) {
+                const size_t x = row + k;
+                if (MODE == DSUM_P) t[u] = (double)p[x];
+                if (MODE == DSUM_YSRC) t[u] = wy * v[x];
+                if (MODE == DSUM_SQ) t[u] = v[x] * v[x];
+                if (MODE == DSUM_RESID2) {
+                    const double r = resid_at(sh, g, p, v, w.i, w.j, k);
+                    if (r_out) r_out[x] = (float)r;
+                    t[u] = r * r;
+                }
+            }
         }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++)
+            if (c0 + 64 * u < N) sum += tree64(t[u]);
     }
-    for (int h = 32; h > 0; h >>= 1) term += __shfl_down(term, h, 64);
-    __shared__ double chunk[16];
-    if ((threadIdx.x & 63) == 0) chunk[threadIdx.x >> 6] = term;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double row = 0.0;
-        for (int c = 0; c < (int)(blockDim.x >> 6); c++) row += chunk[c];
-        rows[(size_t)(i - 1) * (sh.jm - 2) + (j - 1)] = row;
+    if (w.lane == 0) rows[w.r] = sum;
+}
+
+// The end of a converging solve in one pass, one wave per (i, j) row of the whole grid: the radial
+// and polar wall values of the final interior (as k_walls forms them), the corner rows from the radial
+// wall values they read (formed again here, the same bits), the seam copy k = km-1 <- k = 0 of every
+// row, and each interior row's sum of p in k_dsum_rows<DSUM_P>'s order. No row reads a value this pass
+// writes: the walls and corners read interior cells below km-1, the sums too, and the interior rows'
+// only store is their seam cell.
+template <typename P>
+__global__ void __launch_bounds__(64 * kRowWaves) k_post_rows(Shape sh, double c43, double c13, P* __restrict__ p,
+                                                               double* __restrict__ rows, const int* skip) {
+    if (skip && *skip) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int q = blockIdx.x * kRowWaves + wave;
+    if (q >= sh.im * sh.jm) return;
+    const int i = q / sh.jm, j = q % sh.jm, lane = threadIdx.x & 63;
+    const int N = sh.km - 1, L = sh.im - 1, J = sh.jm - 1, ldk = sh.ldk;
+    const size_t PL = sh.plane;
+    const size_t row = ((size_t)i * sh.jm + j) * ldk;
+    const bool ri = (i == 0 || i == L), rj = (j == 0 || j == J);
+    if (!ri && !rj) {
+        double sum = 0.0;
+        for (int c0 = 0; c0 < N; c0 += 64 * kRowBatch) {
+            double t[kRowBatch];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; u++) t[u] = (double)p[row + min(c0 + 64 * u + lane, N - 1)];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; u++)
+                if (c0 + 64 * u < N) sum += tree64(c0 + 64 * u + lane < N ? t[u] : 0.0);
+        }
+        if (lane == 0) {
+            p[row + N] = p[row];
+            rows[(size_t)(i - 1) * (sh.jm - 2) + (j - 1)] = sum;
+        }
+        return;
+    }
+    // radial (i) and polar (j) steps towards the interior
+    const ptrdiff_t di = i == 0 ? (ptrdiff_t)PL : -(ptrdiff_t)PL, dj = j == 0 ? (ptrdiff_t)ldk : -(ptrdiff_t)ldk;
+    for (int k = lane; k < N; k += 64) {
+        const size_t x = row + k;
+        P v;
+        if (ri && !rj) {
+            v = (P)(c43 * (double)p[x + di] - c13 * (double)p[x + 2 * di]);
+        } else if (rj && !ri) {
+            v = (P)(c43 * (double)p[x + dj] - c13 * (double)p[x + 2 * dj]);
+        } else {   // a corner row: the polar form over the radial walls at j +- 1, j +- 2
+            const P w1 = (P)(c43 * (double)p[x + dj + di] - c13 * (double)p[x + dj + 2 * di]);
+            const P w2 = (P)(c43 * (double)p[x + 2 * dj + di] - c13 * (double)p[x + 2 * dj + 2 * di]);
+            v = (P)(c43 * (double)w1 - c13 * (double)w2);
+        }
+        p[x] = v;
+        if (k == 0) p[row + N] = v;
     }
 }
 
-// The row reduction of k_dsum_rows for a block of one row (64-wide trees, chunks in sequence).
-__device__ __forceinline__ void row_sum_store(double term, double* chunk, double* out) {
-    for (int h = 32; h > 0; h >>= 1) term += __shfl_down(term, h, 64);
-    if ((threadIdx.x & 63) == 0) chunk[threadIdx.x >> 6] = term;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double row = 0.0;
-        for (int c = 0; c < (int)(blockDim.x >> 6); c++) row += chunk[c];
-        *out = row;
-    }
-}
-
-// The p part of the solve's first pass at one cell: the value whose interior sum the solve holds,
-// and with pprev the warm start. With pprev2 (quadratic start) the history rotates by buffer: p goes
+// The p part of the solve's first pass at one cell, from p, pprev and pprev2 as read there (the row
+// passes below load them ahead): p is the value whose interior sum the solve holds; with pprev, the
+// warm start. With pprev2 (quadratic start) the history rotates by buffer: p goes
 // into pprev2's buffer (its p_(n-2) read first) and the caller swaps the two pointers, so pprev
 // holds p_n and pprev2 p_(n-1) after the pass.
-__device__ __forceinline__ double setup_p(double* p, double* pprev, double* pprev2, int extrap, size_t x) {
-    const double tp = p[x];
+__device__ __forceinline__ void setup_p_store(double* p, double* pprev, double* pprev2, int extrap, size_t x, double tp,
+                                              double p1, double p2) {
     if (pprev) {
-        const double p1 = pprev[x];
         if (pprev2) {
-            const double p2 = pprev2[x];
             if (extrap == 2) p[x] = 3.0 * tp - 3.0 * p1 + p2;
             else if (extrap) p[x] = 2.0 * tp - p1;
             pprev2[x] = tp;
@@ -521,67 +592,153 @@ __device__ __forceinline__ double setup_p(double* p, double* pprev, double* ppre
             pprev[x] = tp;
         }
     }
-    return tp;
 }
 
 // The two sums the solve starts with, in one pass: y . src and the interior sum of p, with the warm
-// start of setup_p.
-__global__ void __launch_bounds__(1024) k_dsum_rows_ysrc_p(Shape sh, double* p, const double* src, double* rows_y,
-                                                           double* rows_p, double* pprev, double* pprev2, int extrap) {
+// start of setup_p_store. Each batch of chunks loads every value first (lanes past km-2 load the row's last
+// cell and discard it), then forms and stores.
+__global__ void __launch_bounds__(64 * kRowWaves) k_dsum_rows_ysrc_p(Shape sh, double* __restrict__ p,
+                                                                      const double* __restrict__ src,
+                                                                      double* __restrict__ rows_y,
+                                                                      double* __restrict__ rows_p,
+                                                                      double* __restrict__ pprev,
+                                                                      double* __restrict__ pprev2, int extrap) {
+    RowWave w;
+    if (!row_wave(sh, w)) return;
     const int N = sh.km - 1;
-    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
-    const int k = threadIdx.x;
-    double ty = 0.0, tp = 0.0;
-    if (k < N) {
-        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-        ty = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * src[x];
-        tp = setup_p(p, pprev, pprev2, extrap, x);
+    const size_t row = ((size_t)w.i * sh.jm + w.j) * sh.ldk;
+    const double wy = endw(w.i, 1, sh.im - 2) * endw(w.j, 1, sh.jm - 2);
+    const double* q1 = pprev ? pprev : p;
+    const double* q2 = pprev2 ? pprev2 : p;
+    double sy = 0.0, sp = 0.0;
+    for (int c0 = 0; c0 < N; c0 += 64 * kRowBatch) {
+        double sv[kRowBatch], tp[kRowBatch], p1[kRowBatch], p2[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++) {
+            const size_t x = row + min(c0 + 64 * u + w.lane, N - 1);
+            sv[u] = src[x];
+            tp[u] = p[x];
+            p1[u] = q1[x];
+            p2[u] = q2[x];
+        }
+        double ty[kRowBatch], tq[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++) {
+            const int k = c0 + 64 * u + w.lane;
+            ty[u] = 0.0;
+            tq[u] = 0.0;
+            if (k < N) {
+                ty[u] = wy * sv[u];
+                tq[u] = tp[u];
+                setup_p_store(p, pprev, pprev2, extrap, row + k, tp[u], p1[u], p2[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++)
+            if (c0 + 64 * u < N) {
+                sy += tree64(ty[u]);
+                sp += tree64(tq[u]);
+            }
     }
-    __shared__ double chunk_y[16], chunk_p[16];
-    const size_t r = (size_t)(i - 1) * (sh.jm - 2) + (j - 1);
-    row_sum_store(ty, chunk_y, rows_y + r);
-    row_sum_store(tp, chunk_p, rows_p + r);
+    if (w.lane == 0) {
+        rows_y[w.r] = sy;
+        rows_p[w.r] = sp;
+    }
 }
 
 // The same pass forming the source from aux first (the projection's clamped divergence, written to
 // src): k_src and k_dsum_rows_ysrc_p in one launch, the same values.
 template <typename S>
-__global__ void __launch_bounds__(1024) k_src_dsum_rows(Shape sh, DevGeo g, StepConsts c, const S* aux, double* src,
-                                                        double* p, double* rows_y, double* rows_p, double* pprev,
-                                                        double* pprev2, int extrap) {
+__global__ void __launch_bounds__(64 * kRowWaves) k_src_dsum_rows(Shape sh, DevGeo g, StepConsts c,
+                                                                   const S* __restrict__ aux, double* __restrict__ src,
+                                                                   double* __restrict__ p, double* __restrict__ rows_y,
+                                                                   double* __restrict__ rows_p,
+                                                                   double* __restrict__ pprev,
+                                                                   double* __restrict__ pprev2, int extrap) {
+    RowWave w;
+    if (!row_wave(sh, w)) return;
     const int N = sh.km - 1;
-    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
-    const int k = threadIdx.x;
-    double ty = 0.0, tp = 0.0;
-    if (k < N) {
-        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-        const double sv = poisson_src_at<S>(sh, g, c, aux, i, j, k, x);
-        src[x] = sv;
-        ty = (endw(i, 1, sh.im - 2) * endw(j, 1, sh.jm - 2)) * sv;
-        tp = setup_p(p, pprev, pprev2, extrap, x);
+    const size_t row = ((size_t)w.i * sh.jm + w.j) * sh.ldk;
+    const size_t ij = (size_t)w.i * sh.jm + w.j;
+    const double wy = endw(w.i, 1, sh.im - 2) * endw(w.j, 1, sh.jm - 2);
+    const double* q1 = pprev ? pprev : p;
+    const double* q2 = pprev2 ? pprev2 : p;
+    const S* au = aux;
+    const S* av = aux + sh.fs;
+    const S* aw = aux + 2 * sh.fs;
+    double sy = 0.0, sp = 0.0;
+    for (int c0 = 0; c0 < N; c0 += 64 * kRowBatch) {
+        double up[kRowBatch], un[kRowBatch], vp[kRowBatch], vn[kRowBatch], we[kRowBatch], ww[kRowBatch];
+        double tp[kRowBatch], p1[kRowBatch], p2[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++) {
+            const int k = min(c0 + 64 * u + w.lane, N - 1);
+            const size_t x = row + k;
+            up[u] = (double)au[x + sh.plane];
+            un[u] = (double)au[x - sh.plane];
+            vp[u] = (double)av[x + sh.ldk];
+            vn[u] = (double)av[x - sh.ldk];
+            we[u] = (double)aw[row + (k == N - 1 ? 0 : k + 1)];
+            ww[u] = (double)aw[row + (k == 0 ? N - 1 : k - 1)];
+            tp[u] = p[x];
+            p1[u] = q1[x];
+            p2[u] = q2[x];
+        }
+        double ty[kRowBatch], tq[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++) {
+            const int k = c0 + 64 * u + w.lane;
+            ty[u] = 0.0;
+            tq[u] = 0.0;
+            if (k < N) {
+                const size_t x = row + k;
+                const double sv = poisson_src_of(g, c, w.i, ij, up[u], un[u], vp[u], vn[u], we[u], ww[u]);
+                src[x] = sv;
+                ty[u] = wy * sv;
+                tq[u] = tp[u];
+                setup_p_store(p, pprev, pprev2, extrap, x, tp[u], p1[u], p2[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++)
+            if (c0 + 64 * u < N) {
+                sy += tree64(ty[u]);
+                sp += tree64(tq[u]);
+            }
     }
-    __shared__ double chunk_y[16], chunk_p[16];
-    const size_t r = (size_t)(i - 1) * (sh.jm - 2) + (j - 1);
-    row_sum_store(ty, chunk_y, rows_y + r);
-    row_sum_store(tp, chunk_p, rows_p + r);
+    if (w.lane == 0) {
+        rows_y[w.r] = sy;
+        rows_p[w.r] = sp;
+    }
 }
 
 // The consistent source b = src - (y . src) / (y . 1) and the rows of |b|^2, in one pass.
-__global__ void __launch_bounds__(1024) k_make_b_sq_rows(Shape sh, const double* src, const double* sc, double yw,
-                                                         double* b, double* rows) {
+__global__ void __launch_bounds__(64 * kRowWaves) k_make_b_sq_rows(Shape sh, const double* __restrict__ src,
+                                                                    const double* __restrict__ sc, double yw,
+                                                                    double* __restrict__ b, double* __restrict__ rows) {
+    RowWave w;
+    if (!row_wave(sh, w)) return;
     const int N = sh.km - 1;
-    const int j = 1 + blockIdx.x, i = 1 + blockIdx.y;
-    const int k = threadIdx.x;
-    double term = 0.0;
-    if (k < N) {
-        const double shift = sc[0] / yw;
-        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-        const double bv = src[x] - shift;
-        b[x] = bv;
-        term = bv * bv;
+    const size_t row = ((size_t)w.i * sh.jm + w.j) * sh.ldk;
+    const double shift = sc[0] / yw;
+    double sum = 0.0;
+    for (int c0 = 0; c0 < N; c0 += 64 * kRowBatch) {
+        double t[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++) {
+            const int k = c0 + 64 * u + w.lane;
+            t[u] = 0.0;
+            if (k < N) {
+                const double bv = src[row + k] - shift;
+                b[row + k] = bv;
+                t[u] = bv * bv;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; u++)
+            if (c0 + 64 * u < N) sum += tree64(t[u]);
     }
-    __shared__ double chunk[16];
-    row_sum_store(term, chunk, rows + (size_t)(i - 1) * (sh.jm - 2) + (j - 1));
+    if (w.lane == 0) rows[w.r] = sum;
 }
 
 // The total of the row sums: partial t (one thread of 256) adds rows t, t + 256, ... in sequence,
@@ -750,25 +907,28 @@ hipError_t launch_baro(const Shape& sh, const DevGeo& g, const DevSolve& t, P* p
     return hipGetLastError();
 }
 
+// blocks of kRowWaves rows over the interior rows
+inline unsigned row_blocks(const Shape& sh) {
+    return (unsigned)(((sh.im - 2) * (sh.jm - 2) + kRowWaves - 1) / kRowWaves);
+}
+
 template <typename P>
 hipError_t launch_dsum(int mode, const Shape& sh, const DevGeo& g, const P* p, const double* v, double* rows,
                        double* out, hipStream_t s, float* r_out, const SolveCtl& ctl) {
     const int* done = ctl.skip ? ctl.skip : (ctl.flags ? ctl.flags + CTL_DONE : nullptr);
-    const int N = sh.km - 1;
-    const int threads = ((N + 63) / 64) * 64;
-    const dim3 grid(sh.jm - 2, sh.im - 2);
+    const dim3 grid(row_blocks(sh)), threads(64 * kRowWaves);
     switch (mode) {
         case DSUM_P:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_P, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr, done);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_P, P>), grid, threads, 0, s, sh, g, p, v, rows, nullptr, done);
             break;
         case DSUM_YSRC:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_YSRC, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr, done);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_YSRC, P>), grid, threads, 0, s, sh, g, p, v, rows, nullptr, done);
             break;
         case DSUM_SQ:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_SQ, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, nullptr, done);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_SQ, P>), grid, threads, 0, s, sh, g, p, v, rows, nullptr, done);
             break;
         default:
-            hipLaunchKernelGGL((k_dsum_rows<DSUM_RESID2, P>), grid, dim3(threads), 0, s, sh, g, p, v, rows, r_out, done);
+            hipLaunchKernelGGL((k_dsum_rows<DSUM_RESID2, P>), grid, threads, 0, s, sh, g, p, v, rows, r_out, done);
             break;
     }
     hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(kDsumParts), 0, s, (sh.im - 2) * (sh.jm - 2), (const double*)rows,
@@ -780,22 +940,19 @@ hipError_t launch_solve_setup(const Shape& sh, double* p, const double* src, dou
                               double* scal, double yw, double* b, int* flags, double* pprev, double* pprev2,
                               int extrap, hipStream_t s, const void* aux, bool aux_f32, const DevGeo* g,
                               const StepConsts* c) {
-    const int N = sh.km - 1;
-    const int threads = ((N + 63) / 64) * 64;
-    if (threads > 1024) return hipErrorInvalidValue;
-    const dim3 grid(sh.jm - 2, sh.im - 2);
+    const dim3 grid(row_blocks(sh)), threads(64 * kRowWaves);
     const int nrows = (sh.im - 2) * (sh.jm - 2);
     if (!aux)
-        hipLaunchKernelGGL(k_dsum_rows_ysrc_p, grid, dim3(threads), 0, s, sh, p, src, rows, rows2, pprev, pprev2, extrap);
+        hipLaunchKernelGGL(k_dsum_rows_ysrc_p, grid, threads, 0, s, sh, p, src, rows, rows2, pprev, pprev2, extrap);
     else if (aux_f32)
-        hipLaunchKernelGGL((k_src_dsum_rows<float>), grid, dim3(threads), 0, s, sh, *g, *c, (const float*)aux,
+        hipLaunchKernelGGL((k_src_dsum_rows<float>), grid, threads, 0, s, sh, *g, *c, (const float*)aux,
                            (double*)src, p, rows, rows2, pprev, pprev2, extrap);
     else
-        hipLaunchKernelGGL((k_src_dsum_rows<double>), grid, dim3(threads), 0, s, sh, *g, *c, (const double*)aux,
+        hipLaunchKernelGGL((k_src_dsum_rows<double>), grid, threads, 0, s, sh, *g, *c, (const double*)aux,
                            (double*)src, p, rows, rows2, pprev, pprev2, extrap);
     hipLaunchKernelGGL(k_dsum_final2, dim3(1), dim3(kDsumParts), 0, s, nrows, (const double*)rows,
                        (const double*)rows2, scal + 0, scal + 1, flags);
-    hipLaunchKernelGGL(k_make_b_sq_rows, grid, dim3(threads), 0, s, sh, src, (const double*)scal, yw, b, rows);
+    hipLaunchKernelGGL(k_make_b_sq_rows, grid, threads, 0, s, sh, src, (const double*)scal, yw, b, rows);
     hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(kDsumParts), 0, s, nrows, (const double*)rows, scal + 2, SolveCtl());
     return hipGetLastError();
 }
@@ -809,6 +966,17 @@ hipError_t launch_make_b(const Shape& sh, const double* src, const double* sc, d
 hipError_t launch_add_correction(const Shape& sh, const float* e, double* p, hipStream_t s, const int* done) {
     hipLaunchKernelGGL(k_add_correction, dim3((sh.km - 1 + 63) / 64, (sh.jm - 2 + 3) / 4, sh.im - 2), dim3(64, 4), 0,
                        s, sh, e, p, done);
+    return hipGetLastError();
+}
+
+template <typename P>
+hipError_t launch_post_dsum(const Shape& sh, P* p, double* rows, double* out, hipStream_t s, const int* skip) {
+    const unsigned nb = (unsigned)((sh.im * sh.jm + kRowWaves - 1) / kRowWaves);
+    hipLaunchKernelGGL((k_post_rows<P>), dim3(nb), dim3(64 * kRowWaves), 0, s, sh, 4.0 / 3.0, 1.0 / 3.0, p, rows, skip);
+    SolveCtl k;
+    k.skip = skip;
+    hipLaunchKernelGGL(k_dsum_final, dim3(1), dim3(kDsumParts), 0, s, (sh.im - 2) * (sh.jm - 2), (const double*)rows,
+                       out, k);
     return hipGetLastError();
 }
 
@@ -828,6 +996,7 @@ template hipError_t launch_baro<double, double>(const Shape&, const DevGeo&, con
                                                 double*, double*, double*, hipStream_t, const int*);
 template hipError_t launch_baro<float, float>(const Shape&, const DevGeo&, const DevSolve&, float*, const float*,
                                               double*, double*, double*, hipStream_t, const int*);
+template hipError_t launch_post_dsum<double>(const Shape&, double*, double*, double*, hipStream_t, const int*);
 template hipError_t launch_walls<double>(const Shape&, double*, hipStream_t, const int*);
 template hipError_t launch_walls<float>(const Shape&, float*, hipStream_t, const int*);
 template hipError_t launch_seam<double>(const Shape&, double*, hipStream_t, const int*);
