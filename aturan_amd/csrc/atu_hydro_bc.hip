@@ -76,43 +76,49 @@ __global__ void __launch_bounds__(64) k_level_final(Shape sh, const double* rows
 // ---- the column scan: p_hydro[0] = 0, p_hydro[i] = p_hydro[i-1] + 0.5 (f(i-1) + f(i)) dr with
 // f = buoyancy anomaly relative to the level mean (BUOY_RHO_LEVEL) + (v^2 + w^2)/rm + Coriolis
 // (METRIC_SPLIT + CORIOLIS_SPLIT branch, :104-161). One thread per (j, k) column.
-__global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepConsts c, const double* t,
-                                                    const double* v, const double* w, const double* pstat,
-                                                    const double* level, double* phydro) {
+__global__ void __launch_bounds__(256) k_hydro_scan(Shape sh, DevGeo g, StepConsts c, const double* __restrict__ t,
+                                                    const double* __restrict__ v, const double* __restrict__ w,
+                                                    const double* __restrict__ pstat, const double* __restrict__ level,
+                                                    double* __restrict__ phydro) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     if (k >= sh.km) return;
     const double cor = g.cor_hydro[j];
-    auto f_at = [&](int i) -> double {
-        const size_t x = ((size_t)i * sh.jm + j) * sh.ldk + k;
-        const double T = t[x];
+    auto f_of = [&](int i, double T, double ps, double vv, double ww) -> double {
         double buoy = 0.0;
         if (T > 0.0) {
             const double ref = level[i];
-            const double b = c.g * pstat[x] / (c.hyd_bnum * T * c.t_ref) - ref;
+            const double b = c.g * ps / (c.hyd_bnum * T * c.t_ref) - ref;
             const double f = -c.nd_rel * 1.0 * 1.0 * b / ref;
             buoy = isfinite(f) ? f : 0.0;
         }
-        const double vv = v[x], ww = w[x];
         return buoy + (vv * vv + ww * ww) / g.rm[i] + cor * ww;
     };
-    // the f of kScanBatch levels at a time: their loads are independent of the running sum, so
-    // they are all in flight before the sum's dependent adds consume them
+    // kScanBatch levels at a time: every value they read is loaded first (pstat too, whatever the sign
+    // of t), so that the loads are all in flight before the sum's dependent adds consume them
     constexpr int kScanBatch = 8;
+    const size_t col = (size_t)j * sh.ldk + k;
     double p = 0.0;
-    double f_prev = f_at(0);
-    phydro[(size_t)j * sh.ldk + k] = 0.0;
+    double f_prev = f_of(0, t[col], pstat[col], v[col], w[col]);
+    phydro[col] = 0.0;
     for (int i0 = 1; i0 < sh.im; i0 += kScanBatch) {
-        double f[kScanBatch];
+        double T[kScanBatch], ps[kScanBatch], vv[kScanBatch], ww[kScanBatch];
 #pragma unroll
-        for (int u = 0; u < kScanBatch; u++) f[u] = (i0 + u < sh.im) ? f_at(i0 + u) : 0.0;
+        for (int u = 0; u < kScanBatch; u++) {
+            const size_t x = (size_t)min(i0 + u, sh.im - 1) * sh.plane + col;
+            T[u] = t[x];
+            ps[u] = pstat[x];
+            vv[u] = v[x];
+            ww[u] = w[x];
+        }
 #pragma unroll
         for (int u = 0; u < kScanBatch; u++) {
             const int i = i0 + u;
             if (i >= sh.im) break;
-            p = p + 0.5 * (f_prev + f[u]) * c.dr;
-            phydro[((size_t)i * sh.jm + j) * sh.ldk + k] = p;
-            f_prev = f[u];
+            const double f = f_of(i, T[u], ps[u], vv[u], ww[u]);
+            p = p + 0.5 * (f_prev + f) * c.dr;
+            phydro[(size_t)i * sh.plane + col] = p;
+            f_prev = f;
         }
     }
 }
