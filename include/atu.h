@@ -266,7 +266,8 @@ int atu_pressure_solve(atu_ctx* ctx, const atu_solver_opts* opts, atu_solve_stat
  * default; the stage kernel then does not write them. */
 int atu_keep_stage_rhs(atu_ctx* ctx, int on);
 
-/* One outer step as the benchmark loop orders it: hydro prep, RK4, BC, Poisson, `nsteps` times.
+/* One outer step as the benchmark loop orders it: hydro prep, RK4, BC (with clampNegativeSpecies
+ * when params.species_clamp is set: the same values as atu_apply_bc's), Poisson, `nsteps` times.
  * `stats` (optional) receives the last solve's statistics. */
 int atu_step(atu_ctx* ctx, int nsteps, const atu_solver_opts* opts, atu_solve_stats* stats);
 
